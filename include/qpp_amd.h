@@ -1,0 +1,281 @@
+/*
+ * qpp_amd.h -- C ABI of the MI355X (gfx950) gate-application engine.
+ *
+ * The engine replaces the hot path of Quantum++ 7.0.3 (qpp): qpp::apply /
+ * applyCTRL / applyCTRL_fan on kets and density matrices, apply(rho, Kraus),
+ * ptrace, ptranspose, syspermute, sample / measure_seq and the QEngine /
+ * QNoisyEngine execute loop.  Every entry point below names the qpp function
+ * it stands in for (paths relative to the qpp source tree).
+ *
+ * Conventions (identical to qpp):
+ *   - subsystem 0 is the most significant digit of a basis index
+ *     (include/qpp/internal/util.hpp:58-116);
+ *   - a gate acting on target[0..k-1] has target[0] as the most significant
+ *     digit of its row/column index (internal/kernels/qubit/apply.hpp:527-534);
+ *   - matrices, kets and density matrices are column-major arrays of
+ *     interleaved (re, im) doubles, exactly Eigen's storage of qpp::cmat /
+ *     qpp::ket, so a caller passes `U.data()` unchanged; rho(i, j) lives at
+ *     element i + j * D;
+ *   - index vectors (target, ctrl, shift, perm, dims) are uint64_t arrays,
+ *     i.e. std::vector<qpp::idx>::data().
+ *
+ * States live in device memory behind an opaque handle.  Operations on a
+ * handle are in place and ordered on the handle's HIP stream; the qpp-style
+ * value-semantics wrappers (copy in, compute, copy out) sit on top in
+ * qpp_amd.hpp.  Every call validates its arguments on the host, in the order
+ * qpp does, before anything is launched, and returns a qs_status whose
+ * non-zero values correspond one-to-one to qpp::exception classes
+ * (include/qpp/classes/exception.hpp:129-667).  qs_last_error() returns the
+ * message qpp's what() would print ("[WHERE: qpp::apply()] ...!").
+ *
+ * Nothing here allocates on a hidden RNG: every random draw is a uniform
+ * double in [0, 1) supplied by the caller, or drawn from a std::mt19937 the
+ * caller seeds (the generator qpp's RandomDevices holds,
+ * include/qpp/classes/random_devices.hpp:55-91).
+ */
+#ifndef QPP_AMD_H_
+#define QPP_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes: one per qpp::exception class used on the path.              */
+/* ------------------------------------------------------------------------ */
+typedef enum qs_status {
+    QS_OK = 0,
+    QS_ERR_ZERO_SIZE = 1,                /* exception::ZeroSize */
+    QS_ERR_MATRIX_NOT_SQUARE = 2,        /* exception::MatrixNotSquare */
+    QS_ERR_MATRIX_NOT_CVECTOR = 3,       /* exception::MatrixNotCvector */
+    QS_ERR_MATRIX_NOT_SQUARE_NOR_CVECTOR = 4, /* MatrixNotSquareNorCvector */
+    QS_ERR_MATRIX_MISMATCH_SUBSYS = 5,   /* exception::MatrixMismatchSubsys */
+    QS_ERR_DIMS_INVALID = 6,             /* exception::DimsInvalid */
+    QS_ERR_DIMS_NOT_EQUAL = 7,           /* exception::DimsNotEqual */
+    QS_ERR_DIMS_MISMATCH_MATRIX = 8,     /* exception::DimsMismatchMatrix */
+    QS_ERR_DIMS_MISMATCH_CVECTOR = 9,    /* exception::DimsMismatchCvector */
+    QS_ERR_SUBSYS_MISMATCH_DIMS = 10,    /* exception::SubsysMismatchDims */
+    QS_ERR_PERM_INVALID = 11,            /* exception::PermInvalid */
+    QS_ERR_PERM_MISMATCH_DIMS = 12,      /* exception::PermMismatchDims */
+    QS_ERR_OUT_OF_RANGE = 13,            /* exception::OutOfRange */
+    QS_ERR_TYPE_MISMATCH = 14,           /* exception::TypeMismatch */
+    QS_ERR_SIZE_MISMATCH = 15,           /* exception::SizeMismatch */
+    QS_ERR_QUDIT_ALREADY_MEASURED = 16,  /* exception::QuditAlreadyMeasured */
+    QS_ERR_NOT_IMPLEMENTED = 17,         /* exception::NotImplemented */
+    QS_ERR_OVERFLOW = 18,                /* std::overflow_error (normalize) */
+    /* engine-side conditions with no qpp counterpart */
+    QS_ERR_INVALID_HANDLE = 100,
+    QS_ERR_OUT_OF_MEMORY = 101,          /* device memory exhausted */
+    QS_ERR_DEVICE = 102,                 /* HIP runtime error */
+    QS_ERR_NO_DEVICE = 103               /* no gfx950 device / library not built for it */
+} qs_status;
+
+typedef enum qs_dtype { QS_C128 = 0, QS_C64 = 1 } qs_dtype;
+typedef enum qs_kind { QS_KET = 0, QS_RHO = 1 } qs_kind;
+
+/* ptranspose / syspermute semantics.  qpp's generic (documented) path and
+ * its qubit kernels disagree for most targets / permutations
+ * (internal/kernels/qubit/ptranspose.hpp:101-119, syspermute.hpp:70-93
+ * against operations.hpp:2246-2331, 2480-2495); both are offered. */
+typedef enum qs_semantics {
+    QS_SEM_DOCUMENTED = 0, /* generic path: operations.hpp */
+    QS_SEM_AS_BUILT = 1    /* QPP_QUBIT_OPTIMIZATIONS qubit kernels */
+} qs_semantics;
+
+typedef struct qs_state qs_state; /* opaque, device resident */
+
+/* Human-readable message of the last failing call on this thread. */
+const char* qs_last_error(void);
+/* Library / device information (device name, CU count, HBM bytes). */
+int qs_device_info(char* name, size_t name_len, int* num_cu,
+                   uint64_t* hbm_bytes, uint64_t* hbm_free);
+/* Synchronise the device (all streams the library uses). */
+int qs_synchronize(void);
+/* Timing on the stream the library launches its kernels on: begin records a
+ * HIP event, end records a second one, waits for it and returns the elapsed
+ * milliseconds between the two. */
+int qs_profile_begin(void);
+int qs_profile_end(double* ms);
+
+/* ------------------------------------------------------------------------ */
+/* State handles                                                             */
+/* ------------------------------------------------------------------------ */
+/* Allocate a state over subsystems of dimensions dims[0..ndims-1]: a ket of
+ * D = prod(dims) amplitudes or a D x D density matrix.  Contents are |0..0>
+ * (ket) or |0..0><0..0| (rho), as QEngineState's initial state
+ * (internal/classes/qengine_state.hpp:137-143). */
+int qs_state_create(qs_dtype dtype, qs_kind kind, uint32_t ndims,
+                    const uint64_t* dims, qs_state** out);
+int qs_state_destroy(qs_state* st);
+int qs_state_clone(const qs_state* st, qs_state** out);
+int qs_state_info(const qs_state* st, qs_dtype* dtype, qs_kind* kind,
+                  uint32_t* ndims, uint64_t* dims_out /* >= ndims */,
+                  uint64_t* num_elems);
+/* Host <-> device copies of the whole state.  `host` holds num_elems
+ * interleaved complex values of the state's dtype (complex128: 16 B each,
+ * complex64: 8 B each), column-major. */
+int qs_state_upload(qs_state* st, const void* host, uint64_t nbytes);
+int qs_state_download(const qs_state* st, void* host, uint64_t nbytes);
+/* Basis state |index> or |index><index|. */
+int qs_state_set_basis(qs_state* st, uint64_t index);
+/* Raw device pointer and HIP stream of a handle (for zero-copy callers). */
+int qs_state_device_ptr(const qs_state* st, void** dptr);
+
+/* ------------------------------------------------------------------------ */
+/* Gate application (in place).  A: column-major (2^k..) x (..) complex128.   */
+/* ------------------------------------------------------------------------ */
+/* qpp::apply(state, A, target, dims)  operations.hpp:72-372;
+ * qubit kernels apply_psi_{1q,2q,3q,kq}, apply_rho_{1q,2q,3q,kq}
+ * (internal/kernels/qubit/apply.hpp:57-1167). */
+int qs_apply(qs_state* st, const double* A, uint64_t dimA, uint32_t ntarget,
+             const uint64_t* target);
+
+/* qpp::applyCTRL(state, A, ctrl, target, dims, shift)  operations.hpp:616-887;
+ * qubit kernels apply_ctrl_{psi,rho}_{1q,2q,kq} (apply_ctrl.hpp:63-1124).
+ * shift may be NULL (std::nullopt). */
+int qs_apply_ctrl(qs_state* st, const double* A, uint64_t dimA, uint32_t nctrl,
+                  const uint64_t* ctrl, uint32_t ntarget,
+                  const uint64_t* target, const uint64_t* shift);
+
+/* qpp::applyCTRL_fan(state, A, ctrl, target, dims, shift)
+ * operations.hpp:961-1234; apply_ctrl_fan_{psi,rho}
+ * (internal/kernels/qubit/apply_ctrl_fan.hpp:62-303).  A acts on one
+ * subsystem and is applied to every target under the same controls. */
+int qs_apply_ctrl_fan(qs_state* st, const double* A, uint64_t dimA,
+                      uint32_t nctrl, const uint64_t* ctrl, uint32_t ntarget,
+                      const uint64_t* target, const uint64_t* shift);
+
+/* qpp::apply(rho, Ks, target, dims)  operations.hpp:476-560: the channel
+ * sum_K K rho K^dagger, fused into one pass.  Ks: nK matrices, each
+ * dimK x dimK, stored back to back. */
+int qs_apply_kraus(qs_state* rho, const double* Ks, uint32_t nK, uint64_t dimK,
+                   uint32_t ntarget, const uint64_t* target);
+
+/* ------------------------------------------------------------------------ */
+/* Reductions and index maps (out-of-place: create a new handle).            */
+/* ------------------------------------------------------------------------ */
+/* qpp::ptrace(A, target, dims)  operations.hpp:1908-2114;
+ * ptrace_{psi,rho}_kq (internal/kernels/qubit/ptrace.hpp:61-257).
+ * Result: density matrix over the kept subsystems. */
+int qs_ptrace(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+              qs_state** out);
+/* qpp::ptranspose(A, target, dims)  operations.hpp:2167-2335;
+ * ptranspose_{psi,rho}_kq (internal/kernels/qubit/ptranspose.hpp:59-219).
+ * Result: density matrix. */
+int qs_ptranspose(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+                  qs_semantics sem, qs_state** out);
+/* qpp::syspermute(A, perm, dims)  operations.hpp:2386-2514;
+ * syspermute_{psi,rho}_kq (internal/kernels/qubit/syspermute.hpp:58-180). */
+int qs_syspermute(const qs_state* st, const uint64_t* perm, qs_semantics sem,
+                  qs_state** out);
+
+/* Squared 2-norm (ket) or real trace (rho), FP64 accumulation
+ * (qpp::norm functions.hpp:282-294, trace). */
+int qs_norm2(const qs_state* st, double* out);
+/* qpp::normalize(A)  functions.hpp:304-333: divide by the norm (ket) or
+ * by the trace (rho); QS_ERR_OVERFLOW on zero. */
+int qs_normalize(qs_state* st);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement                                                               */
+/* ------------------------------------------------------------------------ */
+/* Z-basis probabilities |psi_i|^2 (ket, re*re + im*im as std::norm) or
+ * Re rho(i,i) (rho), written to host memory (num basis states doubles).
+ * qpp::abssq functions.hpp:1756-1800; instruments.hpp:783-792. */
+int qs_probs(const qs_state* st, double* host_out, uint64_t n_out);
+
+/* qpp::sample(A, target, dims) repeated nshots times with caller uniforms
+ * u[0..nshots-1]: each shot picks basis index
+ *   lower_bound(partial_sum(p / sum(p)) with last := 1, u)
+ * exactly as libstdc++'s discrete_distribution does
+ * (instruments.hpp:746-809; /usr/include/c++/11/bits/random.tcc:2656-2712).
+ * Outputs (host, any may be NULL): full basis index per shot, target digits
+ * per shot (nshots x ntarget, row per shot), probability p[index] per shot.
+ * The chosen indices are bit-identical to the sequential reference. */
+int qs_sample(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+              uint64_t nshots, const double* u, uint64_t* out_index,
+              uint64_t* out_digits, double* out_prob);
+
+/* qpp::measure_seq(A, target, dims, destructive)  instruments.hpp:985-1099,
+ * in place, with one caller uniform for the sample.  results/probs: ntarget
+ * entries in target order.  A destructive measurement removes the measured
+ * subsystems from the handle (its dims shrink). */
+int qs_measure_seq(qs_state* st, uint32_t ntarget, const uint64_t* target,
+                   int destructive, double u, uint64_t* results,
+                   double* probs);
+
+/* ------------------------------------------------------------------------ */
+/* Circuits and engines                                                      */
+/* ------------------------------------------------------------------------ */
+/* A circuit is a flat program of steps (the VarStep list of qpp::QCircuit,
+ * classes/qcircuit.hpp:139-143, lowered to POD).  Gate steps reference a
+ * matrix table; index lists live in one pool. */
+typedef enum qs_step_type {
+    QS_STEP_NOP = 0,
+    QS_STEP_GATE = 1,        /* SINGLE/TWO/THREE/JOINT: apply(U, target) */
+    QS_STEP_FAN = 2,         /* FAN: apply(U, {t}) for every target */
+    QS_STEP_CTRL = 3,        /* CTRL: applyCTRL(U, ctrl, target, shift) */
+    QS_STEP_CTRL_FAN = 4,    /* CTRL_FAN: applyCTRL_fan */
+    QS_STEP_MEASURE = 5,     /* MEASURE / MEASURE_ND (flag) on target[0] */
+    QS_STEP_MEASURE_MANY = 6 /* MEASURE_MANY / _ND on all targets */
+} qs_step_type;
+
+typedef struct qs_step {
+    uint32_t type;        /* qs_step_type */
+    uint32_t gate;        /* matrix index (gate steps) */
+    uint32_t nctrl;       /* number of controls */
+    uint32_t ntarget;     /* number of targets */
+    uint32_t ctrl_off;    /* offset of the controls in the index pool */
+    uint32_t target_off;  /* offset of the targets in the index pool */
+    uint32_t shift_off;   /* offset of the shifts, UINT32_MAX = no shift */
+    uint32_t c_reg;       /* first classical dit written by a measurement */
+    uint32_t destructive; /* measurement: 1 = destructive (MEASURE), 0 = _ND */
+    uint32_t reserved;
+} qs_step;
+
+typedef struct qs_circuit {
+    uint32_t nq;              /* number of qudits */
+    uint32_t nc;              /* number of classical dits */
+    uint32_t d;               /* local dimension (2 for qubits) */
+    uint32_t nsteps;
+    const qs_step* steps;
+    uint32_t nmats;
+    const uint64_t* mat_dim;  /* nmats: matrix dimension */
+    const uint64_t* mat_off;  /* nmats: offset in complex elements */
+    const double* mats;       /* interleaved complex, column-major */
+    const uint64_t* pool;     /* index pool */
+} qs_circuit;
+
+typedef struct qs_engine qs_engine;
+
+/* Engine over a circuit (a deep copy is taken).  noise_p < 0: QEngine;
+ * noise_p >= 0: QNoisyEngine<QubitDepolarizingNoise(noise_p)>
+ * (classes/qengine.hpp:77-1516, classes/qnoisy_engine.hpp:59-178,
+ * classes/noise.hpp:388-408).  The engine's std::mt19937 is seeded with
+ * `seed`; it draws uniforms exactly as qpp's engines do. */
+int qs_engine_create(const qs_circuit* qc, qs_dtype dtype, double noise_p,
+                     uint64_t seed, qs_engine** out);
+int qs_engine_destroy(qs_engine* e);
+/* QEngineT::execute(reps)  qengine.hpp:1341-1405 (QNoisyEngine: 129-140). */
+int qs_engine_execute(qs_engine* e, uint64_t reps);
+/* Statistics: number of distinct outcomes, then (outcome dits, count)
+ * pairs in lexicographic order (QEngineStatistics, std::map order). */
+int qs_engine_num_outcomes(const qs_engine* e, uint64_t* n);
+int qs_engine_get_stats(const qs_engine* e, uint64_t* dits /* n x nc */,
+                        uint64_t* counts /* n */);
+int qs_engine_get_dits(const qs_engine* e, uint64_t* dits /* nc */);
+int qs_engine_get_probs(const qs_engine* e, double* probs /* nc */);
+/* Current quantum state (over the non-measured qudits). */
+int qs_engine_get_state(const qs_engine* e, qs_state** out);
+int qs_engine_set_state(qs_engine* e, const qs_state* st);
+/* QEngine::reset(): |0..0>, stats cleared. */
+int qs_engine_reset(qs_engine* e);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* QPP_AMD_H_ */

@@ -1,0 +1,128 @@
+/*
+ * oracle.h -- CPU restatement of qpp 7.0.3's gate-application path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU engine is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (qpp_amd/) never calls it.
+ *
+ * qpp itself cannot be compiled here (it needs Eigen >= 5.0, fetched at
+ * configure time, cmake/qpp_eigen.cmake:9-43), so this is a restatement in
+ * plain C++17 + OpenMP of the reference's algorithms, each function citing
+ * the file:line it follows (paths relative to the qpp source tree).
+ *
+ * Storage: interleaved complex doubles, column-major (Eigen's layout);
+ * rho(i, j) at element i + j*D.  Index vectors are uint64_t.
+ * Return value: 0 on success, otherwise a qs_status code (qpp_amd.h).
+ */
+#ifndef QPP_ORACLE_H_
+#define QPP_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* apply(state, A, target, dims): operations.hpp:72-372 dispatch;
+ * qubit kernels apply.hpp:57-1167 when all dims are 2. */
+int orc_apply(const double* state, int is_rho, uint32_t ndims,
+              const uint64_t* dims, const double* A, uint32_t ntarget,
+              const uint64_t* target, double* out);
+
+/* applyCTRL: operations.hpp:616-887; apply_ctrl.hpp:63-1124. */
+int orc_apply_ctrl(const double* state, int is_rho, uint32_t ndims,
+                   const uint64_t* dims, const double* A, uint32_t nctrl,
+                   const uint64_t* ctrl, uint32_t ntarget,
+                   const uint64_t* target, const uint64_t* shift,
+                   double* out);
+
+/* applyCTRL_fan: operations.hpp:961-1234; apply_ctrl_fan.hpp:62-303. */
+int orc_apply_ctrl_fan(const double* state, int is_rho, uint32_t ndims,
+                       const uint64_t* dims, const double* A, uint32_t nctrl,
+                       const uint64_t* ctrl, uint32_t ntarget,
+                       const uint64_t* target, const uint64_t* shift,
+                       double* out);
+
+/* apply(rho, Ks, target, dims): operations.hpp:476-560. */
+int orc_apply_kraus(const double* rho, uint32_t ndims, const uint64_t* dims,
+                    uint32_t nK, const double* Ks, uint32_t ntarget,
+                    const uint64_t* target, double* out);
+
+/* ptrace: operations.hpp:1908-2114; ptrace.hpp:61-257.  out: Dbar x Dbar. */
+int orc_ptrace(const double* A, int is_rho, uint32_t ndims,
+               const uint64_t* dims, uint32_t ntarget, const uint64_t* target,
+               double* out);
+
+/* ptranspose: operations.hpp:2167-2335 (sem 0); ptranspose.hpp:59-219
+ * (sem 1, qubits only).  out: D x D. */
+int orc_ptranspose(const double* A, int is_rho, uint32_t ndims,
+                   const uint64_t* dims, uint32_t ntarget,
+                   const uint64_t* target, int sem, double* out);
+
+/* syspermute: operations.hpp:2386-2514 (sem 0); syspermute.hpp:58-180
+ * (sem 1, qubits only). */
+int orc_syspermute(const double* A, int is_rho, uint32_t ndims,
+                   const uint64_t* dims, const uint64_t* perm, int sem,
+                   double* out);
+
+/* std::norm per element / Re diag: functions.hpp:1756-1800,
+ * instruments.hpp:783-792. */
+int orc_probs(const double* A, int is_rho, uint64_t D, double* out);
+
+/* libstdc++ discrete_distribution<size_t>(p, p+N) evaluated at uniform u
+ * (random.tcc:2656-2712): normalise by the sequential sum, sequential
+ * partial_sum, last := 1, lower_bound.  Returns the index. */
+uint64_t orc_discrete_draw(const double* p, uint64_t N, double u);
+/* Same, also returning the cumulative table (N doubles) it built. */
+void orc_discrete_table(const double* p, uint64_t N, double* cp);
+
+/* sample(A, target, dims) with the given uniform: instruments.hpp:746-809. */
+int orc_sample(const double* A, int is_rho, uint32_t ndims,
+               const uint64_t* dims, uint32_t ntarget, const uint64_t* target,
+               double u, uint64_t* out_index, uint64_t* out_digits,
+               double* out_prob);
+
+/* measure_seq(A, target, dims, destructive) with the given uniform:
+ * instruments.hpp:985-1099.  out_state receives the post-measurement state
+ * (capacity: the input size), *out_len its number of complex elements. */
+int orc_measure_seq(const double* A, int is_rho, uint32_t ndims,
+                    const uint64_t* dims, uint32_t ntarget,
+                    const uint64_t* target, int destructive, double u,
+                    uint64_t* results, double* probs, double* out_state,
+                    uint64_t* out_len);
+
+/* std::mt19937 + std::generate_canonical<double, 53> (the draws qpp's
+ * RandomDevices feeds to every distribution; random.h:164-192). */
+void* orc_rng_new(uint64_t seed);
+double orc_rng_canonical(void* rng);
+void orc_rng_free(void* rng);
+
+/* CPU baseline: runs `ngates` gates on |0..0> of n qubits with the restated
+ * qubit kernels apply_psi_1q (kind 0, apply.hpp:57-126) and
+ * apply_ctrl_psi_1q (kind 1, apply_ctrl.hpp:63-177), value semantics and
+ * OpenMP placement as in qpp, and writes each gate's wall time (seconds,
+ * std::chrono::steady_clock as qpp::Timer, classes/timer.hpp:51) to
+ * seconds[].  mats: 4 complex (column-major) per gate.  Returns the thread
+ * count OpenMP used. */
+int orc_time_gates(uint32_t n, uint32_t ngates, const uint32_t* kind,
+                   const uint32_t* target, const uint32_t* ctrl,
+                   const double* mats, double* seconds);
+
+/* QEngineT / QNoisyEngineT execute(reps) over a qs_circuit program
+ * (qengine.hpp:240-481, 533-823, 1341-1405; qnoisy_engine.hpp:104-140;
+ * noise.hpp:131-160, 388-408).  noise_p < 0 means no noise.  is_rho
+ * selects QEngineT<cmat>.  Returns statistics (outcomes in std::map order),
+ * the final classical dits/probs and the final state.
+ * stats_dits: capacity max_outcomes * nc; returns count in *n_outcomes. */
+struct qs_circuit;
+int orc_engine_run(const struct qs_circuit* qc, int is_rho, double noise_p,
+                   uint64_t seed, uint64_t reps, uint64_t max_outcomes,
+                   uint64_t* n_outcomes, uint64_t* stats_dits,
+                   uint64_t* stats_counts, uint64_t* dits, double* probs,
+                   double* out_state, uint64_t* out_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QPP_ORACLE_H_ */

@@ -1,0 +1,508 @@
+"""qpp_amd -- MI355X (gfx950) engine for Quantum++'s gate-application path.
+
+Python mirror of the qpp functions the engine replaces, over the C ABI of
+libqpp_amd.so (include/qpp_amd.h).  Names, argument meaning and error
+behaviour follow qpp 7.0.3:
+
+    apply(state, A, target, dims_or_d)          operations.hpp:72-409
+    apply(rho, Ks, target, dims_or_d)           operations.hpp:476-593
+    applyCTRL(state, A, ctrl, target, dims_or_d, shift=None)
+                                                operations.hpp:616-936
+    applyCTRL_fan(state, A, ctrl, target, dims_or_d, shift=None)
+                                                operations.hpp:961-1284
+    ptrace(A, target, dims_or_d)                operations.hpp:1908-2152
+    ptranspose(A, target, dims_or_d, semantics) operations.hpp:2167-2372
+    syspermute(A, perm, dims_or_d, semantics)   operations.hpp:2386-2550
+    sample(A, target, dims_or_d, u)             instruments.hpp:746-929
+    measure_seq(A, target, dims_or_d, destructive, u)
+                                                instruments.hpp:985-1135
+
+Value semantics as in qpp: numpy arrays in, new numpy arrays out (a ket is a
+1-D complex array, a density matrix a 2-D one).  For device-resident work
+use `State` (in-place operations on a device handle).  Random draws are
+explicit uniforms `u` in [0, 1) (what qpp draws from its mt19937 through
+std::generate_canonical); `Rng` reproduces that generator.
+
+Errors raise `QppError` subclasses named after qpp::exception classes.  The
+library has no CPU fallback: without a gfx950 device every compute call
+raises `NoDevice`.
+"""
+import ctypes
+import math
+import os
+from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqpp_amd.so")
+
+C128, C64 = 0, 1
+KET, RHO = 0, 1
+DOCUMENTED, AS_BUILT = 0, 1
+
+# ---------------------------------------------------------------------------
+# exceptions (include/qpp/classes/exception.hpp)
+# ---------------------------------------------------------------------------
+
+
+class QppError(Exception):
+    """Base of every error raised by the engine (qpp::exception::Exception)."""
+
+    code = -1
+
+
+def _mk(name, code):
+    return type(name, (QppError,), {"code": code})
+
+
+ZeroSize = _mk("ZeroSize", 1)
+MatrixNotSquare = _mk("MatrixNotSquare", 2)
+MatrixNotCvector = _mk("MatrixNotCvector", 3)
+MatrixNotSquareNorCvector = _mk("MatrixNotSquareNorCvector", 4)
+MatrixMismatchSubsys = _mk("MatrixMismatchSubsys", 5)
+DimsInvalid = _mk("DimsInvalid", 6)
+DimsNotEqual = _mk("DimsNotEqual", 7)
+DimsMismatchMatrix = _mk("DimsMismatchMatrix", 8)
+DimsMismatchCvector = _mk("DimsMismatchCvector", 9)
+SubsysMismatchDims = _mk("SubsysMismatchDims", 10)
+PermInvalid = _mk("PermInvalid", 11)
+PermMismatchDims = _mk("PermMismatchDims", 12)
+OutOfRange = _mk("OutOfRange", 13)
+TypeMismatch = _mk("TypeMismatch", 14)
+SizeMismatch = _mk("SizeMismatch", 15)
+QuditAlreadyMeasured = _mk("QuditAlreadyMeasured", 16)
+NotImplemented_ = _mk("NotImplemented", 17)
+OverflowError_ = _mk("OverflowError", 18)
+InvalidHandle = _mk("InvalidHandle", 100)
+OutOfMemory = _mk("OutOfMemory", 101)
+DeviceError = _mk("DeviceError", 102)
+NoDevice = _mk("NoDevice", 103)
+
+_BY_CODE = {c.code: c for c in [
+    ZeroSize, MatrixNotSquare, MatrixNotCvector, MatrixNotSquareNorCvector,
+    MatrixMismatchSubsys, DimsInvalid, DimsNotEqual, DimsMismatchMatrix,
+    DimsMismatchCvector, SubsysMismatchDims, PermInvalid, PermMismatchDims,
+    OutOfRange, TypeMismatch, SizeMismatch, QuditAlreadyMeasured,
+    NotImplemented_, OverflowError_, InvalidHandle, OutOfMemory, DeviceError,
+    NoDevice]}
+
+# ---------------------------------------------------------------------------
+# library
+# ---------------------------------------------------------------------------
+_lib = None
+
+
+def lib():
+    """Load libqpp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run python qpp_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        u64p = POINTER(c_uint64)
+        dp = POINTER(c_double)
+        vp = c_void_p
+        sigs = {
+            "qs_last_error": (c_char_p, []),
+            "qs_device_info": (c_int, [c_char_p, c_size_t, POINTER(c_int), u64p, u64p]),
+            "qs_synchronize": (c_int, []),
+            "qs_state_create": (c_int, [c_int, c_int, c_uint32, u64p, POINTER(vp)]),
+            "qs_state_destroy": (c_int, [vp]),
+            "qs_state_clone": (c_int, [vp, POINTER(vp)]),
+            "qs_state_info": (c_int, [vp, POINTER(c_int), POINTER(c_int), POINTER(c_uint32), u64p, u64p]),
+            "qs_state_upload": (c_int, [vp, vp, c_uint64]),
+            "qs_state_download": (c_int, [vp, vp, c_uint64]),
+            "qs_state_set_basis": (c_int, [vp, c_uint64]),
+            "qs_state_device_ptr": (c_int, [vp, POINTER(vp)]),
+            "qs_apply": (c_int, [vp, dp, c_uint64, c_uint32, u64p]),
+            "qs_apply_ctrl": (c_int, [vp, dp, c_uint64, c_uint32, u64p, c_uint32, u64p, u64p]),
+            "qs_apply_ctrl_fan": (c_int, [vp, dp, c_uint64, c_uint32, u64p, c_uint32, u64p, u64p]),
+            "qs_apply_kraus": (c_int, [vp, dp, c_uint32, c_uint64, c_uint32, u64p]),
+            "qs_ptrace": (c_int, [vp, c_uint32, u64p, POINTER(vp)]),
+            "qs_ptranspose": (c_int, [vp, c_uint32, u64p, c_int, POINTER(vp)]),
+            "qs_syspermute": (c_int, [vp, u64p, c_int, POINTER(vp)]),
+            "qs_norm2": (c_int, [vp, dp]),
+            "qs_normalize": (c_int, [vp]),
+            "qs_probs": (c_int, [vp, dp, c_uint64]),
+            "qs_sample": (c_int, [vp, c_uint32, u64p, c_uint64, dp, u64p, u64p, dp]),
+            "qs_measure_seq": (c_int, [vp, c_uint32, u64p, c_int, c_double, u64p, dp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().qs_last_error().decode(errors="replace")
+        raise _BY_CODE.get(rc, QppError)(msg)
+
+
+def _u64(seq):
+    a = np.ascontiguousarray(np.asarray(list(seq), dtype=np.uint64))
+    return a, a.ctypes.data_as(POINTER(c_uint64))
+
+
+def _mat(A):
+    """Column-major complex128 buffer of a square matrix (Eigen layout)."""
+    A = np.asarray(A, dtype=np.complex128)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise MatrixNotSquare("[WHERE: qpp_amd] Matrix is not square!")
+    buf = np.asfortranarray(A)
+    return buf, buf.ctypes.data_as(POINTER(c_double)), A.shape[0]
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = c_int()
+    tot = c_uint64()
+    free = c_uint64()
+    _check(lib().qs_device_info(name, 256, byref(cu), byref(tot), byref(free)))
+    return {"name": name.value.decode(), "cu": cu.value, "hbm_bytes": tot.value,
+            "hbm_free": free.value}
+
+
+def synchronize():
+    _check(lib().qs_synchronize())
+
+
+# ---------------------------------------------------------------------------
+# device-resident state
+# ---------------------------------------------------------------------------
+
+
+class State:
+    """A ket or density matrix in device memory (qs_state handle)."""
+
+    def __init__(self, dims, kind=KET, dtype=C128, _handle=None):
+        self._h = c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            arr, p = _u64(dims)
+            _check(lib().qs_state_create(dtype, kind, len(arr), p, byref(self._h)))
+
+    @classmethod
+    def _wrap(cls, handle):
+        return cls(None, _handle=handle)
+
+    @classmethod
+    def from_array(cls, A, dims, dtype=C128):
+        A = np.asarray(A)
+        kind = RHO if A.ndim == 2 and A.shape[1] != 1 else KET
+        if A.ndim == 2 and A.shape[1] == 1:
+            A = A[:, 0]
+        st = cls(dims, kind, dtype)
+        st.upload(A)
+        return st
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().qs_state_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        dt, kd, nd = c_int(), c_int(), c_uint32()
+        dims = (c_uint64 * 128)()
+        ne = c_uint64()
+        _check(lib().qs_state_info(self._h, byref(dt), byref(kd), byref(nd), dims, byref(ne)))
+        return dt.value, kd.value, [dims[i] for i in range(nd.value)], ne.value
+
+    @property
+    def dims(self):
+        return self.info()[2]
+
+    @property
+    def kind(self):
+        return self.info()[1]
+
+    @property
+    def dtype(self):
+        return self.info()[0]
+
+    def _np_dtype(self):
+        return np.complex128 if self.dtype == C128 else np.complex64
+
+    def upload(self, A):
+        dt, kd, dims, ne = self.info()
+        npdt = np.complex128 if dt == C128 else np.complex64
+        A = np.asarray(A, dtype=npdt)
+        buf = np.asfortranarray(A).reshape(-1, order="F") if A.ndim == 2 else np.ascontiguousarray(A)
+        _check(lib().qs_state_upload(self._h, buf.ctypes.data_as(c_void_p), buf.nbytes))
+
+    def download(self):
+        dt, kd, dims, ne = self.info()
+        npdt = np.complex128 if dt == C128 else np.complex64
+        out = np.empty(ne, dtype=npdt)
+        _check(lib().qs_state_download(self._h, out.ctypes.data_as(c_void_p), out.nbytes))
+        if kd == RHO:
+            D = int(round(math.sqrt(ne)))
+            return out.reshape((D, D), order="F")
+        return out
+
+    def clone(self):
+        h = c_void_p()
+        _check(lib().qs_state_clone(self._h, byref(h)))
+        return State._wrap(h)
+
+    def set_basis(self, index):
+        _check(lib().qs_state_set_basis(self._h, index))
+
+    # in-place operations -------------------------------------------------
+    def apply(self, A, target):
+        buf, p, dim = _mat(A)
+        t, tp = _u64(target)
+        _check(lib().qs_apply(self._h, p, dim, len(t), tp))
+        return self
+
+    def apply_ctrl(self, A, ctrl, target, shift=None):
+        buf, p, dim = _mat(A)
+        c, cp = _u64(ctrl)
+        t, tp = _u64(target)
+        s, sp = _u64(shift) if shift is not None else (None, None)
+        _check(lib().qs_apply_ctrl(self._h, p, dim, len(c), cp, len(t), tp, sp))
+        return self
+
+    def apply_ctrl_fan(self, A, ctrl, target, shift=None):
+        buf, p, dim = _mat(A)
+        c, cp = _u64(ctrl)
+        t, tp = _u64(target)
+        s, sp = _u64(shift) if shift is not None else (None, None)
+        _check(lib().qs_apply_ctrl_fan(self._h, p, dim, len(c), cp, len(t), tp, sp))
+        return self
+
+    def apply_kraus(self, Ks, target):
+        Ks = [np.asarray(K, dtype=np.complex128) for K in Ks]
+        if not Ks:
+            raise ZeroSize("[WHERE: qpp::apply()] Object has zero size! [CONTEXT: Ks]")
+        dim = Ks[0].shape[0]
+        for K in Ks:
+            if K.shape != Ks[0].shape:
+                raise DimsNotEqual("[WHERE: qpp::apply()] Dimensions not equal! [CONTEXT: K]")
+        buf = np.concatenate([np.asfortranarray(K).reshape(-1, order="F") for K in Ks])
+        t, tp = _u64(target)
+        _check(lib().qs_apply_kraus(self._h, buf.ctypes.data_as(POINTER(c_double)),
+                                    len(Ks), dim, len(t), tp))
+        return self
+
+    def ptrace(self, target):
+        t, tp = _u64(target)
+        h = c_void_p()
+        _check(lib().qs_ptrace(self._h, len(t), tp, byref(h)))
+        return State._wrap(h)
+
+    def ptranspose(self, target, semantics=DOCUMENTED):
+        t, tp = _u64(target)
+        h = c_void_p()
+        _check(lib().qs_ptranspose(self._h, len(t), tp, semantics, byref(h)))
+        return State._wrap(h)
+
+    def syspermute(self, perm, semantics=DOCUMENTED):
+        p, pp = _u64(perm)
+        h = c_void_p()
+        _check(lib().qs_syspermute(self._h, pp, semantics, byref(h)))
+        return State._wrap(h)
+
+    def norm2(self):
+        v = c_double()
+        _check(lib().qs_norm2(self._h, byref(v)))
+        return v.value
+
+    def normalize(self):
+        _check(lib().qs_normalize(self._h))
+        return self
+
+    def probs(self):
+        dt, kd, dims, ne = self.info()
+        D = int(np.prod(dims, dtype=np.uint64))
+        out = np.empty(D, dtype=np.float64)
+        _check(lib().qs_probs(self._h, out.ctypes.data_as(POINTER(c_double)), D))
+        return out
+
+    def sample(self, target, u):
+        """Indices, target digits and probabilities of len(u) shots."""
+        u = np.ascontiguousarray(np.atleast_1d(np.asarray(u, dtype=np.float64)))
+        t, tp = _u64(target)
+        n = len(u)
+        idx = np.empty(n, dtype=np.uint64)
+        dig = np.empty(max(1, n * len(t)), dtype=np.uint64)
+        pr = np.empty(n, dtype=np.float64)
+        _check(lib().qs_sample(self._h, len(t), tp, n, u.ctypes.data_as(POINTER(c_double)),
+                               idx.ctypes.data_as(POINTER(c_uint64)),
+                               dig.ctypes.data_as(POINTER(c_uint64)),
+                               pr.ctypes.data_as(POINTER(c_double))))
+        return idx, dig[: n * len(t)].reshape(n, len(t)), pr
+
+    def measure_seq(self, target, u, destructive=True):
+        t, tp = _u64(target)
+        res = np.empty(max(1, len(t)), dtype=np.uint64)
+        pr = np.empty(max(1, len(t)), dtype=np.float64)
+        _check(lib().qs_measure_seq(self._h, len(t), tp, int(bool(destructive)), float(u),
+                                    res.ctypes.data_as(POINTER(c_uint64)),
+                                    pr.ctypes.data_as(POINTER(c_double))))
+        return [int(x) for x in res[: len(t)]], [float(x) for x in pr[: len(t)]]
+
+
+# ---------------------------------------------------------------------------
+# qpp-style value-semantics functions
+# ---------------------------------------------------------------------------
+
+
+def get_num_subsys(D, d):
+    """internal/util.hpp:389-397: llround(log2(D) / log2(d))."""
+    return int(round(math.log2(D) / math.log2(d)))
+
+
+def _dims_of(state, dims_or_d):
+    state = np.asarray(state)
+    D = state.shape[0]
+    if np.isscalar(dims_or_d) or isinstance(dims_or_d, (int, np.integer)):
+        d = int(dims_or_d)
+        if d < 2:
+            raise DimsInvalid("[WHERE: qpp_amd] Invalid dimension(s)! [CONTEXT: d]")
+        return [d] * get_num_subsys(D, d)
+    return [int(x) for x in dims_or_d]
+
+
+def _as_state(state, dims, dtype=C128):
+    A = np.asarray(state)
+    if A.size == 0:
+        raise ZeroSize("[WHERE: qpp_amd] Object has zero size! [CONTEXT: state]")
+    if A.ndim == 2 and A.shape[1] == 1:
+        A = A[:, 0]
+    is_rho = A.ndim == 2
+    if is_rho and A.shape[0] != A.shape[1]:
+        raise MatrixNotSquareNorCvector(
+            "[WHERE: qpp_amd] Matrix is not square nor column vector! [CONTEXT: state]")
+    D = int(np.prod(dims)) if dims else 0
+    if not dims or any(x == 0 for x in dims):
+        raise DimsInvalid("[WHERE: qpp_amd] Invalid dimension(s)! [CONTEXT: dims]")
+    if D != A.shape[0]:
+        if is_rho:
+            raise DimsMismatchMatrix("[WHERE: qpp_amd] Dimension(s) mismatch matrix size! [CONTEXT: dims/state]")
+        raise DimsMismatchCvector("[WHERE: qpp_amd] Dimension(s) mismatch column vector size! [CONTEXT: dims/state]")
+    return State.from_array(A, dims, dtype)
+
+
+def apply(state, A, target, dims=2):
+    """qpp::apply(state, A, target, dims|d) -- or, when A is a list of Kraus
+    operators, qpp::apply(rho, Ks, target, dims|d)."""
+    if isinstance(A, (list, tuple)):
+        dims = _dims_of(state, dims)
+        st = _as_state(state, dims)
+        st.apply_kraus(A, target)
+        return st.download()
+    dims = _dims_of(state, dims)
+    st = _as_state(state, dims)
+    st.apply(A, target)
+    return st.download()
+
+
+def applyCTRL(state, A, ctrl, target, dims=2, shift=None):
+    dims = _dims_of(state, dims)
+    st = _as_state(state, dims)
+    st.apply_ctrl(A, ctrl, target, shift)
+    return st.download()
+
+
+def applyCTRL_fan(state, A, ctrl, target, dims=2, shift=None):
+    dims = _dims_of(state, dims)
+    st = _as_state(state, dims)
+    st.apply_ctrl_fan(A, ctrl, target, shift)
+    return st.download()
+
+
+def ptrace(A, target, dims=2):
+    dims = _dims_of(A, dims)
+    st = _as_state(A, dims)
+    return st.ptrace(target).download()
+
+
+def ptranspose(A, target, dims=2, semantics=DOCUMENTED):
+    dims = _dims_of(A, dims)
+    st = _as_state(A, dims)
+    return st.ptranspose(target, semantics).download()
+
+
+def syspermute(A, perm, dims=2, semantics=DOCUMENTED):
+    dims = _dims_of(A, dims)
+    if len(perm) != len(dims):
+        if sorted(perm) != list(range(len(perm))):
+            raise PermInvalid("[WHERE: qpp::syspermute()] Invalid permutation! [CONTEXT: perm]")
+        raise PermMismatchDims("[WHERE: qpp::syspermute()] Permutation mismatch dimensions! [CONTEXT: dims/perm]")
+    st = _as_state(A, dims)
+    return st.syspermute(perm, semantics).download()
+
+
+def sample(A, target, dims=2, u=None):
+    """qpp::sample(A, target, dims): (target digits, probability) for one
+    uniform u, or for an array u the per-shot digits."""
+    dims = _dims_of(A, dims)
+    st = _as_state(A, dims)
+    idx, dig, pr = st.sample(target, u)
+    if np.ndim(u) == 0:
+        return [int(x) for x in dig[0]], float(pr[0])
+    return dig, pr
+
+
+def measure_seq(A, target, dims=2, destructive=True, u=None):
+    """qpp::measure_seq -> (results, probs, post-measurement state)."""
+    dims = _dims_of(A, dims)
+    st = _as_state(A, dims)
+    res, pr = st.measure_seq(target, u, destructive)
+    return res, pr, st.download()
+
+
+class Rng:
+    """std::mt19937 + std::generate_canonical<double, 53> (the uniforms qpp's
+    RandomDevices feeds its distributions; bits/random.h:164-192)."""
+
+    def __init__(self, seed=5489):
+        self.mt = [0] * 624
+        self.idx = 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(624):
+            y = (mt[i] & 0x80000000) | (mt[(i + 1) % 624] & 0x7FFFFFFF)
+            mt[i] = mt[(i + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+        self.idx = 0
+
+    def next_u32(self):
+        if self.idx >= 624:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y & 0xFFFFFFFF
+
+    def canonical(self):
+        # generate_canonical<double, 53> with a 32-bit engine: k = 2 draws,
+        # sum = x0 + x1 * 2^32, r = sum / 2^64, clamped below 1
+        x0 = self.next_u32()
+        x1 = self.next_u32()
+        s = float(x0) + float(x1) * 4294967296.0
+        r = s / 18446744073709551616.0
+        if r >= 1.0:
+            r = math.nextafter(1.0, 0.0)
+        return r

@@ -1,0 +1,512 @@
+// qs_maps.hip -- ptrace, ptranspose and syspermute (out of place).
+//
+//   k_bitperm    out[y] = in[G(y)] for a permutation G of the index bits:
+//                syspermute of qubit kets / density matrices and ptranspose
+//                of qubit density matrices.  A workgroup moves one tile whose
+//                bit set holds the low output bits and the output bits fed by
+//                the low input bits, so both the gather from HBM and the
+//                scatter to HBM run over contiguous 2^L-element segments; the
+//                transposition happens in LDS.
+//   k_digitperm  the same map for qudit states (mixed-radix digits).
+//   k_ket_pt     ptranspose of a ket: out(i, j) = psi(i') conj(psi(j')).
+//   k_ptrace     partial trace; `LANES` lanes cooperate on one output entry
+//                and reduce with wavefront shuffles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "qs_internal.hpp"
+
+namespace {
+
+using qs::fail;
+
+template <typename T>
+struct Real;
+template <>
+struct Real<double2> {
+    using R = double;
+};
+template <>
+struct Real<float2> {
+    using R = float;
+};
+
+// ---------------------------------------------------------------------------
+// bit-permutation gather through LDS
+// ---------------------------------------------------------------------------
+struct BitPerm {
+    uint32_t nb;           // index bits
+    uint32_t tbits;        // |T|
+    uint32_t nrest;        // bits outside T
+    uint8_t src[64];       // input bit of output bit b
+    uint8_t load_out[16];  // load order: e bit k -> output bit load_out[k]
+    uint8_t store_out[16]; // store order: e bit k -> output bit store_out[k]
+    uint8_t tpos[16];      // T bits ascending (LDS packing order)
+    uint8_t rest[64];      // output bits outside T, ascending
+};
+
+__device__ __forceinline__ uint64_t gather_bits(uint64_t y, const uint8_t* src,
+                                                uint32_t nb) {
+    uint64_t x = 0;
+    for (uint32_t b = 0; b < nb; ++b) x |= ((y >> b) & 1) << src[b];
+    return x;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bitperm(const T* __restrict__ in,
+                                                 T* __restrict__ out,
+                                                 uint64_t ntiles, BitPerm bp) {
+    extern __shared__ unsigned char smem_raw[];
+    T* s = reinterpret_cast<T*>(smem_raw);
+    const uint32_t tsz = 1u << bp.tbits;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        // output bits outside T from the tile number
+        uint64_t ybase = 0;
+        for (uint32_t k = 0; k < bp.nrest; ++k) ybase |= ((tile >> k) & 1) << bp.rest[k];
+        for (uint32_t e = threadIdx.x; e < tsz; e += blockDim.x) {
+            uint64_t y = ybase;
+            uint32_t packed = 0;
+            for (uint32_t k = 0; k < bp.tbits; ++k) {
+                const uint32_t bit = (e >> k) & 1;
+                y |= uint64_t(bit) << bp.load_out[k];
+            }
+            for (uint32_t k = 0; k < bp.tbits; ++k)
+                packed |= uint32_t((y >> bp.tpos[k]) & 1) << k;
+            s[packed] = in[gather_bits(y, bp.src, bp.nb)];
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < tsz; e += blockDim.x) {
+            uint64_t y = ybase;
+            for (uint32_t k = 0; k < bp.tbits; ++k)
+                y |= uint64_t((e >> k) & 1) << bp.store_out[k];
+            uint32_t packed = 0;
+            for (uint32_t k = 0; k < bp.tbits; ++k)
+                packed |= uint32_t((y >> bp.tpos[k]) & 1) << k;
+            out[y] = s[packed];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mixed-radix digit permutation: output digit k (dims odim[k], stride ostr)
+// takes the input digit at input stride istr[k]
+// ---------------------------------------------------------------------------
+struct DigitPerm {
+    uint32_t nd;
+    uint64_t odim[80];
+    uint64_t istr[80];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_digitperm(const T* __restrict__ in,
+                                                   T* __restrict__ out,
+                                                   uint64_t n, DigitPerm dp) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t y = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; y < n;
+         y += stride) {
+        uint64_t rem = y, x = 0;
+        for (int k = int(dp.nd) - 1; k >= 0; --k) {
+            x += (rem % dp.odim[k]) * dp.istr[k];
+            rem /= dp.odim[k];
+        }
+        out[y] = in[x];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ket partial transpose: out(i, j) = psi(r) conj(psi(c)) with r, c the row
+// and column indices after swapping the target digits (mixed radix)
+// ---------------------------------------------------------------------------
+struct KetPT {
+    uint32_t n;
+    uint64_t dim[40];
+    uint64_t stride[40];
+    uint8_t swap[40];
+    uint64_t D;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ket_pt(const T* __restrict__ psi,
+                                                T* __restrict__ out, KetPT p) {
+    const uint64_t total = p.D * p.D;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t y = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; y < total;
+         y += stride) {
+        const uint64_t i = y % p.D, j = y / p.D;
+        uint64_t ri = i, rj = j, r = 0, c = 0;
+        for (int k = int(p.n) - 1; k >= 0; --k) {
+            const uint64_t di = ri % p.dim[k], dj = rj % p.dim[k];
+            ri /= p.dim[k];
+            rj /= p.dim[k];
+            r += (p.swap[k] ? dj : di) * p.stride[k];
+            c += (p.swap[k] ? di : dj) * p.stride[k];
+        }
+        const T a = psi[r], b = psi[c];
+        T v;
+        v.x = a.x * b.x + a.y * b.y;
+        v.y = a.y * b.x - a.x * b.y;
+        out[y] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// partial trace
+// ---------------------------------------------------------------------------
+struct PT {
+    uint32_t nkeep, ntr;
+    uint64_t kdim[40], kstr[40];  // kept subsystems (ascending), row strides
+    uint64_t tdim[40], tstr[40];  // traced subsystems (target order)
+    uint64_t Dbar, Dsub, D;
+    uint32_t rho;
+};
+
+__device__ __forceinline__ uint64_t mixed(uint64_t v, uint32_t n,
+                                          const uint64_t* dim,
+                                          const uint64_t* str) {
+    uint64_t off = 0;
+    for (int k = int(n) - 1; k >= 0; --k) {
+        off += (v % dim[k]) * str[k];
+        v /= dim[k];
+    }
+    return off;
+}
+
+template <typename T, int LANES>
+__global__ __launch_bounds__(256) void k_ptrace(const T* __restrict__ a,
+                                                T* __restrict__ out, PT p) {
+    using R = typename Real<T>::R;
+    const uint64_t groups = uint64_t(gridDim.x) * (256 / LANES);
+    const uint32_t lane = threadIdx.x % LANES;
+    const uint64_t total = p.Dbar * p.Dbar;
+    for (uint64_t o = uint64_t(blockIdx.x) * (256 / LANES) + threadIdx.x / LANES;
+         o < total + 0 * groups; o += groups) {
+        const uint64_t i = o % p.Dbar, j = o / p.Dbar;
+        const uint64_t rb = mixed(i, p.nkeep, p.kdim, p.kstr);
+        const uint64_t cb = mixed(j, p.nkeep, p.kdim, p.kstr);
+        R sr = 0, si = 0;
+        for (uint64_t t = lane; t < p.Dsub; t += LANES) {
+            const uint64_t off = mixed(t, p.ntr, p.tdim, p.tstr);
+            if (p.rho) {
+                const T v = a[(rb + off) + (cb + off) * p.D];
+                sr += v.x;
+                si += v.y;
+            } else {
+                const T x = a[rb + off], y = a[cb + off];
+                // x * conj(y)
+                sr += x.x * y.x + x.y * y.y;
+                si += x.y * y.x - x.x * y.y;
+            }
+        }
+        for (int m = LANES / 2; m > 0; m >>= 1) {
+            sr += __shfl_xor(sr, m, LANES);
+            si += __shfl_xor(si, m, LANES);
+        }
+        if (lane == 0) {
+            T v;
+            v.x = sr;
+            v.y = si;
+            out[o] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+bool valid(const qs_state* st) { return st != nullptr && st->d != nullptr; }
+
+uint64_t row_stride(const qs_state* st, uint64_t q) {
+    uint64_t s = 1;
+    for (uint64_t i = st->dims.size(); i-- > q + 1;) s *= st->dims[i];
+    return s;
+}
+
+bool check_subsys(const std::vector<uint64_t>& subsys,
+                  const std::vector<uint64_t>& dims) {
+    if (subsys.size() > dims.size()) return false;
+    std::vector<uint64_t> s = subsys;
+    std::sort(s.begin(), s.end());
+    if (std::unique(s.begin(), s.end()) != s.end()) return false;
+    for (uint64_t i : subsys)
+        if (i + 1 > dims.size()) return false;
+    return true;
+}
+
+uint32_t grid_for(uint64_t work, uint64_t per) {
+    uint64_t b = (work + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > (uint64_t{1} << 30)) b = uint64_t{1} << 30;
+    return static_cast<uint32_t>(b);
+}
+
+int make_out(const qs_state* st, qs_kind kind, const std::vector<uint64_t>& dims,
+             qs_state** out) {
+    if (int rc = qs_state_create(st->dtype, kind, static_cast<uint32_t>(dims.size()),
+                                 dims.data(), out))
+        return rc;
+    // the new state must not be touched before `st`'s pending work is done:
+    // both live on the library stream
+    (*out)->stream = st->stream;
+    return QS_OK;
+}
+
+// out[y] = in[G(y)], src[b] = input bit of output bit b
+template <typename T>
+int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t>& src) {
+    const uint32_t nb = static_cast<uint32_t>(src.size());
+    BitPerm bp{};
+    bp.nb = nb;
+    for (uint32_t b = 0; b < nb; ++b) bp.src[b] = static_cast<uint8_t>(src[b]);
+    const uint32_t L = std::min<uint32_t>(nb, sizeof(T) == 16 ? 5 : 6);
+    // T = low L output bits + output bits whose source is a low input bit
+    std::vector<uint32_t> low_out, low_in_fed(L);
+    for (uint32_t b = 0; b < L; ++b) low_out.push_back(b);
+    std::vector<bool> inT(nb, false);
+    for (uint32_t b = 0; b < L; ++b) inT[b] = true;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (src[b] < L) {
+            low_in_fed[src[b]] = b;
+            inT[b] = true;
+        }
+    std::vector<uint32_t> tset;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (inT[b]) tset.push_back(b);
+    bp.tbits = static_cast<uint32_t>(tset.size());
+    for (size_t k = 0; k < tset.size(); ++k) bp.tpos[k] = static_cast<uint8_t>(tset[k]);
+    // load order: low e bits feed the output bits sourced by input bits 0..L-1
+    std::vector<uint32_t> load(low_in_fed.begin(), low_in_fed.end());
+    for (uint32_t b : tset)
+        if (std::find(load.begin(), load.end(), b) == load.end()) load.push_back(b);
+    // store order: low e bits are output bits 0..L-1
+    std::vector<uint32_t> store(low_out.begin(), low_out.end());
+    for (uint32_t b : tset)
+        if (std::find(store.begin(), store.end(), b) == store.end()) store.push_back(b);
+    for (size_t k = 0; k < tset.size(); ++k) {
+        bp.load_out[k] = static_cast<uint8_t>(load[k]);
+        bp.store_out[k] = static_cast<uint8_t>(store[k]);
+    }
+    std::vector<uint32_t> rest;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (!inT[b]) rest.push_back(b);
+    bp.nrest = static_cast<uint32_t>(rest.size());
+    for (size_t k = 0; k < rest.size(); ++k) bp.rest[k] = static_cast<uint8_t>(rest[k]);
+    const uint64_t ntiles = uint64_t{1} << bp.nrest;
+    const size_t shmem = sizeof(T) << bp.tbits;
+    hipLaunchKernelGGL(k_bitperm<T>, dim3(grid_for(ntiles, 1)), dim3(256), shmem,
+                       in->stream, static_cast<const T*>(in->d),
+                       static_cast<T*>(out->d), ntiles, bp);
+    QS_HIP(hipGetLastError(), "qpp_amd::bitperm");
+    return QS_OK;
+}
+
+int bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t>& src) {
+    return in->dtype == QS_C128 ? launch_bitperm<double2>(in, out, src)
+                                : launch_bitperm<float2>(in, out, src);
+}
+
+template <typename T>
+int launch_digitperm(const qs_state* in, qs_state* out, const DigitPerm& dp) {
+    hipLaunchKernelGGL(k_digitperm<T>, dim3(grid_for(out->nelem, 256)), dim3(256), 0,
+                       in->stream, static_cast<const T*>(in->d),
+                       static_cast<T*>(out->d), out->nelem, dp);
+    QS_HIP(hipGetLastError(), "qpp_amd::digitperm");
+    return QS_OK;
+}
+
+int digitperm(const qs_state* in, qs_state* out, const DigitPerm& dp) {
+    return in->dtype == QS_C128 ? launch_digitperm<double2>(in, out, dp)
+                                : launch_digitperm<float2>(in, out, dp);
+}
+
+template <typename T>
+int launch_ket_pt(const qs_state* in, qs_state* out, const KetPT& p) {
+    hipLaunchKernelGGL(k_ket_pt<T>, dim3(grid_for(out->nelem, 256)), dim3(256), 0,
+                       in->stream, static_cast<const T*>(in->d),
+                       static_cast<T*>(out->d), p);
+    QS_HIP(hipGetLastError(), "qpp_amd::ptranspose");
+    return QS_OK;
+}
+
+template <typename T>
+int launch_ptrace(const qs_state* in, qs_state* out, const PT& p) {
+    const uint64_t total = p.Dbar * p.Dbar;
+    if (p.Dsub >= 64) {
+        hipLaunchKernelGGL((k_ptrace<T, 64>), dim3(grid_for(total, 4)), dim3(256), 0,
+                           in->stream, static_cast<const T*>(in->d),
+                           static_cast<T*>(out->d), p);
+    } else if (p.Dsub >= 8) {
+        hipLaunchKernelGGL((k_ptrace<T, 8>), dim3(grid_for(total, 32)), dim3(256), 0,
+                           in->stream, static_cast<const T*>(in->d),
+                           static_cast<T*>(out->d), p);
+    } else {
+        hipLaunchKernelGGL((k_ptrace<T, 1>), dim3(grid_for(total, 256)), dim3(256), 0,
+                           in->stream, static_cast<const T*>(in->d),
+                           static_cast<T*>(out->d), p);
+    }
+    QS_HIP(hipGetLastError(), "qpp::ptrace()");
+    return QS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// qpp::ptrace(A, target, dims), operations.hpp:1908-2114; qubit kernels
+// ptrace.hpp:61-257 (same sums: traced digits in target order, kept digits
+// ascending).
+int qs_ptrace(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+              qs_state** out) {
+    constexpr const char* W = "qpp::ptrace()";
+    if (!valid(st) || !out) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    std::vector<uint64_t> tv(target, target + ntarget);
+    if (!check_subsys(tv, st->dims)) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    const uint64_t n = st->dims.size();
+    std::vector<uint64_t> keep;
+    for (uint64_t q = 0; q < n; ++q)
+        if (std::find(tv.begin(), tv.end(), q) == tv.end()) keep.push_back(q);
+    PT p{};
+    p.rho = st->kind == QS_RHO;
+    p.D = st->D;
+    p.Dbar = 1;
+    p.Dsub = 1;
+    p.nkeep = static_cast<uint32_t>(keep.size());
+    p.ntr = static_cast<uint32_t>(tv.size());
+    if (p.nkeep > 40 || p.ntr > 40) return fail(QS_ERR_NOT_IMPLEMENTED, W, "subsystems");
+    for (size_t k = 0; k < keep.size(); ++k) {
+        p.kdim[k] = st->dims[keep[k]];
+        p.kstr[k] = row_stride(st, keep[k]);
+        p.Dbar *= p.kdim[k];
+    }
+    for (size_t k = 0; k < tv.size(); ++k) {
+        p.tdim[k] = st->dims[tv[k]];
+        p.tstr[k] = row_stride(st, tv[k]);
+        p.Dsub *= p.tdim[k];
+    }
+    std::vector<uint64_t> odims;
+    for (uint64_t q : keep) odims.push_back(st->dims[q]);
+    if (odims.empty()) odims.push_back(1);  // full trace: a 1 x 1 matrix
+    if (int rc = make_out(st, QS_RHO, odims, out)) return rc;
+    return st->dtype == QS_C128 ? launch_ptrace<double2>(st, *out, p)
+                                : launch_ptrace<float2>(st, *out, p);
+}
+
+// qpp::ptranspose(A, target, dims), operations.hpp:2167-2335 (documented);
+// ptranspose_{psi,rho}_kq, ptranspose.hpp:59-219 (as-built: for
+// 0 < |target| < n the qubit kernels transpose the subsystems
+// complement({n-1-t : t in target})).
+int qs_ptranspose(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+                  qs_semantics sem, qs_state** out) {
+    constexpr const char* W = "qpp::ptranspose()";
+    if (!valid(st) || !out) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    std::vector<uint64_t> tv(target, target + ntarget);
+    if (!check_subsys(tv, st->dims)) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    const uint64_t n = st->dims.size();
+    // subsystems whose row and column digits are swapped
+    std::vector<bool> sw(n, false);
+    if (sem == QS_SEM_AS_BUILT && st->qubits && !tv.empty() && tv.size() != n) {
+        std::vector<bool> inR(n, false);
+        for (uint64_t t : tv) inR[n - 1 - t] = true;
+        for (uint64_t q = 0; q < n; ++q) sw[q] = !inR[q];
+    } else {
+        for (uint64_t t : tv) sw[t] = true;
+    }
+    if (int rc = make_out(st, QS_RHO, st->dims, out)) return rc;
+    if (st->kind == QS_KET) {
+        KetPT p{};
+        p.n = static_cast<uint32_t>(n);
+        p.D = st->D;
+        if (n > 40) return fail(QS_ERR_NOT_IMPLEMENTED, W, "subsystems");
+        for (uint64_t q = 0; q < n; ++q) {
+            p.dim[q] = st->dims[q];
+            p.stride[q] = row_stride(st, q);
+            p.swap[q] = sw[q];
+        }
+        return st->dtype == QS_C128 ? launch_ket_pt<double2>(st, *out, p)
+                                    : launch_ket_pt<float2>(st, *out, p);
+    }
+    if (st->qubits) {
+        std::vector<uint32_t> src(2 * n);
+        for (uint64_t q = 0; q < n; ++q) {
+            const uint32_t r = static_cast<uint32_t>(n - 1 - q), c = static_cast<uint32_t>(2 * n - 1 - q);
+            src[r] = sw[q] ? c : r;
+            src[c] = sw[q] ? r : c;
+        }
+        return bitperm(st, *out, src);
+    }
+    DigitPerm dp{};
+    dp.nd = static_cast<uint32_t>(2 * n);
+    if (dp.nd > 80) return fail(QS_ERR_NOT_IMPLEMENTED, W, "subsystems");
+    // output digits: column digits (most significant) then row digits
+    for (uint64_t q = 0; q < n; ++q) {
+        const uint64_t rs = row_stride(st, q), cs = rs * st->D;
+        dp.odim[q] = st->dims[q];
+        dp.istr[q] = sw[q] ? rs : cs;
+        dp.odim[n + q] = st->dims[q];
+        dp.istr[n + q] = sw[q] ? cs : rs;
+    }
+    return digitperm(st, *out, dp);
+}
+
+// qpp::syspermute(A, perm, dims), operations.hpp:2386-2514 (documented:
+// output subsystem k is input subsystem perm[k]); syspermute_{psi,rho}_kq,
+// syspermute.hpp:58-180 (as-built: output LSB-bit k is input LSB-bit
+// inv_perm[k]).
+int qs_syspermute(const qs_state* st, const uint64_t* perm, qs_semantics sem,
+                  qs_state** out) {
+    constexpr const char* W = "qpp::syspermute()";
+    if (!valid(st) || !out) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    const uint64_t n = st->dims.size();
+    if (!perm) return fail(QS_ERR_PERM_INVALID, W, "perm");
+    std::vector<uint64_t> pv(perm, perm + n);
+    {
+        std::vector<uint64_t> s = pv;
+        std::sort(s.begin(), s.end());
+        for (uint64_t i = 0; i < n; ++i)
+            if (s[i] != i) return fail(QS_ERR_PERM_INVALID, W, "perm");
+    }
+    const bool rho = st->kind == QS_RHO;
+    if (st->qubits) {
+        // output bit b <- input bit src[b], for the row bits and column bits
+        std::vector<uint32_t> srcn(n);
+        if (sem == QS_SEM_AS_BUILT) {
+            std::vector<uint64_t> inv(n);
+            for (uint64_t k = 0; k < n; ++k) inv[pv[k]] = k;
+            for (uint64_t k = 0; k < n; ++k) srcn[k] = static_cast<uint32_t>(inv[k]);
+        } else {
+            for (uint64_t k = 0; k < n; ++k)
+                srcn[n - 1 - k] = static_cast<uint32_t>(n - 1 - pv[k]);
+        }
+        if (int rc = make_out(st, st->kind, st->dims, out)) return rc;
+        std::vector<uint32_t> src(rho ? 2 * n : n);
+        for (uint64_t b = 0; b < n; ++b) {
+            src[b] = srcn[b];
+            if (rho) src[n + b] = static_cast<uint32_t>(n + srcn[b]);
+        }
+        return bitperm(st, *out, src);
+    }
+    std::vector<uint64_t> odims(n);
+    for (uint64_t k = 0; k < n; ++k) odims[k] = st->dims[pv[k]];
+    if (int rc = make_out(st, st->kind, odims, out)) return rc;
+    DigitPerm dp{};
+    dp.nd = static_cast<uint32_t>(rho ? 2 * n : n);
+    if (dp.nd > 80) return fail(QS_ERR_NOT_IMPLEMENTED, W, "subsystems");
+    for (uint64_t k = 0; k < n; ++k) {
+        dp.odim[k + (rho ? n : 0)] = odims[k];
+        dp.istr[k + (rho ? n : 0)] = row_stride(st, pv[k]);
+        if (rho) {
+            dp.odim[k] = odims[k];
+            dp.istr[k] = row_stride(st, pv[k]) * st->D;
+        }
+    }
+    return digitperm(st, *out, dp);
+}
+
+}  // extern "C"

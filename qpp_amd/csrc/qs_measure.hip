@@ -1,0 +1,748 @@
+// qs_measure.hip -- norms, probabilities, sampling and measure_seq.
+//
+// Sampling is bit-identical to qpp's, which builds libstdc++'s
+// discrete_distribution over all basis probabilities (instruments.hpp:783-799,
+// /usr/include/c++/11/bits/random.tcc:2656-2712):
+//   s    = p_0 + p_1 + ... (sequential),   q_k = p_k / s,
+//   cp_k = cp_{k-1} + q_k (sequential),    cp_{N-1} := 1,
+//   index = first k with cp_k >= u.
+// A sequential sum has no parallel form, so the GPU proceeds in two steps:
+//  1. k_chunk_sums / k_scan_chunks / k_search: a parallel prefix E_k of p in
+//     FP64 locates the candidate k with E_{k-1}/E < u <= E_k/E.
+//  2. The candidate is accepted when u is farther than B from both edges,
+//     where B bounds |cp_k - E_k/E| over all k (sum of the rounding errors of
+//     the sequential sum, the division and the parallel prefix; computed in
+//     bound_for()).  Shots within B of an edge are resolved by
+//     k_serial_exact, one workgroup replaying the reference's arithmetic in
+//     its order (the division is IEEE correctly rounded on gfx950, so q_k is
+//     computed in parallel; only the two running sums are serial).
+// Probabilities are re*re + im*im rounded as std::norm does (no FMA).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "qs_internal.hpp"
+
+namespace {
+
+using qs::fail;
+
+constexpr int kChunk = 4096;  // elements per chunk of the prefix
+
+// std::norm(z) rounding: two products and one sum.  Contraction into an FMA
+// is switched off here (the __dmul_rn / __dadd_rn helpers are plain
+// operators that clang would otherwise fuse after inlining).
+__device__ __forceinline__ double norm_exact(double x, double y) {
+#pragma clang fp contract(off)
+    const double xx = x * x;
+    const double yy = y * y;
+    return xx + yy;
+}
+__device__ __forceinline__ double norm_exact(float x, float y) {
+#pragma clang fp contract(off)
+    const double dx = x, dy = y;
+    const double xx = dx * dx;
+    const double yy = dy * dy;
+    return xx + yy;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_probs(const T* __restrict__ a,
+                                               double* __restrict__ p, uint64_t D,
+                                               int rho) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < D;
+         i += stride) {
+        if (rho)
+            p[i] = static_cast<double>(a[i + i * D].x);
+        else
+            p[i] = norm_exact(a[i].x, a[i].y);
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// block sum of 256 threads
+__device__ double block_sum(double v) {
+    __shared__ double s[4];
+    v = wave_sum(v);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) s[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0) r = s[0] + s[1] + s[2] + s[3];
+    __syncthreads();
+    return r;
+}
+
+// per-chunk sums of p: one block per chunk
+__global__ __launch_bounds__(256) void k_chunk_sums(const double* __restrict__ p,
+                                                    uint64_t D,
+                                                    double* __restrict__ sums,
+                                                    uint64_t nchunks) {
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint64_t b = c * kChunk;
+        double v = 0;
+        for (uint64_t i = b + threadIdx.x; i < b + kChunk && i < D; i += 256) v += p[i];
+        v = block_sum(v);
+        if (threadIdx.x == 0) sums[c] = v;
+    }
+}
+
+// exclusive prefix of the chunk sums, one block of 1024 threads; writes the
+// total to prefix[nchunks]
+__global__ __launch_bounds__(1024) void k_scan_chunks(const double* __restrict__ sums,
+                                                      uint64_t nchunks,
+                                                      double* __restrict__ prefix) {
+    __shared__ double part[1024];
+    const uint64_t per = (nchunks + 1023) / 1024;
+    const uint64_t b = threadIdx.x * per;
+    double v = 0;
+    for (uint64_t i = b; i < b + per && i < nchunks; ++i) v += sums[i];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = 0;
+        for (int t = 0; t < 1024; ++t) {
+            const double x = part[t];
+            part[t] = run;
+            run += x;
+        }
+        prefix[nchunks] = run;
+    }
+    __syncthreads();
+    double run = part[threadIdx.x];
+    for (uint64_t i = b; i < b + per && i < nchunks; ++i) {
+        prefix[i] = run;
+        run += sums[i];
+    }
+}
+
+// One wave per shot: binary search over chunk prefixes, then an in-chunk
+// wave scan.  Writes the candidate index and whether it is certain.
+__global__ __launch_bounds__(256) void k_search(const double* __restrict__ p,
+                                                uint64_t D,
+                                                const double* __restrict__ prefix,
+                                                uint64_t nchunks,
+                                                const double* __restrict__ u,
+                                                uint64_t nshots, double B,
+                                                uint64_t* __restrict__ idx,
+                                                int* __restrict__ certain) {
+    const uint64_t wave = (uint64_t(blockIdx.x) * 256 + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    if (wave >= nshots) return;
+    const double total = prefix[nchunks];
+    const double uu = u[wave];
+    const double target = uu * total;
+    // last chunk whose exclusive prefix is < target (chunk 0 if none)
+    uint64_t lo = 0, hi = nchunks;  // invariant: prefix[lo] < target or lo == 0
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (prefix[mid] < target)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    // scan the chunk for the first element with E_k >= target
+    double run = prefix[lo];
+    uint64_t found = D;  // D = not found in this chunk
+    double e_prev = 0, e_at = 0;
+    const uint64_t b = lo * kChunk;
+    for (uint64_t s = b; s < b + kChunk && s < D; s += 64) {
+        const uint64_t i = s + lane;
+        const double v = i < D ? p[i] : 0.0;
+        // inclusive wave scan
+        double inc = v;
+        for (int m = 1; m < 64; m <<= 1) {
+            const double o = __shfl_up(inc, m, 64);
+            if (lane >= m) inc += o;
+        }
+        const double e = run + inc;
+        const unsigned long long hit = __ballot(i < D && e >= target);
+        if (hit) {
+            const int l = __ffsll(hit) - 1;
+            found = s + l;
+            e_at = __shfl(e, l, 64);
+            e_prev = __shfl(e - v, l, 64);
+            break;
+        }
+        run += __shfl(inc, 63, 64);
+    }
+    if (lane == 0) {
+        if (found == D) {
+            // past the end of the chunk: the sequential table's last entry is
+            // forced to 1, so the answer is certain only away from the edge
+            idx[wave] = D - 1;
+            certain[wave] = 0;
+        } else {
+            idx[wave] = found;
+            const double lo_edge = e_prev / total, hi_edge = e_at / total;
+            certain[wave] = (lo_edge < uu - B) && (hi_edge >= uu + B) &&
+                            (found != D - 1 || hi_edge >= uu + B);
+        }
+    }
+}
+
+// q_k = p_k / s with s the sequential sum; one block
+__global__ __launch_bounds__(256) void k_serial_sum(const double* __restrict__ p,
+                                                    uint64_t D, double* __restrict__ s_out) {
+    // thread 0 adds in order; the block stages 4096-element tiles in LDS
+    __shared__ double tile[kChunk];
+    double s = 0;
+    const uint64_t ntiles = (D + kChunk - 1) / kChunk;
+    for (uint64_t t = 0; t < ntiles; ++t) {
+        const uint64_t b = t * kChunk;
+        for (uint64_t i = threadIdx.x; i < kChunk; i += 256)
+            tile[i] = (b + i < D) ? p[b + i] : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t n = (D - b < kChunk) ? D - b : kChunk;
+            for (uint64_t i = 0; i < n; ++i) s = __dadd_rn(s, tile[i]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *s_out = s;
+}
+
+__global__ __launch_bounds__(256) void k_div(const double* __restrict__ p,
+                                             uint64_t D, const double* __restrict__ s,
+                                             double* __restrict__ q) {
+    const double sv = *s;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < D;
+         i += stride)
+        q[i] = __ddiv_rn(p[i], sv);
+}
+
+// The reference's cumulative table, replayed in order, for the shots sorted
+// by u (order[] gives the shot of each sorted position).  One block.
+__global__ __launch_bounds__(256) void k_serial_exact(const double* __restrict__ q,
+                                                      uint64_t D,
+                                                      const double* __restrict__ u,
+                                                      const uint32_t* __restrict__ order,
+                                                      uint32_t nsorted,
+                                                      uint64_t* __restrict__ idx) {
+    __shared__ double tile[kChunk];
+    double cp = 0;
+    uint32_t next = 0;
+    const uint64_t ntiles = (D + kChunk - 1) / kChunk;
+    for (uint64_t t = 0; t < ntiles && next < nsorted; ++t) {
+        const uint64_t b = t * kChunk;
+        for (uint64_t i = threadIdx.x; i < kChunk; i += 256)
+            tile[i] = (b + i < D) ? q[b + i] : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint64_t n = (D - b < kChunk) ? D - b : kChunk;
+            for (uint64_t i = 0; i < n && next < nsorted; ++i) {
+                const uint64_t k = b + i;
+                cp = __dadd_rn(cp, tile[i]);
+                const double c = (k == D - 1) ? 1.0 : cp;
+                while (next < nsorted && c >= u[order[next]]) {
+                    idx[order[next]] = k;
+                    ++next;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reductions and scaling
+// ---------------------------------------------------------------------------
+// partial sums: ket -> sum |a|^2 (x), rho -> trace (x, y)
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm_partial(const T* __restrict__ a,
+                                                      uint64_t D, int rho,
+                                                      double2* __restrict__ part) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    double sx = 0, sy = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < D;
+         i += stride) {
+        if (rho) {
+            const T v = a[i + i * D];
+            sx += v.x;
+            sy += v.y;
+        } else {
+            const T v = a[i];
+            sx += double(v.x) * v.x + double(v.y) * v.y;
+        }
+    }
+    sx = block_sum(sx);
+    sy = block_sum(sy);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sx, sy);
+}
+
+__global__ __launch_bounds__(256) void k_norm_final(const double2* __restrict__ part,
+                                                    int n, double2* __restrict__ out) {
+    double sx = 0, sy = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        sx += part[i].x;
+        sy += part[i].y;
+    }
+    sx = block_sum(sx);
+    sy = block_sum(sy);
+    if (threadIdx.x == 0) *out = make_double2(sx, sy);
+}
+
+// a *= 1 / c  (complex c, computed in FP64)
+template <typename T>
+__global__ __launch_bounds__(256) void k_div_scalar(T* __restrict__ a, uint64_t n,
+                                                    double cr, double ci) {
+    const double den = cr * cr + ci * ci;
+    const double ir = cr / den, ii = -ci / den;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        const T v = a[i];
+        T r;
+        if (ci == 0.0) {
+            r.x = static_cast<decltype(v.x)>(double(v.x) / cr);
+            r.y = static_cast<decltype(v.x)>(double(v.y) / cr);
+        } else {
+            r.x = static_cast<decltype(v.x)>(double(v.x) * ir - double(v.y) * ii);
+            r.y = static_cast<decltype(v.x)>(double(v.x) * ii + double(v.y) * ir);
+        }
+        a[i] = r;
+    }
+}
+
+// zero every element whose digit (stride, dim) differs from m; for a
+// density matrix both the row and the column digit must equal m
+template <typename T>
+__global__ __launch_bounds__(256) void k_project(T* __restrict__ a, uint64_t n,
+                                                 uint64_t D, uint64_t stride,
+                                                 uint64_t dim, uint64_t m, int rho) {
+    const uint64_t gs = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += gs) {
+        const uint64_t r = i % D, c = i / D;
+        bool keep = (r / stride) % dim == m;
+        if (rho) keep = keep && ((c / stride) % dim == m);
+        if (!keep) {
+            T z;
+            z.x = 0;
+            z.y = 0;
+            a[i] = z;
+        }
+    }
+}
+
+// destructive contraction: out has subsystem `i` removed; element y of out
+// reads the input with digit m inserted (row and column for rho)
+template <typename T>
+__global__ __launch_bounds__(256) void k_contract(const T* __restrict__ in,
+                                                  T* __restrict__ out, uint64_t nout,
+                                                  uint64_t Dout, uint64_t Din,
+                                                  uint64_t stride, uint64_t dim,
+                                                  uint64_t m, int rho) {
+    const uint64_t gs = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t y = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; y < nout;
+         y += gs) {
+        const uint64_t r = y % Dout, c = y / Dout;
+        // insert digit m at `stride` (stride of the removed subsystem in the
+        // input equals the product of the dims after it, which is also the
+        // block of lower digits in the output)
+        const uint64_t ri = (r / stride) * stride * dim + m * stride + (r % stride);
+        uint64_t x = ri;
+        if (rho) {
+            const uint64_t ci = (c / stride) * stride * dim + m * stride + (c % stride);
+            x = ri + ci * Din;
+        }
+        out[y] = in[x];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+bool valid(const qs_state* st) { return st != nullptr && st->d != nullptr; }
+
+uint32_t grid_for(uint64_t work, uint64_t per) {
+    uint64_t b = (work + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > 65536) b = 65536;
+    return static_cast<uint32_t>(b);
+}
+
+uint64_t row_stride(const std::vector<uint64_t>& dims, uint64_t q) {
+    uint64_t s = 1;
+    for (uint64_t i = dims.size(); i-- > q + 1;) s *= dims[i];
+    return s;
+}
+
+bool check_subsys(const std::vector<uint64_t>& subsys,
+                  const std::vector<uint64_t>& dims) {
+    if (subsys.size() > dims.size()) return false;
+    std::vector<uint64_t> s = subsys;
+    std::sort(s.begin(), s.end());
+    if (std::unique(s.begin(), s.end()) != s.end()) return false;
+    for (uint64_t i : subsys)
+        if (i + 1 > dims.size()) return false;
+    return true;
+}
+
+// Device buffers reused across calls (grown on demand).  Calls are
+// serialised on the library stream, and every user below synchronises
+// before returning, so reuse between calls is safe.
+struct Buf {
+    void* p = nullptr;
+    size_t n = 0;
+    int need(size_t bytes) {
+        if (bytes <= n) return QS_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        QS_HIP(hipMalloc(&p, bytes), "qpp_amd::scratch");
+        n = bytes;
+        return QS_OK;
+    }
+};
+thread_local Buf g_p, g_aux, g_q;
+
+int norm_complex(const qs_state* st, double* re, double* im) {
+    const int nb = 1024;
+    if (int rc = g_aux.need(sizeof(double2) * (nb + 1))) return rc;
+    auto* part = static_cast<double2*>(g_aux.p);
+    const bool rho = st->kind == QS_RHO;
+    if (st->dtype == QS_C128)
+        hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nb), dim3(256), 0, st->stream,
+                           static_cast<const double2*>(st->d), rho ? st->D : st->nelem,
+                           int(rho), part);
+    else
+        hipLaunchKernelGGL(k_norm_partial<float2>, dim3(nb), dim3(256), 0, st->stream,
+                           static_cast<const float2*>(st->d), rho ? st->D : st->nelem,
+                           int(rho), part);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, st->stream, part, nb,
+                       part + nb);
+    double2 h;
+    QS_HIP(hipMemcpyAsync(&h, part + nb, sizeof(double2), hipMemcpyDeviceToHost,
+                          st->stream),
+           "qpp::norm()");
+    QS_HIP(hipStreamSynchronize(st->stream), "qpp::norm()");
+    *re = h.x;
+    *im = h.y;
+    return QS_OK;
+}
+
+int div_scalar(qs_state* st, double cr, double ci) {
+    if (st->dtype == QS_C128)
+        hipLaunchKernelGGL(k_div_scalar<double2>, dim3(grid_for(st->nelem, 256)),
+                           dim3(256), 0, st->stream, static_cast<double2*>(st->d),
+                           st->nelem, cr, ci);
+    else
+        hipLaunchKernelGGL(k_div_scalar<float2>, dim3(grid_for(st->nelem, 256)),
+                           dim3(256), 0, st->stream, static_cast<float2*>(st->d),
+                           st->nelem, cr, ci);
+    QS_HIP(hipGetLastError(), "qpp::normalize()");
+    return QS_OK;
+}
+
+int probs_device(const qs_state* st, double* p) {
+    const bool rho = st->kind == QS_RHO;
+    if (st->dtype == QS_C128)
+        hipLaunchKernelGGL(k_probs<double2>, dim3(grid_for(st->D, 256)), dim3(256), 0,
+                           st->stream, static_cast<const double2*>(st->d), p, st->D,
+                           int(rho));
+    else
+        hipLaunchKernelGGL(k_probs<float2>, dim3(grid_for(st->D, 256)), dim3(256), 0,
+                           st->stream, static_cast<const float2*>(st->d), p, st->D,
+                           int(rho));
+    QS_HIP(hipGetLastError(), "qpp::sample()");
+    return QS_OK;
+}
+
+// |cp_k (sequential) - E_k / E (parallel)| <= bound_for(N) for all k, with
+// probabilities p >= 0 normalised to sum 1 by construction of cp:
+//   sequential sum s: |s - S| <= (N-1) eps S             (recursive summation)
+//   q_k = p_k / s rounded: relative eps each
+//   sequential cp_k: absolute error <= k eps max|cp| <= k eps (1 + tiny)
+//   parallel prefix E_k and total E: relative (log2 N + chunk) eps each
+// so |cp_k - E_k/E| <= (2N + 2 log2 N + 2 kChunk + 8) eps (1 + o(1)); the
+// bound adds a factor 2 of slack.  eps = 2^-53.
+double bound_for(uint64_t N) {
+    const double eps = std::ldexp(1.0, -53);
+    const double lg = std::log2(double(N) + 1.0);
+    return 2.0 * (2.0 * double(N) + 2.0 * lg + 2.0 * kChunk + 8.0) * eps;
+}
+
+// Indices of nshots draws.  u_dev: device uniforms.  Results in idx_dev.
+int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
+                  std::vector<uint64_t>& out_idx) {
+    const uint64_t D = st->D;
+    out_idx.assign(nshots, 0);
+    if (D < 2) return QS_OK;  // a one-element table returns 0 (random.tcc:2700)
+    const uint64_t nchunks = (D + kChunk - 1) / kChunk;
+    // layout in g_aux: sums[nchunks] | prefix[nchunks+1] | u[nshots] |
+    // idx[nshots] | certain[nshots] | order[nshots] | s
+    const size_t off_prefix = nchunks * 8;
+    const size_t off_u = off_prefix + (nchunks + 1) * 8;
+    const size_t off_idx = off_u + nshots * 8;
+    const size_t off_cert = off_idx + nshots * 8;
+    const size_t off_order = off_cert + nshots * 4;
+    const size_t off_s = ((off_order + nshots * 4 + 15) / 16) * 16;
+    if (int rc = g_aux.need(off_s + 16)) return rc;
+    if (int rc = g_p.need(D * 8)) return rc;
+    auto* base = static_cast<unsigned char*>(g_aux.p);
+    auto* sums = reinterpret_cast<double*>(base);
+    auto* prefix = reinterpret_cast<double*>(base + off_prefix);
+    auto* ud = reinterpret_cast<double*>(base + off_u);
+    auto* idx = reinterpret_cast<uint64_t*>(base + off_idx);
+    auto* cert = reinterpret_cast<int*>(base + off_cert);
+    auto* order = reinterpret_cast<uint32_t*>(base + off_order);
+    auto* sdev = reinterpret_cast<double*>(base + off_s);
+    auto* p = static_cast<double*>(g_p.p);
+    if (int rc = probs_device(st, p)) return rc;
+    QS_HIP(hipMemcpyAsync(ud, u_host, nshots * 8, hipMemcpyHostToDevice, st->stream),
+           "qpp::sample()");
+    hipLaunchKernelGGL(k_chunk_sums, dim3(grid_for(nchunks, 1)), dim3(256), 0,
+                       st->stream, p, D, sums, nchunks);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, st->stream, sums,
+                       nchunks, prefix);
+    const double B = bound_for(D);
+    hipLaunchKernelGGL(k_search, dim3(grid_for(nshots * 64, 256)), dim3(256), 0,
+                       st->stream, p, D, prefix, nchunks, ud, nshots, B, idx, cert);
+    QS_HIP(hipGetLastError(), "qpp::sample()");
+    std::vector<int> hc(nshots);
+    QS_HIP(hipMemcpyAsync(out_idx.data(), idx, nshots * 8, hipMemcpyDeviceToHost,
+                          st->stream),
+           "qpp::sample()");
+    QS_HIP(hipMemcpyAsync(hc.data(), cert, nshots * 4, hipMemcpyDeviceToHost, st->stream),
+           "qpp::sample()");
+    QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+    std::vector<uint32_t> amb;
+    for (uint64_t s = 0; s < nshots; ++s)
+        if (!hc[s]) amb.push_back(static_cast<uint32_t>(s));
+    if (amb.empty()) return QS_OK;
+    // exact replay for the ambiguous shots, sorted by u
+    std::sort(amb.begin(), amb.end(), [&](uint32_t a, uint32_t b) {
+        return u_host[a] < u_host[b] || (u_host[a] == u_host[b] && a < b);
+    });
+    if (int rc = g_q.need(D * 8)) return rc;
+    auto* q = static_cast<double*>(g_q.p);
+    QS_HIP(hipMemcpyAsync(order, amb.data(), amb.size() * 4, hipMemcpyHostToDevice,
+                          st->stream),
+           "qpp::sample()");
+    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(256), 0, st->stream, p, D, sdev);
+    hipLaunchKernelGGL(k_div, dim3(grid_for(D, 256)), dim3(256), 0, st->stream, p, D,
+                       sdev, q);
+    hipLaunchKernelGGL(k_serial_exact, dim3(1), dim3(256), 0, st->stream, q, D, ud,
+                       order, static_cast<uint32_t>(amb.size()), idx);
+    QS_HIP(hipGetLastError(), "qpp::sample()");
+    QS_HIP(hipMemcpyAsync(out_idx.data(), idx, nshots * 8, hipMemcpyDeviceToHost,
+                          st->stream),
+           "qpp::sample()");
+    QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+    return QS_OK;
+}
+
+int project(qs_state* st, uint64_t stride, uint64_t dim, uint64_t m) {
+    const int rho = st->kind == QS_RHO;
+    if (st->dtype == QS_C128)
+        hipLaunchKernelGGL(k_project<double2>, dim3(grid_for(st->nelem, 256)), dim3(256),
+                           0, st->stream, static_cast<double2*>(st->d), st->nelem,
+                           st->D, stride, dim, m, rho);
+    else
+        hipLaunchKernelGGL(k_project<float2>, dim3(grid_for(st->nelem, 256)), dim3(256),
+                           0, st->stream, static_cast<float2*>(st->d), st->nelem,
+                           st->D, stride, dim, m, rho);
+    QS_HIP(hipGetLastError(), "qpp::measure_seq()");
+    return QS_OK;
+}
+
+// remove subsystem i, keeping digit m (ket: <m| contraction; rho: the (m, m)
+// block, i.e. ptrace over i of the projected state)
+int contract(qs_state* st, uint64_t i, uint64_t m) {
+    std::vector<uint64_t> nd = st->dims;
+    const uint64_t stride = row_stride(st->dims, i), dim = st->dims[i];
+    nd.erase(nd.begin() + static_cast<std::ptrdiff_t>(i));
+    if (nd.empty()) nd.push_back(1);
+    uint64_t Dn = 1;
+    for (uint64_t d : nd) Dn *= d;
+    const int rho = st->kind == QS_RHO;
+    const uint64_t nout = rho ? Dn * Dn : Dn;
+    void* nbuf = nullptr;
+    QS_HIP(hipMalloc(&nbuf, nout * st->elem_bytes()), "qpp::measure_seq()");
+    if (st->dtype == QS_C128)
+        hipLaunchKernelGGL(k_contract<double2>, dim3(grid_for(nout, 256)), dim3(256), 0,
+                           st->stream, static_cast<const double2*>(st->d),
+                           static_cast<double2*>(nbuf), nout, Dn, st->D, stride, dim, m,
+                           rho);
+    else
+        hipLaunchKernelGGL(k_contract<float2>, dim3(grid_for(nout, 256)), dim3(256), 0,
+                           st->stream, static_cast<const float2*>(st->d),
+                           static_cast<float2*>(nbuf), nout, Dn, st->D, stride, dim, m,
+                           rho);
+    QS_HIP(hipGetLastError(), "qpp::measure_seq()");
+    QS_HIP(hipStreamSynchronize(st->stream), "qpp::measure_seq()");
+    (void)hipFree(st->d);
+    st->d = nbuf;
+    st->dims = nd;
+    st->D = Dn;
+    st->nelem = nout;
+    st->qubits = std::all_of(nd.begin(), nd.end(), [](uint64_t d) { return d == 2; });
+    st->nbits = st->qubits && !(nd.size() == 1 && nd[0] == 1)
+                    ? static_cast<uint32_t>((rho ? 2 : 1) * nd.size())
+                    : 0;
+    if (nd.size() == 1 && nd[0] == 1) st->qubits = false;
+    return QS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qs_norm2(const qs_state* st, double* out) {
+    if (!valid(st) || !out) return fail(QS_ERR_INVALID_HANDLE, "qpp::norm()");
+    double re = 0, im = 0;
+    if (int rc = norm_complex(st, &re, &im)) return rc;
+    *out = re;
+    return QS_OK;
+}
+
+// qpp::normalize, functions.hpp:304-333
+int qs_normalize(qs_state* st) {
+    constexpr const char* W = "qpp::normalize()";
+    if (!valid(st)) return fail(QS_ERR_INVALID_HANDLE, W);
+    double re = 0, im = 0;
+    if (int rc = norm_complex(st, &re, &im)) return rc;
+    if (st->kind == QS_KET) {
+        const double nrm = std::sqrt(re);
+        if (nrm == 0) return fail(QS_ERR_OVERFLOW, W);
+        return div_scalar(st, nrm, 0.0);
+    }
+    if (std::hypot(re, im) == 0) return fail(QS_ERR_OVERFLOW, W);
+    return div_scalar(st, re, im);
+}
+
+int qs_probs(const qs_state* st, double* host_out, uint64_t n_out) {
+    if (!valid(st) || !host_out) return fail(QS_ERR_INVALID_HANDLE, "qpp::abssq()");
+    if (n_out != st->D) return fail(QS_ERR_SIZE_MISMATCH, "qpp::abssq()", "n_out");
+    if (int rc = g_p.need(st->D * 8)) return rc;
+    if (int rc = probs_device(st, static_cast<double*>(g_p.p))) return rc;
+    QS_HIP(hipMemcpyAsync(host_out, g_p.p, st->D * 8, hipMemcpyDeviceToHost, st->stream),
+           "qpp::abssq()");
+    QS_HIP(hipStreamSynchronize(st->stream), "qpp::abssq()");
+    return QS_OK;
+}
+
+// qpp::sample, instruments.hpp:746-929
+int qs_sample(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+              uint64_t nshots, const double* u, uint64_t* out_index,
+              uint64_t* out_digits, double* out_prob) {
+    constexpr const char* W = "qpp::sample()";
+    if (!valid(st)) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    std::vector<uint64_t> tv(target, target + ntarget);
+    if (!check_subsys(tv, st->dims)) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    if (nshots == 0) return fail(QS_ERR_OUT_OF_RANGE, W, "num_samples");
+    if (!u) return fail(QS_ERR_ZERO_SIZE, W, "u");
+    std::vector<uint64_t> idx;
+    if (int rc = sample_device(st, u, nshots, idx)) return rc;
+    if (out_index) std::copy(idx.begin(), idx.end(), out_index);
+    const uint64_t n = st->dims.size();
+    if (out_digits)
+        for (uint64_t s = 0; s < nshots; ++s) {
+            uint64_t rem = idx[s];
+            std::vector<uint64_t> mid(n);
+            for (uint64_t k = n; k-- > 0;) {
+                mid[k] = rem % st->dims[k];
+                rem /= st->dims[k];
+            }
+            for (uint32_t t = 0; t < ntarget; ++t) out_digits[s * ntarget + t] = mid[tv[t]];
+        }
+    if (out_prob) {
+        // p[index] of each shot (instruments.hpp:808)
+        for (uint64_t s = 0; s < nshots; ++s)
+            QS_HIP(hipMemcpyAsync(out_prob + s, static_cast<double*>(g_p.p) + idx[s], 8,
+                                  hipMemcpyDeviceToHost, st->stream),
+                   W);
+        QS_HIP(hipStreamSynchronize(st->stream), W);
+    }
+    return QS_OK;
+}
+
+// qpp::measure_seq(A, target, dims, destructive), instruments.hpp:985-1099
+int qs_measure_seq(qs_state* st, uint32_t ntarget, const uint64_t* target,
+                   int destructive, double u, uint64_t* results, double* probs) {
+    constexpr const char* W = "qpp::measure_seq()";
+    if (!valid(st)) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    std::vector<uint64_t> tv(target, target + ntarget);
+    if (!check_subsys(tv, st->dims)) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    // one sample of all targets
+    std::vector<uint64_t> idx;
+    if (int rc = sample_device(st, &u, 1, idx)) return rc;
+    const uint64_t n0 = st->dims.size();
+    std::vector<uint64_t> mid(n0), res(ntarget);
+    {
+        uint64_t rem = idx[0];
+        for (uint64_t k = n0; k-- > 0;) {
+            mid[k] = rem % st->dims[k];
+            rem /= st->dims[k];
+        }
+        for (uint32_t t = 0; t < ntarget; ++t) res[t] = mid[tv[t]];
+    }
+    std::vector<double> ps(ntarget, 0.0);
+    const bool rho = st->kind == QS_RHO;
+    std::vector<uint32_t> order(ntarget);
+    std::iota(order.begin(), order.end(), 0u);
+    if (rho && n0 <= 10) {
+        // instruments.hpp:1039-1054: project in the given order, then one
+        // ptrace over all targets
+        for (uint32_t cnt = 0; cnt < ntarget; ++cnt) {
+            const uint64_t i = tv[cnt];
+            if (int rc = project(st, row_stride(st->dims, i), st->dims[i], res[cnt])) return rc;
+            double tr = 0, ti = 0;
+            if (int rc = norm_complex(st, &tr, &ti)) return rc;
+            ps[cnt] = tr;
+            if (int rc = div_scalar(st, tr, 0.0)) return rc;
+        }
+        if (destructive) {
+            // remove targets in descending order (each contraction keeps the
+            // measured digit, equal to the ptrace of the projected state)
+            std::vector<uint32_t> desc = order;
+            std::sort(desc.begin(), desc.end(), [&](uint32_t a, uint32_t b) { return tv[a] > tv[b]; });
+            for (uint32_t k : desc)
+                if (int rc = contract(st, tv[k], res[k])) return rc;
+        }
+    } else {
+        if (destructive)
+            std::sort(order.begin(), order.end(),
+                      [&](uint32_t a, uint32_t b) { return tv[a] > tv[b]; });
+        for (uint32_t cnt = 0; cnt < ntarget; ++cnt) {
+            const uint32_t ip = order[cnt];
+            const uint64_t i = tv[ip];
+            if (!rho && destructive) {
+                if (int rc = contract(st, i, res[ip])) return rc;
+                double nr = 0, ni = 0;
+                if (int rc = norm_complex(st, &nr, &ni)) return rc;
+                ps[ip] = nr;
+                if (int rc = div_scalar(st, std::sqrt(nr), 0.0)) return rc;
+            } else {
+                if (int rc = project(st, row_stride(st->dims, i), st->dims[i], res[ip])) return rc;
+                double nr = 0, ni = 0;
+                if (int rc = norm_complex(st, &nr, &ni)) return rc;
+                ps[ip] = nr;
+                if (int rc = div_scalar(st, rho ? nr : std::sqrt(nr), 0.0)) return rc;
+                if (destructive)
+                    if (int rc = contract(st, i, res[ip])) return rc;
+            }
+        }
+    }
+    QS_HIP(hipStreamSynchronize(st->stream), W);
+    for (uint32_t t = 0; t < ntarget; ++t) {
+        if (results) results[t] = res[t];
+        if (probs) probs[t] = ps[t];
+    }
+    return QS_OK;
+}
+
+}  // extern "C"

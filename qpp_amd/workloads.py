@@ -1,0 +1,73 @@
+"""Seeded synthetic workloads (the benchmark configurations).
+
+A circuit is a list of gates (kind, targets, controls, matrix):
+  ("u",  [q],    [],  U 2x2)      dense 1-qubit gate, qpp::apply(psi, U, {q})
+  ("cx", [t],    [c], X)          CNOT, qpp::applyCTRL(psi, X, {c}, {t})
+  ("cz", [t],    [c], Z)          CZ,   qpp::applyCTRL(rho, Z, {c}, {t})
+
+Haar unitaries are QR decompositions of complex Gaussian matrices with the
+phases of R's diagonal divided out (the construction of qpp::randU,
+random.hpp:257-283), drawn from numpy's PCG64 with a fixed seed.
+"""
+import numpy as np
+
+X = np.array([[0, 1], [1, 0]], dtype=complex)
+Z = np.array([[1, 0], [0, -1]], dtype=complex)
+
+
+def haar(d, rng):
+    z = (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def random_layers(n, depth, seed):
+    """Random circuit: `depth` layers of a Haar 1-qubit gate on every qubit
+    (q = 0..n-1) followed by CNOTs on a random perfect pairing
+    (perm[2i] -> perm[2i+1]; with odd n one qubit stays unpaired)."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(depth):
+        for q in range(n):
+            gates.append(("u", [q], [], haar(2, rng)))
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            gates.append(("cx", [int(perm[i + 1])], [int(perm[i])], X))
+    return gates
+
+
+def dm_layers(n, depth, seed):
+    """Density-matrix circuit: `depth` layers of Haar 1-qubit gates on every
+    qubit followed by CZ on adjacent pairs (2i, 2i+1)."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(depth):
+        for q in range(n):
+            gates.append(("u", [q], [], haar(2, rng)))
+        for i in range(0, n - 1, 2):
+            gates.append(("cz", [i + 1], [i], Z))
+    return gates
+
+
+def gate_bytes(gate, n, elem_bytes=16, rho=False):
+    """Algorithmic HBM bytes of one gate applied on its own: every amplitude
+    a dense gate touches is read once and written once; a controlled gate
+    touches the 2^(n-c) control-satisfying amplitudes."""
+    nb = 2 * n if rho else n
+    kind, target, ctrl, _ = gate
+    c = len(ctrl) * (2 if rho else 1)
+    if rho and ctrl:
+        # rows pass and columns pass each touch the control-satisfying half
+        return 2 * 2 * elem_bytes * 2 ** (nb - len(ctrl))
+    if rho:
+        return 2 * 2 * elem_bytes * 2 ** nb  # U on rows, conj(U) on columns
+    return 2 * elem_bytes * 2 ** (nb - c)
+
+
+def apply_gates(st, gates):
+    """Apply a gate list gate by gate on a qpp_amd.State."""
+    for kind, target, ctrl, U in gates:
+        if ctrl:
+            st.apply_ctrl(U, ctrl, target)
+        else:
+            st.apply(U, target)
