@@ -107,6 +107,11 @@ def main():
     ap.add_argument("--mode", choices=["fused", "gates"], default="fused")
     ap.add_argument("--cpu-gates", type=int, default=6,
                     help="gates in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--reg-bits", type=int, default=4)
+    ap.add_argument("--low-bits", type=int, default=5)
+    ap.add_argument("--max-ops", type=int, default=64)
+    ap.add_argument("--sweep", action="store_true",
+                    help="time fused options on the first layers and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -125,10 +130,24 @@ def main():
     st = qa.State([2] * n)
 
     mode = args.mode
-    if mode == "fused" and not hasattr(qa, "Program"):
-        mode = "gates"
+    if args.sweep:
+        sub = gates[: (n + n // 2) * 4]
+        for rb in (3, 4):
+            for low in (5, 6):
+                for mo in (24, 48):
+                    for gc in (512, 1024, 2048, 8192):
+                        p = qa.Program(n, sub, reg_bits=rb, low_bits=low, max_ops=mo, grid_cap=gc)
+                        p.run(st)
+                        ms = profile_region(qa, lambda: p.run(st))
+                        npas = p.num_passes()
+                        gbs = npas * 2 * 16 * 2 ** n / (ms / 1e3) / 1e9
+                        log(f"sweep reg_bits={rb} low_bits={low} max_ops={mo} grid={gc}: "
+                            f"passes={npas} phases={p.info()['phases']} {ms:.1f} ms "
+                            f"({ms / npas:.2f} ms/pass, {gbs:.0f} GB/s per pass, "
+                            f"{len(sub) / ms * 1e3:.0f} gates/s)")
     if mode == "fused":
-        prog = qa.Program(n, gates)
+        prog = qa.Program(n, gates, reg_bits=args.reg_bits, low_bits=args.low_bits,
+                          max_ops=args.max_ops)
         run = lambda: prog.run(st)  # noqa: E731
         npasses = prog.num_passes()
         pass_bytes = npasses * 2 * 16 * 2 ** n

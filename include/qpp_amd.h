@@ -249,6 +249,29 @@ typedef struct qs_circuit {
     const uint64_t* pool;     /* index pool */
 } qs_circuit;
 
+/* Fused programs: the gate steps [step_begin, step_end) of a qubit circuit
+ * compiled into tile passes (each pass streams the state through HBM once
+ * and applies many gates from registers).  Gate steps only (GATE with 1 or
+ * 2 targets, FAN, CTRL / CTRL_FAN with one target per application).  The
+ * same gates applied one qs_apply / qs_apply_ctrl at a time give the same
+ * state up to rounding (the pass order only swaps gates on disjoint
+ * qubits). */
+typedef struct qs_program qs_program;
+typedef struct qs_fuse_opts {
+    uint32_t reg_bits; /* log2(amplitudes per thread): 3 or 4 (tile 2^11 or 2^12) */
+    uint32_t low_bits; /* low index bits kept in every tile (coalescing) */
+    uint32_t max_ops;  /* gates per pass at most */
+    uint32_t grid_cap; /* workgroups per pass launch at most (0 = default) */
+} qs_fuse_opts;
+int qs_program_create(const qs_circuit* qc, uint32_t step_begin,
+                      uint32_t step_end, qs_kind kind, qs_dtype dtype,
+                      const qs_fuse_opts* opts /* NULL: defaults */,
+                      qs_program** out);
+int qs_program_run(qs_program* p, qs_state* st);
+int qs_program_info(const qs_program* p, uint64_t* num_gates,
+                    uint64_t* num_passes, uint64_t* num_phases);
+int qs_program_destroy(qs_program* p);
+
 typedef struct qs_engine qs_engine;
 
 /* Engine over a circuit (a deep copy is taken).  noise_p < 0: QEngine;

@@ -506,3 +506,174 @@ class Rng:
         if r >= 1.0:
             r = math.nextafter(1.0, 0.0)
         return r
+
+
+# ---------------------------------------------------------------------------
+# circuits (qpp::QCircuit lowered to the qs_circuit program format)
+# ---------------------------------------------------------------------------
+STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY = range(7)
+
+
+class _Step(ctypes.Structure):
+    _fields_ = [(f, c_uint32) for f in ("type", "gate", "nctrl", "ntarget", "ctrl_off",
+                                         "target_off", "shift_off", "c_reg", "destructive",
+                                         "reserved")]
+
+
+class _Circuit(ctypes.Structure):
+    _fields_ = [("nq", c_uint32), ("nc", c_uint32), ("d", c_uint32), ("nsteps", c_uint32),
+                ("steps", POINTER(_Step)), ("nmats", c_uint32),
+                ("mat_dim", POINTER(c_uint64)), ("mat_off", POINTER(c_uint64)),
+                ("mats", POINTER(c_double)), ("pool", POINTER(c_uint64))]
+
+
+class QCircuit:
+    """Step list of a qpp::QCircuit (classes/qcircuit.hpp): gate / gate_fan /
+    gate_joint / CTRL / CTRL_fan / measure / nop, in program order."""
+
+    def __init__(self, nq, nc=0, d=2):
+        self.nq, self.nc, self.d = int(nq), int(nc), int(d)
+        self.steps = []       # tuples (type, gate, ctrl, target, shift, c_reg, destructive)
+        self.mats = []
+        self._mat_index = {}
+
+    def _mat(self, U):
+        U = np.asarray(U, dtype=np.complex128)
+        key = U.tobytes()
+        if key not in self._mat_index:
+            self._mat_index[key] = len(self.mats)
+            self.mats.append(U)
+        return self._mat_index[key]
+
+    def gate(self, U, *targets):
+        t = list(targets[0]) if len(targets) == 1 and not np.isscalar(targets[0]) else list(targets)
+        self.steps.append((STEP_GATE, self._mat(U), [], t, None, 0, 0))
+        return self
+
+    gate_joint = gate
+
+    def gate_fan(self, U, targets):
+        self.steps.append((STEP_FAN, self._mat(U), [], list(targets), None, 0, 0))
+        return self
+
+    def CTRL(self, U, ctrl, target, shift=None):
+        c = [ctrl] if np.isscalar(ctrl) else list(ctrl)
+        t = [target] if np.isscalar(target) else list(target)
+        self.steps.append((STEP_CTRL, self._mat(U), c, t, None if shift is None else list(shift), 0, 0))
+        return self
+
+    def CTRL_fan(self, U, ctrl, target, shift=None):
+        c = [ctrl] if np.isscalar(ctrl) else list(ctrl)
+        self.steps.append((STEP_CTRL_FAN, self._mat(U), c, list(target),
+                           None if shift is None else list(shift), 0, 0))
+        return self
+
+    def measure(self, target, c_reg=None, destructive=True):
+        if np.isscalar(target):
+            t = [int(target)]
+            kind = STEP_MEASURE
+        else:
+            t = list(target)
+            kind = STEP_MEASURE_MANY
+        c = t[0] if c_reg is None else int(c_reg)
+        self.steps.append((kind, 0, [], t, None, c, int(bool(destructive))))
+        return self
+
+    def nop(self):
+        self.steps.append((STEP_NOP, 0, [], [], None, 0, 0))
+        return self
+
+    @classmethod
+    def from_gates(cls, n, gates):
+        """From a workloads gate list (kind, targets, controls, matrix)."""
+        qc = cls(n)
+        for kind, target, ctrl, U in gates:
+            if ctrl:
+                qc.CTRL(U, ctrl, target)
+            else:
+                qc.gate(U, target)
+        return qc
+
+    def struct(self):
+        """Build the C structure (arrays kept alive on the returned object)."""
+        pool = []
+        steps = (_Step * max(1, len(self.steps)))()
+        for i, (tp, g, ctrl, tg, shift, creg, dest) in enumerate(self.steps):
+            s = steps[i]
+            s.type, s.gate, s.nctrl, s.ntarget = tp, g, len(ctrl), len(tg)
+            s.ctrl_off = len(pool)
+            pool += ctrl
+            s.target_off = len(pool)
+            pool += tg
+            if shift is None:
+                s.shift_off = 0xFFFFFFFF
+            else:
+                s.shift_off = len(pool)
+                pool += shift
+            s.c_reg, s.destructive, s.reserved = creg, dest, 0
+        pool_a = np.ascontiguousarray(np.asarray(pool + [0], dtype=np.uint64))
+        dims = np.ascontiguousarray(np.asarray([m.shape[0] for m in self.mats] + [0], dtype=np.uint64))
+        offs, off = [], 0
+        for m in self.mats:
+            offs.append(off)
+            off += m.size
+        offs_a = np.ascontiguousarray(np.asarray(offs + [0], dtype=np.uint64))
+        flat = np.ascontiguousarray(np.concatenate(
+            [np.asfortranarray(m).reshape(-1, order="F") for m in self.mats] + [np.zeros(1, complex)]))
+        c = _Circuit()
+        c.nq, c.nc, c.d, c.nsteps = self.nq, self.nc, self.d, len(self.steps)
+        c.steps = steps
+        c.nmats = len(self.mats)
+        c.mat_dim = dims.ctypes.data_as(POINTER(c_uint64))
+        c.mat_off = offs_a.ctypes.data_as(POINTER(c_uint64))
+        c.mats = flat.ctypes.data_as(POINTER(c_double))
+        c.pool = pool_a.ctypes.data_as(POINTER(c_uint64))
+        c._keep = (steps, pool_a, dims, offs_a, flat)
+        return c
+
+
+class _FuseOpts(ctypes.Structure):
+    _fields_ = [("reg_bits", c_uint32), ("low_bits", c_uint32), ("max_ops", c_uint32),
+                ("grid_cap", c_uint32)]
+
+
+class Program:
+    """Gate steps of a circuit compiled into tile-fused passes."""
+
+    def __init__(self, n_or_circuit, gates=None, kind=KET, dtype=C128, reg_bits=4,
+                 low_bits=5, max_ops=64, grid_cap=0, step_begin=0, step_end=None):
+        qc = n_or_circuit if isinstance(n_or_circuit, QCircuit) else QCircuit.from_gates(n_or_circuit, gates)
+        self.circuit = qc
+        self._cs = qc.struct()
+        L = lib()
+        L.qs_program_create.argtypes = [POINTER(_Circuit), c_uint32, c_uint32, c_int, c_int,
+                                        POINTER(_FuseOpts), POINTER(c_void_p)]
+        L.qs_program_run.argtypes = [c_void_p, c_void_p]
+        L.qs_program_info.argtypes = [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]
+        L.qs_program_destroy.argtypes = [c_void_p]
+        o = _FuseOpts(reg_bits, low_bits, max_ops, grid_cap)
+        self._h = c_void_p()
+        end = len(qc.steps) if step_end is None else step_end
+        _check(L.qs_program_create(ctypes.byref(self._cs), step_begin, end, kind, dtype,
+                                   ctypes.byref(o), ctypes.byref(self._h)))
+
+    def run(self, st):
+        _check(lib().qs_program_run(self._h, st.handle))
+
+    def info(self):
+        g, p, h = c_uint64(), c_uint64(), c_uint64()
+        _check(lib().qs_program_info(self._h, byref(g), byref(p), byref(h)))
+        return {"gates": g.value, "passes": p.value, "phases": h.value}
+
+    def num_passes(self):
+        return self.info()["passes"]
+
+    def kernel_name(self):
+        return "k_tile<double,16>"
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().qs_program_destroy(self._h)
+        except Exception:
+            pass

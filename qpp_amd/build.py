@@ -19,6 +19,7 @@ SOURCES = [
     "qs_maps.hip",
     "qs_measure.hip",
     "qs_fused.hip",
+    "qs_program.hip",
     "qs_engine.hip",
 ]
 

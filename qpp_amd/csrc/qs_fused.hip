@@ -1,0 +1,706 @@
+// qs_fused.hip -- tile-fused gate passes (the circuit fast path).
+//
+// A circuit is compiled on the host into passes.  A pass owns a set S of m
+// index bits (the "tile bits": the low L bits of the element index, kept for
+// coalescing, plus the target bits of the pass's gates).  A workgroup of 256
+// threads holds one tile -- the 2^m elements that share all bits outside S --
+// in registers (NR = 2^m / 256 per thread), applies every gate of the pass,
+// and writes the tile back: the state streams through HBM once per pass.
+// Workgroups are persistent and fetch the next tile into registers while
+// they compute the current one.
+//
+// Inside a pass the gates are grouped into phases.  In a phase, r = log2(NR)
+// tile bits are register bits (the NR elements of a thread differ only in
+// them) and the other 8 tile bits index the thread.  Gates on register bits
+// run in VALU without data exchange; changing the register bits is one round
+// trip through LDS.  The host precomputes every uniform quantity of a phase
+// and an op (global offsets, swizzled LDS offsets, the split of a control
+// into register / thread / tile parts), so per element the kernel spends
+// only the gate arithmetic, one XOR per LDS access and one add per global
+// access.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "qs_fused.hpp"
+#include "qs_internal.hpp"
+
+namespace {
+
+using qs::fail;
+
+template <typename R>
+struct cx;
+template <>
+struct cx<double> {
+    using T = double2;
+};
+template <>
+struct cx<float> {
+    using T = float2;
+};
+template <typename R>
+using cxt = typename cx<R>::T;
+
+template <typename R>
+__device__ __forceinline__ cxt<R> mk(R x, R y) {
+    cxt<R> r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+
+// u0 * a + u1 * b
+template <typename R>
+__device__ __forceinline__ cxt<R> mac2(cxt<R> u0, cxt<R> a, cxt<R> u1, cxt<R> b) {
+    cxt<R> r;
+    r.x = u0.x * a.x - u0.y * a.y + u1.x * b.x - u1.y * b.y;
+    r.y = u0.x * a.y + u0.y * a.x + u1.x * b.y + u1.y * b.x;
+    return r;
+}
+
+template <typename R>
+__device__ __forceinline__ cxt<R> cmul(cxt<R> u, cxt<R> a) {
+    return mk<R>(u.x * a.x - u.y * a.y, u.x * a.y + u.y * a.x);
+}
+
+template <typename R>
+__device__ __forceinline__ cxt<R> sel(bool c, cxt<R> a, cxt<R> b) {
+    return mk<R>(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+template <typename R>
+__device__ __forceinline__ cxt<R> opm(const FOp* op, int i) {
+    return mk<R>(static_cast<R>(op->m[2 * i]), static_cast<R>(op->m[2 * i + 1]));
+}
+
+constexpr int kThreads = 256;
+
+template <int NR>
+struct RBof;
+template <>
+struct RBof<8> {
+    static constexpr int v = 3;
+};
+template <>
+struct RBof<16> {
+    static constexpr int v = 4;
+};
+
+// ---------------------------------------------------------------------------
+// ops on the register array x[NR]
+// ---------------------------------------------------------------------------
+template <typename R, int NR, int J, bool CT>
+__device__ __forceinline__ void do_u1(cxt<R>* x, const FOp* op, bool tok) {
+    const cxt<R> u00 = opm<R>(op, 0), u10 = opm<R>(op, 1);
+    const cxt<R> u01 = opm<R>(op, 2), u11 = opm<R>(op, 3);
+    const uint32_t regok = op->regok;
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) {
+        if ((rho >> J) & 1) continue;
+        if (CT && !((regok >> rho) & 1u)) continue;
+        const int r1 = rho | (1 << J);
+        const cxt<R> a = x[rho], b = x[r1];
+        const cxt<R> y0 = mac2<R>(u00, a, u01, b);
+        const cxt<R> y1 = mac2<R>(u10, a, u11, b);
+        x[rho] = CT ? sel<R>(tok, y0, a) : y0;
+        x[r1] = CT ? sel<R>(tok, y1, b) : y1;
+    }
+}
+
+template <typename R, int NR, int J, bool CT>
+__device__ __forceinline__ void do_x(cxt<R>* x, const FOp* op, bool tok) {
+    const uint32_t regok = op->regok;
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) {
+        if ((rho >> J) & 1) continue;
+        if (CT && !((regok >> rho) & 1u)) continue;
+        const int r1 = rho | (1 << J);
+        const cxt<R> a = x[rho], b = x[r1];
+        x[rho] = CT ? sel<R>(tok, b, a) : b;
+        x[r1] = CT ? sel<R>(tok, a, b) : a;
+    }
+}
+
+template <typename R, int NR, int J, bool CT>
+__device__ __forceinline__ void do_diag_r(cxt<R>* x, const FOp* op, bool tok) {
+    const cxt<R> d0 = opm<R>(op, 0), d1 = opm<R>(op, 1);
+    const uint32_t regok = op->regok;
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) {
+        if (CT && !((regok >> rho) & 1u)) continue;
+        const cxt<R> y = cmul<R>(((rho >> J) & 1) ? d1 : d0, x[rho]);
+        x[rho] = CT ? sel<R>(tok, y, x[rho]) : y;
+    }
+}
+
+// diagonal factor d (per lane or uniform) on every element
+template <typename R, int NR, bool CT>
+__device__ __forceinline__ void do_scale(cxt<R>* x, const FOp* op, cxt<R> d, bool tok) {
+    const uint32_t regok = op->regok;
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) {
+        if (CT && !((regok >> rho) & 1u)) continue;
+        const cxt<R> y = cmul<R>(d, x[rho]);
+        x[rho] = CT ? sel<R>(tok, y, x[rho]) : y;
+    }
+}
+
+template <typename R, int NR, int J0, int J1, bool CT>
+__device__ __forceinline__ void do_u2(cxt<R>* x, const FOp* op, const cxt<R>* __restrict__ mats,
+                                      bool tok) {
+    // column-major 4x4; gate index bit 0 <-> register bit J0, bit 1 <-> J1
+    const cxt<R>* M = mats + op->moff;
+    cxt<R> m[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = M[i];
+    const uint32_t regok = op->regok;
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) {
+        if (((rho >> J0) & 1) || ((rho >> J1) & 1)) continue;
+        if (CT && !((regok >> rho) & 1u)) continue;
+        const int i0 = rho, i1 = rho | (1 << J0), i2 = rho | (1 << J1),
+                  i3 = rho | (1 << J0) | (1 << J1);
+        const cxt<R> v[4] = {x[i0], x[i1], x[i2], x[i3]};
+        cxt<R> o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cxt<R> s = mac2<R>(m[r], v[0], m[r + 4], v[1]);
+            const cxt<R> t2 = mac2<R>(m[r + 8], v[2], m[r + 12], v[3]);
+            s.x += t2.x;
+            s.y += t2.y;
+            o[r] = s;
+        }
+        if (CT) {
+            x[i0] = sel<R>(tok, o[0], v[0]);
+            x[i1] = sel<R>(tok, o[1], v[1]);
+            x[i2] = sel<R>(tok, o[2], v[2]);
+            x[i3] = sel<R>(tok, o[3], v[3]);
+        } else {
+            x[i0] = o[0];
+            x[i1] = o[1];
+            x[i2] = o[2];
+            x[i3] = o[3];
+        }
+    }
+}
+
+#define QS_J_CASES(F, ...)                                         \
+    switch (op->j0) {                                              \
+        case 0: F<R, NR, 0, CT>(__VA_ARGS__); break;               \
+        case 1: F<R, NR, 1, CT>(__VA_ARGS__); break;               \
+        case 2: F<R, NR, 2, CT>(__VA_ARGS__); break;               \
+        case 3: if constexpr (NR > 8) F<R, NR, 3, CT>(__VA_ARGS__); break; \
+        default: break;                                            \
+    }
+
+template <typename R, int NR, bool CT>
+__device__ __forceinline__ void run_op_ct(cxt<R>* x, const FOp* op, const cxt<R>* mats,
+                                          uint32_t t, uint64_t base, bool tok) {
+    switch (op->kind) {
+        case OP_U1:
+            QS_J_CASES(do_u1, x, op, tok)
+            break;
+        case OP_X:
+            QS_J_CASES(do_x, x, op, tok)
+            break;
+        case OP_DIAG_R:
+            QS_J_CASES(do_diag_r, x, op, tok)
+            break;
+        case OP_DIAG_T: {
+            const bool one = (t & op->tbit) != 0;
+            const cxt<R> d = sel<R>(one, opm<R>(op, 1), opm<R>(op, 0));
+            do_scale<R, NR, CT>(x, op, d, tok);
+            break;
+        }
+        case OP_DIAG_O: {
+            const cxt<R> d = (base & op->dbit) ? opm<R>(op, 1) : opm<R>(op, 0);
+            do_scale<R, NR, CT>(x, op, d, tok);
+            break;
+        }
+        case OP_U2: {
+            const int key = op->j0 * 8 + op->j1;
+            switch (key) {
+                case 0 * 8 + 1: do_u2<R, NR, 0, 1, CT>(x, op, mats, tok); break;
+                case 0 * 8 + 2: do_u2<R, NR, 0, 2, CT>(x, op, mats, tok); break;
+                case 1 * 8 + 2: do_u2<R, NR, 1, 2, CT>(x, op, mats, tok); break;
+                case 0 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 0, 3, CT>(x, op, mats, tok); break;
+                case 1 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 1, 3, CT>(x, op, mats, tok); break;
+                case 2 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 2, 3, CT>(x, op, mats, tok); break;
+                default: break;
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void run_op(cxt<R>* x, const FOp* op, const cxt<R>* mats,
+                                       uint32_t t, uint64_t base) {
+    if ((base & op->omask) != op->oval) return;  // out-of-tile control, uniform
+    if (op->ctrl) {
+        const bool tok = (t & op->tmask) == op->tval;
+        run_op_ct<R, NR, true>(x, op, mats, t, base, tok);
+    } else {
+        run_op_ct<R, NR, false>(x, op, mats, t, base, true);
+    }
+}
+
+// thread part of a phase's global index / swizzled LDS index
+__device__ __forceinline__ uint64_t thr_g(const FPhase* ph, uint32_t t) {
+    uint64_t g = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if ((t >> j) & 1u) g |= ph->g_thr[j];
+    return g;
+}
+
+__device__ __forceinline__ uint32_t thr_s(const FPhase* ph, uint32_t t) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if ((t >> j) & 1u) s ^= ph->s_thr[j];
+    return s;
+}
+
+// One pass over all tiles of the state.
+template <typename R, int NR>
+__global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
+                                                   const FPass* __restrict__ passes,
+                                                   const FPhase* __restrict__ phases,
+                                                   const FOp* __restrict__ ops,
+                                                   const cxt<R>* __restrict__ mats,
+                                                   int pass_id) {
+    extern __shared__ unsigned char smem_raw[];
+    cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
+    const FPass* __restrict__ ps = passes + pass_id;
+    const uint32_t t = threadIdx.x;
+    const int pb = ps->phase_begin, pe = ps->phase_end, nrest = ps->nrest;
+    const uint64_t ntiles = ps->ntiles;
+    const FPhase* ld = phases + pb;  // load layout (lanes on tile bits 0..5)
+    const uint64_t gt_ld = thr_g(ld, t);
+    auto tile_base = [&](uint64_t tile) {
+        uint64_t base = 0;
+        for (int i = 0; i < nrest; ++i) base |= ((tile >> i) & 1) << ps->rest[i];
+        return base;
+    };
+    cxt<R> x[NR], y[NR];
+    uint64_t tile = blockIdx.x;
+    if (tile < ntiles) {
+        const cxt<R>* p = a + (tile_base(tile) | gt_ld);
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) y[rho] = p[ld->g_reg[rho]];
+    }
+    for (; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = tile_base(tile);
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) x[rho] = y[rho];
+        // software pipeline: fetch the next tile while computing this one
+        const uint64_t next = tile + gridDim.x;
+        if (next < ntiles) {
+            const cxt<R>* p = a + (tile_base(next) | gt_ld);
+#pragma unroll
+            for (int rho = 0; rho < NR; ++rho) y[rho] = p[ld->g_reg[rho]];
+        }
+        const FPhase* cur = ld;
+        for (int pi = pb; pi < pe; ++pi) {
+            const FPhase* ph = phases + pi;
+            if (ph != cur) {
+                // remap through LDS: write with the current layout, read with
+                // the new one (index = swizzled thread part ^ register part)
+                const uint32_t sw = thr_s(cur, t);
+#pragma unroll
+                for (int rho = 0; rho < NR; ++rho) lds[sw ^ cur->s_reg[rho]] = x[rho];
+                __syncthreads();
+                const uint32_t sr = thr_s(ph, t);
+#pragma unroll
+                for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ ph->s_reg[rho]];
+                __syncthreads();
+                cur = ph;
+            }
+            const int ob = ph->op_begin, oe = ph->op_end;
+            for (int o = ob; o < oe; ++o) run_op<R, NR>(x, ops + o, mats, t, base);
+        }
+        if (!cur->coalesced) {
+            const uint32_t sw = thr_s(cur, t);
+#pragma unroll
+            for (int rho = 0; rho < NR; ++rho) lds[sw ^ cur->s_reg[rho]] = x[rho];
+            __syncthreads();
+            const uint32_t sr = thr_s(ld, t);
+#pragma unroll
+            for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ ld->s_reg[rho]];
+            __syncthreads();
+            cur = ld;
+        }
+        cxt<R>* p = a + (base | (cur == ld ? gt_ld : thr_g(cur, t)));
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) p[cur->g_reg[rho]] = x[rho];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host: compiler
+// ---------------------------------------------------------------------------
+namespace qs {
+
+namespace {
+
+using C = std::complex<double>;
+
+// LDS swizzle: the low nibble of the tile index is XOR-ed with the two
+// nibbles above it, so lanes that differ only in high tile bits still hit
+// different banks.  It is linear over XOR, which lets the kernel combine the
+// thread part and the register part of an index with one XOR.
+uint32_t swz(uint32_t e) { return e ^ (((e >> 4) ^ (e >> 8)) & 15u); }
+
+// 4x4 column-major with gate digits (hi, lo) swapped
+std::vector<C> swap_digits_4x4(const std::vector<C>& m) {
+    auto sw = [](int i) { return ((i & 1) << 1) | ((i >> 1) & 1); };
+    std::vector<C> r(16);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r[sw(i) + 4 * sw(j)] = m[i + 4 * j];
+    return r;
+}
+
+bool is_x(const std::vector<C>& m) {
+    return m.size() == 4 && m[0] == C(0, 0) && m[3] == C(0, 0) && m[1] == C(1, 0) &&
+           m[2] == C(1, 0);
+}
+
+bool is_diag(const std::vector<C>& m) {
+    return m.size() == 4 && m[1] == C(0, 0) && m[2] == C(0, 0);
+}
+
+// Greedy capacity scheduling: walks `items` in order and takes every item
+// whose bits fit (with the bits already taken) into `cap` bits and that does
+// not touch a blocked qubit; items that do not fit block all their qubits so
+// later items on them keep their order.  Returns the indices taken.
+struct Item {
+    std::vector<uint32_t> need;   // bits that must be in the set
+    std::vector<uint32_t> touch;  // every bit the item depends on
+};
+
+std::vector<size_t> greedy(const std::vector<Item>& items, const std::vector<size_t>& todo,
+                           std::set<uint32_t>& taken, size_t cap, size_t max_take) {
+    std::set<uint32_t> blocked;
+    std::vector<size_t> out;
+    for (size_t id : todo) {
+        const Item& it = items[id];
+        bool blk = false;
+        for (uint32_t b : it.touch)
+            if (blocked.count(b)) blk = true;
+        if (!blk && out.size() < max_take) {
+            std::set<uint32_t> u = taken;
+            for (uint32_t b : it.need) u.insert(b);
+            if (u.size() <= cap) {
+                taken = u;
+                out.push_back(id);
+                continue;
+            }
+        }
+        for (uint32_t b : it.touch) blocked.insert(b);
+    }
+    return out;
+}
+
+// Layout of a phase: register tile bits R (ascending; register bit k is
+// R[k]); the thread number enumerates the other tile bits in ascending
+// order.
+FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
+                  const std::vector<uint32_t>& Sv) {
+    FPhase ph{};
+    std::vector<uint32_t> Rv(Rset.begin(), Rset.end());
+    ph.coalesced = 1;
+    for (uint32_t k = 0; k < RB; ++k) {
+        ph.regbits[k] = static_cast<uint8_t>(Rv[k]);
+        if (Rv[k] < 6) ph.coalesced = 0;
+    }
+    const uint32_t NR = 1u << RB;
+    for (uint32_t rho = 0; rho < NR; ++rho) {
+        uint64_t g = 0;
+        uint32_t e = 0;
+        for (uint32_t k = 0; k < RB; ++k)
+            if ((rho >> k) & 1) {
+                g |= uint64_t{1} << Sv[Rv[k]];
+                e |= 1u << Rv[k];
+            }
+        ph.g_reg[rho] = g;
+        ph.s_reg[rho] = swz(e);
+    }
+    uint32_t j = 0;
+    for (uint32_t b = 0; b < m; ++b)
+        if (!Rset.count(b)) {
+            ph.tbits[j] = static_cast<uint8_t>(b);
+            ph.g_thr[j] = uint64_t{1} << Sv[b];
+            ph.s_thr[j] = swz(1u << b);
+            ++j;
+        }
+    return ph;
+}
+
+// Lowers one gate into an op of the given phase layout.
+FOp lower_op(const BitGate& g, const FPhase& ph, uint32_t RB, const std::vector<uint32_t>& Sv,
+             std::vector<C>& mats) {
+    FOp op{};
+    // where each global bit of the tile lives in this layout
+    std::map<uint32_t, int> reg_of, thr_of;
+    for (uint32_t k = 0; k < RB; ++k) reg_of[Sv[ph.regbits[k]]] = static_cast<int>(k);
+    for (uint32_t j = 0; j < 8; ++j) thr_of[Sv[ph.tbits[j]]] = static_cast<int>(j);
+    const uint32_t NR = 1u << RB;
+    // control split
+    uint32_t regok = 0;
+    for (uint32_t rho = 0; rho < NR; ++rho) {
+        bool ok = true;
+        for (auto& kv : reg_of) {
+            const uint64_t bit = uint64_t{1} << kv.first;
+            if (!(g.cmask & bit)) continue;
+            const bool want = (g.cval & bit) != 0;
+            if ((((rho >> kv.second) & 1) != 0) != want) ok = false;
+        }
+        if (ok) regok |= 1u << rho;
+    }
+    op.regok = regok;
+    for (auto& kv : thr_of) {
+        const uint64_t bit = uint64_t{1} << kv.first;
+        if (!(g.cmask & bit)) continue;
+        op.tmask |= 1u << kv.second;
+        if (g.cval & bit) op.tval |= 1u << kv.second;
+    }
+    uint64_t in_tile = 0;
+    for (uint32_t s : Sv) in_tile |= uint64_t{1} << s;
+    op.omask = g.cmask & ~in_tile;
+    op.oval = g.cval & op.omask;
+    op.ctrl = (regok != (NR == 32 ? 0xFFFFFFFFu : ((1u << NR) - 1))) || op.tmask != 0;
+    auto put = [&](int i, C v) {
+        op.m[2 * i] = v.real();
+        op.m[2 * i + 1] = v.imag();
+    };
+    if (g.bits.size() == 1 && is_diag(g.m)) {
+        const uint32_t b = g.bits[0];
+        put(0, g.m[0]);
+        put(1, g.m[3]);
+        if (reg_of.count(b)) {
+            op.kind = OP_DIAG_R;
+            op.j0 = reg_of[b];
+        } else if (thr_of.count(b)) {
+            op.kind = OP_DIAG_T;
+            op.tbit = 1u << thr_of[b];
+        } else {
+            op.kind = OP_DIAG_O;
+            op.dbit = uint64_t{1} << b;
+        }
+    } else if (g.bits.size() == 1) {
+        op.j0 = reg_of.at(g.bits[0]);
+        if (is_x(g.m)) {
+            op.kind = OP_X;
+        } else {
+            op.kind = OP_U1;
+            for (int i = 0; i < 4; ++i) put(i, g.m[i]);
+        }
+    } else {
+        // bits[0] is the MSB of the gate index; kernel: LSB on j0 < j1
+        const int r0 = reg_of.at(g.bits[0]), r1 = reg_of.at(g.bits[1]);
+        op.kind = OP_U2;
+        op.moff = static_cast<uint32_t>(mats.size());
+        std::vector<C> mm = g.m;
+        if (r1 < r0) {
+            op.j0 = r1;
+            op.j1 = r0;
+        } else {
+            op.j0 = r0;
+            op.j1 = r1;
+            mm = swap_digits_4x4(mm);
+        }
+        mats.insert(mats.end(), mm.begin(), mm.end());
+    }
+    return op;
+}
+
+}  // namespace
+
+int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOptions& opt,
+                  FusedProgram& P) {
+    P = FusedProgram{};
+    P.nb = nb;
+    const uint32_t RB = opt.reg_bits;  // 3 or 4
+    const uint32_t m = 8 + RB;
+    if (nb < m) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::fused", "state below tile size");
+    const uint32_t L = std::min<uint32_t>(opt.low_bits, m);
+    // scheduling items: the bits a gate needs in the tile (none for a
+    // diagonal 1-bit gate) and every bit it depends on
+    std::vector<Item> items(gates.size());
+    for (size_t i = 0; i < gates.size(); ++i) {
+        const BitGate& g = gates[i];
+        Item& it = items[i];
+        const bool diag = g.bits.size() == 1 && is_diag(g.m);
+        if (!diag) it.need = g.bits;
+        it.touch = g.bits;
+        for (uint32_t b = 0; b < 64; ++b)
+            if ((g.cmask >> b) & 1) it.touch.push_back(b);
+    }
+    std::vector<size_t> todo(gates.size());
+    for (size_t i = 0; i < todo.size(); ++i) todo[i] = i;
+    std::vector<C> mats;
+    while (!todo.empty()) {
+        std::set<uint32_t> S;
+        for (uint32_t b = 0; b < L; ++b) S.insert(b);
+        std::vector<size_t> take = greedy(items, todo, S, m, opt.max_ops);
+        if (take.empty()) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::fused", "gate wider than a tile");
+        // fill S up to m bits with the lowest unused bits
+        for (uint32_t b = 0; S.size() < m && b < nb; ++b) S.insert(b);
+        FPass ps{};
+        ps.m = static_cast<int>(m);
+        std::vector<uint32_t> Sv(S.begin(), S.end());
+        for (uint32_t i = 0; i < m; ++i) ps.S[i] = static_cast<uint8_t>(Sv[i]);
+        std::vector<uint32_t> rest;
+        for (uint32_t b = 0; b < nb; ++b)
+            if (!S.count(b)) rest.push_back(b);
+        ps.nrest = static_cast<int>(rest.size());
+        for (size_t i = 0; i < rest.size(); ++i) ps.rest[i] = static_cast<uint8_t>(rest[i]);
+        ps.ntiles = uint64_t{1} << rest.size();
+        auto tile_bit = [&](uint32_t g) {
+            return static_cast<uint32_t>(std::find(Sv.begin(), Sv.end(), g) - Sv.begin());
+        };
+        // phases: greedy over the pass's gates with capacity RB register bits
+        std::vector<Item> pit(take.size());
+        for (size_t i = 0; i < take.size(); ++i) {
+            const Item& it = items[take[i]];
+            for (uint32_t b : it.need) pit[i].need.push_back(tile_bit(b));
+            pit[i].touch = it.touch;  // dependencies on global bits
+        }
+        std::vector<size_t> ptodo(take.size());
+        for (size_t i = 0; i < ptodo.size(); ++i) ptodo[i] = i;
+        ps.phase_begin = static_cast<int>(P.phases.size());
+        // When a phase's register bits avoid tile bits 0..5 its lanes sit on
+        // tile bits 0..5 and HBM access coalesces: the first phase can load
+        // and the last can store without an LDS round trip.  Otherwise a
+        // load phase (top RB tile bits, no ops) comes first and the kernel
+        // returns to it before storing.
+        bool first = true;
+        while (!ptodo.empty()) {
+            std::set<uint32_t> R;
+            std::vector<size_t> ptake = greedy(pit, ptodo, R, RB, SIZE_MAX);
+            // complete R with unused tile bits, preferring high ones
+            for (int b = int(m) - 1; R.size() < RB && b >= 0; --b) R.insert(uint32_t(b));
+            FPhase ph = make_phase(R, m, RB, Sv);
+            if (first && !ph.coalesced) {
+                std::set<uint32_t> R0;
+                for (uint32_t k = 0; k < RB; ++k) R0.insert(m - RB + k);
+                FPhase ldp = make_phase(R0, m, RB, Sv);
+                ldp.op_begin = ldp.op_end = static_cast<int>(P.ops.size());
+                P.phases.push_back(ldp);
+            }
+            ph.op_begin = static_cast<int>(P.ops.size());
+            for (size_t pi : ptake) P.ops.push_back(lower_op(gates[take[pi]], ph, RB, Sv, mats));
+            ph.op_end = static_cast<int>(P.ops.size());
+            P.phases.push_back(ph);
+            std::vector<size_t> rem;
+            std::set<size_t> tk(ptake.begin(), ptake.end());
+            for (size_t i : ptodo)
+                if (!tk.count(i)) rem.push_back(i);
+            ptodo.swap(rem);
+            first = false;
+        }
+        ps.phase_end = static_cast<int>(P.phases.size());
+        P.passes.push_back(ps);
+        P.gates_per_pass.push_back(take.size());
+        std::vector<size_t> rem;
+        std::set<size_t> tk(take.begin(), take.end());
+        for (size_t i : todo)
+            if (!tk.count(i)) rem.push_back(i);
+        todo.swap(rem);
+    }
+    P.mats = mats;
+    P.reg_bits = RB;
+    return QS_OK;
+}
+
+// upload program tables
+int fused_upload(FusedProgram& P, qs_dtype dtype) {
+    fused_free(P);
+    const size_t pb = P.passes.size() * sizeof(FPass);
+    const size_t hb = P.phases.size() * sizeof(FPhase);
+    const size_t ob = std::max<size_t>(1, P.ops.size()) * sizeof(FOp);
+    const size_t es = dtype == QS_C128 ? 16 : 8;
+    const size_t mb = std::max<size_t>(1, P.mats.size()) * es;
+    std::vector<unsigned char> blob(pb + hb + ob + mb + 64);
+    size_t off = 0;
+    auto put = [&](const void* src, size_t n) {
+        std::memcpy(blob.data() + off, src, n);
+        const size_t o = off;
+        off += (n + 15) / 16 * 16;
+        return o;
+    };
+    const size_t o_p = put(P.passes.data(), pb);
+    const size_t o_h = put(P.phases.data(), hb);
+    std::vector<FOp> opsv = P.ops;
+    if (opsv.empty()) opsv.resize(1);
+    const size_t o_o = put(opsv.data(), ob);
+    size_t o_m;
+    if (dtype == QS_C128) {
+        std::vector<double2> mm(std::max<size_t>(1, P.mats.size()));
+        for (size_t i = 0; i < P.mats.size(); ++i) mm[i] = make_double2(P.mats[i].real(), P.mats[i].imag());
+        o_m = put(mm.data(), mm.size() * 16);
+    } else {
+        std::vector<float2> mm(std::max<size_t>(1, P.mats.size()));
+        for (size_t i = 0; i < P.mats.size(); ++i)
+            mm[i] = make_float2(static_cast<float>(P.mats[i].real()), static_cast<float>(P.mats[i].imag()));
+        o_m = put(mm.data(), mm.size() * 8);
+    }
+    QS_HIP(hipMalloc(&P.dblob, blob.size()), "qpp_amd::fused");
+    QS_HIP(hipMemcpy(P.dblob, blob.data(), blob.size(), hipMemcpyHostToDevice), "qpp_amd::fused");
+    auto* b = static_cast<unsigned char*>(P.dblob);
+    P.d_passes = reinterpret_cast<FPass*>(b + o_p);
+    P.d_phases = reinterpret_cast<FPhase*>(b + o_h);
+    P.d_ops = reinterpret_cast<FOp*>(b + o_o);
+    P.d_mats = b + o_m;
+    P.dtype = dtype;
+    return QS_OK;
+}
+
+void fused_free(FusedProgram& P) {
+    if (P.dblob) (void)hipFree(P.dblob);
+    P.dblob = nullptr;
+}
+
+template <typename R, int NR>
+int launch_pass(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap) {
+    const FPass& ps = P.passes[pass];
+    const size_t shmem = sizeof(cxt<R>) << ps.m;
+    uint64_t grid = ps.ntiles;
+    if (grid > grid_cap) grid = grid_cap;
+    hipLaunchKernelGGL((k_tile<R, NR>), dim3(static_cast<uint32_t>(grid)), dim3(kThreads), shmem,
+                       st->stream, static_cast<cxt<R>*>(st->d), P.d_passes, P.d_phases,
+                       P.d_ops, reinterpret_cast<const cxt<R>*>(P.d_mats), pass);
+    return QS_OK;
+}
+
+int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap) {
+    if (!P.dblob) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::fused", "program not uploaded");
+    if (st->dtype != P.dtype) return fail(QS_ERR_TYPE_MISMATCH, "qpp_amd::fused", "dtype");
+    if (!st->qubits || st->nbits != P.nb)
+        return fail(QS_ERR_DIMS_MISMATCH_CVECTOR, "qpp_amd::fused", "state/program");
+    for (size_t p = 0; p < P.passes.size(); ++p) {
+        int rc;
+        if (st->dtype == QS_C128)
+            rc = P.reg_bits == 3 ? launch_pass<double, 8>(P, st, int(p), grid_cap)
+                                 : launch_pass<double, 16>(P, st, int(p), grid_cap);
+        else
+            rc = P.reg_bits == 3 ? launch_pass<float, 8>(P, st, int(p), grid_cap)
+                                 : launch_pass<float, 16>(P, st, int(p), grid_cap);
+        if (rc) return rc;
+    }
+    QS_HIP(hipGetLastError(), "qpp_amd::fused");
+    return QS_OK;
+}
+
+}  // namespace qs

@@ -1,0 +1,100 @@
+// qs_fused.hpp -- tables of the tile-fused pass engine (see qs_fused.hip).
+#ifndef QPP_AMD_QS_FUSED_HPP_
+#define QPP_AMD_QS_FUSED_HPP_
+
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/qpp_amd.h"
+
+enum FOpKind : int {
+    OP_U1 = 0,      // 2x2 dense on register bit j0
+    OP_X = 1,       // swap on register bit j0 (X / CNOT target)
+    OP_DIAG_R = 2,  // diag(d0, d1) on register bit j0
+    OP_DIAG_T = 3,  // diag(d0, d1) on a thread-number bit (tbit)
+    OP_DIAG_O = 4,  // diag(d0, d1) on a bit outside the tile (dbit)
+    OP_U2 = 5       // 4x4 dense on register bits j0 < j1 (j1 = gate MSB)
+};
+
+// One gate of a phase.  Its control (global mask, value) is split by where
+// the control bits live in the phase's layout:
+//   register bits -> regok: bit rho set when register group rho satisfies it
+//   thread bits   -> (t & tmask) == tval on the thread number t
+//   other bits    -> (tile base & omask) == oval, uniform per tile
+struct FOp {
+    int kind;
+    int j0, j1;
+    uint32_t regok;
+    uint32_t tmask, tval;
+    uint32_t tbit;
+    uint32_t moff;      // OP_U2: matrix offset (complex elements)
+    uint64_t omask, oval;
+    uint64_t dbit;      // OP_DIAG_O: global bit
+    int ctrl;           // 1 when regok / tmask restrict the op per element
+    int pad;
+    double m[8];        // U1: u00 u10 u01 u11 (re, im); DIAG: d0 d1
+};
+
+constexpr int kMaxNR = 16;
+
+struct FPhase {
+    uint64_t g_thr[8];            // global index bit of thread-number bit j
+    uint64_t g_reg[kMaxNR];       // global index offset of register group rho
+    uint32_t s_thr[8];            // swizzled LDS index bit of thread-number bit j
+    uint32_t s_reg[kMaxNR];       // swizzled LDS index of register group rho
+    uint8_t regbits[8];           // tile bits held in registers
+    uint8_t tbits[16];            // tile bits indexed by the thread number
+    int op_begin, op_end;
+    int coalesced;                // lanes on tile bits 0..5: loads/stores coalesce
+    int pad;
+};
+
+struct FPass {
+    int m;               // tile bits
+    int nrest;           // bits outside the tile
+    uint8_t S[16];       // tile bit i -> global bit S[i] (ascending)
+    uint8_t rest[64];    // global bits outside the tile (ascending)
+    uint64_t ntiles;
+    int phase_begin, phase_end;
+};
+
+// A gate at bit level: dense matrix on bits (bits[0] = MSB of the gate
+// index), column-major, with a control (mask, value) on global bits.
+struct BitGate {
+    std::vector<uint32_t> bits;
+    std::vector<std::complex<double>> m;
+    uint64_t cmask = 0, cval = 0;
+};
+
+struct FusedOptions {
+    uint32_t reg_bits = 4;  // log2(elements per thread): tile = 2^(8 + reg_bits)
+    uint32_t low_bits = 6;  // low index bits every tile keeps (coalescing)
+    size_t max_ops = 64;    // gates per pass at most (balances VALU vs HBM)
+};
+
+struct FusedProgram {
+    uint32_t nb = 0;
+    uint32_t reg_bits = 4;
+    std::vector<FPass> passes;
+    std::vector<FPhase> phases;
+    std::vector<FOp> ops;
+    std::vector<std::complex<double>> mats;  // OP_U2 matrices
+    std::vector<size_t> gates_per_pass;
+    qs_dtype dtype = QS_C128;
+    void* dblob = nullptr;
+    FPass* d_passes = nullptr;
+    FPhase* d_phases = nullptr;
+    FOp* d_ops = nullptr;
+    void* d_mats = nullptr;
+};
+
+namespace qs {
+int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOptions& opt,
+                  FusedProgram& P);
+int fused_upload(FusedProgram& P, qs_dtype dtype);
+void fused_free(FusedProgram& P);
+int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap);
+}  // namespace qs
+
+#endif  // QPP_AMD_QS_FUSED_HPP_

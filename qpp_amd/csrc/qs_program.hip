@@ -1,0 +1,141 @@
+// qs_program.hip -- C ABI of fused programs: lowering of circuit gate steps
+// (qpp::QCircuit gate / CTRL / FAN steps, classes/qcircuit.hpp:1166-2372,
+// executed by QEngineT::visit_gate_step_, classes/qengine.hpp:533-579) to
+// bit-level gates, compiled into tile passes by qs_fused.hip.
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <vector>
+
+#include "qs_fused.hpp"
+#include "qs_internal.hpp"
+
+struct qs_program {
+    FusedProgram P;
+    uint64_t ngates = 0;
+    uint32_t grid_cap = 2048;
+};
+
+namespace {
+
+using qs::fail;
+using C = std::complex<double>;
+
+std::vector<C> mat_of(const qs_circuit* qc, uint32_t g, bool conj) {
+    const uint64_t d = qc->mat_dim[g];
+    const double* p = qc->mats + 2 * qc->mat_off[g];
+    std::vector<C> m(d * d);
+    for (uint64_t i = 0; i < d * d; ++i) m[i] = C(p[2 * i], conj ? -p[2 * i + 1] : p[2 * i + 1]);
+    return m;
+}
+
+}  // namespace
+
+namespace qs {
+
+// Lowers gate steps to bit gates: a ket over n qubits uses bit n-1-q; a
+// density matrix applies U on the row bits and conj(U) on the column bits
+// (bit 2n-1-q), each under the controls of its own index.
+int lower_steps(const qs_circuit* qc, uint32_t b, uint32_t e, bool rho,
+                std::vector<BitGate>& out) {
+    const uint32_t n = qc->nq;
+    auto bit = [&](uint64_t q, bool col) { return static_cast<uint32_t>((col ? 2 * n : n) - 1 - q); };
+    for (uint32_t s = b; s < e; ++s) {
+        const qs_step& st = qc->steps[s];
+        if (st.type == QS_STEP_NOP) continue;
+        if (st.type == QS_STEP_MEASURE || st.type == QS_STEP_MEASURE_MANY)
+            return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::program", "measurement step in a program");
+        const uint64_t* tg = qc->pool + st.target_off;
+        const uint64_t* cg = qc->pool + st.ctrl_off;
+        const uint64_t* sh = st.shift_off == UINT32_MAX ? nullptr : qc->pool + st.shift_off;
+        const bool fan = st.type == QS_STEP_FAN || st.type == QS_STEP_CTRL_FAN;
+        const bool ctrl = st.type == QS_STEP_CTRL || st.type == QS_STEP_CTRL_FAN;
+        std::vector<std::vector<uint64_t>> groups;
+        if (fan)
+            for (uint32_t i = 0; i < st.ntarget; ++i) groups.push_back({tg[i]});
+        else
+            groups.push_back(std::vector<uint64_t>(tg, tg + st.ntarget));
+        for (const auto& g : groups) {
+            if (g.size() > 2)
+                return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::program", "gate on more than 2 qubits");
+            for (int pass = 0; pass < (rho ? 2 : 1); ++pass) {
+                const bool col = pass == 1;
+                BitGate bg;
+                for (uint64_t q : g) bg.bits.push_back(bit(q, col));
+                bg.m = mat_of(qc, st.gate, col);
+                if (ctrl)
+                    for (uint32_t c = 0; c < st.nctrl; ++c) {
+                        const uint64_t m = uint64_t{1} << bit(cg[c], col);
+                        bg.cmask |= m;
+                        if (!sh || sh[c] == 0) bg.cval |= m;
+                    }
+                out.push_back(bg);
+            }
+        }
+    }
+    return QS_OK;
+}
+
+}  // namespace qs
+
+extern "C" {
+
+int qs_program_create(const qs_circuit* qc, uint32_t step_begin, uint32_t step_end,
+                      qs_kind kind, qs_dtype dtype, const qs_fuse_opts* opts,
+                      qs_program** out) {
+    constexpr const char* W = "qpp_amd::program";
+    if (!qc || !out) return fail(QS_ERR_INVALID_HANDLE, W, "circuit");
+    *out = nullptr;
+    if (qc->d != 2) return fail(QS_ERR_NOT_IMPLEMENTED, W, "qudit circuit");
+    if (step_end > qc->nsteps || step_begin > step_end)
+        return fail(QS_ERR_OUT_OF_RANGE, W, "steps");
+    if (int rc = qs::ensure_device()) return rc;
+    std::vector<BitGate> gates;
+    if (int rc = qs::lower_steps(qc, step_begin, step_end, kind == QS_RHO, gates)) return rc;
+    FusedOptions o;
+    uint32_t cap = 2048;  // resident workgroups walk the tiles
+    if (opts) {
+        if (opts->reg_bits) o.reg_bits = opts->reg_bits;
+        if (opts->low_bits) o.low_bits = opts->low_bits;
+        if (opts->max_ops) o.max_ops = opts->max_ops;
+        if (opts->grid_cap) cap = opts->grid_cap;
+    }
+    if (o.reg_bits != 3 && o.reg_bits != 4) return fail(QS_ERR_OUT_OF_RANGE, W, "reg_bits");
+    auto* p = new qs_program;
+    p->ngates = gates.size();
+    p->grid_cap = cap;
+    const uint32_t nb = (kind == QS_RHO ? 2 : 1) * qc->nq;
+    if (int rc = qs::fused_compile(gates, nb, o, p->P)) {
+        delete p;
+        return rc;
+    }
+    if (int rc = qs::fused_upload(p->P, dtype)) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return QS_OK;
+}
+
+int qs_program_run(qs_program* p, qs_state* st) {
+    if (!p || !st || !st->d) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
+    return qs::fused_run(p->P, st, p->grid_cap);
+}
+
+int qs_program_info(const qs_program* p, uint64_t* num_gates, uint64_t* num_passes,
+                    uint64_t* num_phases) {
+    if (!p) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
+    if (num_gates) *num_gates = p->ngates;
+    if (num_passes) *num_passes = p->P.passes.size();
+    if (num_phases) *num_phases = p->P.phases.size();
+    return QS_OK;
+}
+
+int qs_program_destroy(qs_program* p) {
+    if (!p) return QS_OK;
+    qs::fused_free(p->P);
+    delete p;
+    return QS_OK;
+}
+
+}  // extern "C"

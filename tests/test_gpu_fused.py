@@ -1,0 +1,133 @@
+"""Tile-fused programs vs the oracle applied gate by gate."""
+import numpy as np
+import pytest
+
+from helpers import CNOT, CZ, H, X, Y, Z, haar, randket, randrho
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def maxabs(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+def oracle_run(orc, state, qc):
+    """Apply a QCircuit's gate steps with the oracle, step by step."""
+    s = state
+    for tp, g, ctrl, tg, shift, creg, dest in qc.steps:
+        U = qc.mats[g]
+        if tp == 1:  # GATE
+            s = orc.apply(s, U, tg, 2)
+        elif tp == 2:  # FAN
+            for t in tg:
+                s = orc.apply(s, U, [t], 2)
+        elif tp == 3:
+            s = orc.applyCTRL(s, U, ctrl, tg, 2, shift)
+        elif tp == 4:
+            s = orc.applyCTRL_fan(s, U, ctrl, tg, 2, shift)
+    return s
+
+
+def mixed_circuit(qa, n, depth, seed):
+    rng = np.random.default_rng(seed)
+    qc = qa.QCircuit(n)
+    T = np.diag([1, np.exp(1j * np.pi / 4)])
+    for _ in range(depth):
+        for q in range(n):
+            r = rng.random()
+            if r < 0.6:
+                qc.gate(haar(2, rng), q)
+            elif r < 0.7:
+                qc.gate(T, q)
+            elif r < 0.8:
+                qc.gate(H, q)
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            a, b = int(perm[i]), int(perm[i + 1])
+            r = rng.random()
+            if r < 0.4:
+                qc.CTRL(X, a, b)
+            elif r < 0.55:
+                qc.CTRL(Z, a, b)
+            elif r < 0.7:
+                qc.gate(haar(4, rng), a, b)
+            elif r < 0.8:
+                qc.gate(CZ, a, b)
+            elif r < 0.9:
+                c2 = int(perm[(i + 2) % n])
+                if c2 not in (a, b):
+                    qc.CTRL(haar(2, rng), [a, c2], b, shift=[int(rng.integers(2)), int(rng.integers(2))])
+            else:
+                qc.CTRL_fan(haar(2, rng), a, [b] + ([int(perm[(i + 3) % n])] if int(perm[(i + 3) % n]) not in (a, b) else []))
+        if rng.random() < 0.3:
+            qc.gate_fan(haar(2, rng), [int(x) for x in rng.permutation(n)[:3]])
+    return qc
+
+
+@pytest.mark.parametrize("n,reg", [(12, 4), (12, 3), (14, 4), (17, 4), (20, 3)])
+def test_fused_random_layers(qa, orc, n, reg):
+    from qpp_amd import workloads
+    gates = workloads.random_layers(n, 4, seed=n)
+    qc = qa.QCircuit.from_gates(n, gates)
+    rng = np.random.default_rng(n)
+    psi = randket(2 ** n, rng)
+    st = qa.State.from_array(psi, [2] * n)
+    prog = qa.Program(qc, reg_bits=reg)
+    info = prog.info()
+    assert info["gates"] == len(gates)
+    prog.run(st)
+    ref = oracle_run(orc, psi, qc)
+    err = maxabs(st.download(), ref)
+    print(f"n={n} reg={reg} {info} err={err:.2e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("n", [12, 13, 16])
+def test_fused_mixed_gates(qa, orc, n):
+    qc = mixed_circuit(qa, n, 3, seed=100 + n)
+    rng = np.random.default_rng(n + 1)
+    psi = randket(2 ** n, rng)
+    st = qa.State.from_array(psi, [2] * n)
+    qa.Program(qc).run(st)
+    assert maxabs(st.download(), oracle_run(orc, psi, qc)) <= TOL
+
+
+@pytest.mark.parametrize("n", [6, 7, 8])
+def test_fused_rho(qa, orc, n):
+    qc = mixed_circuit(qa, n, 2, seed=200 + n)
+    rng = np.random.default_rng(n + 2)
+    rho = randrho(2 ** n, rng)
+    st = qa.State.from_array(rho, [2] * n)
+    qa.Program(qc, kind=qa.RHO).run(st)
+    assert maxabs(st.download(), oracle_run(orc, rho, qc)) <= TOL
+
+
+def test_fused_low_bits_variants(qa, orc):
+    n = 15
+    qc = mixed_circuit(qa, n, 3, seed=7)
+    rng = np.random.default_rng(3)
+    psi = randket(2 ** n, rng)
+    ref = oracle_run(orc, psi, qc)
+    for low, maxops in ((0, 64), (3, 8), (6, 1), (5, 200)):
+        st = qa.State.from_array(psi, [2] * n)
+        qa.Program(qc, low_bits=low, max_ops=maxops).run(st)
+        assert maxabs(st.download(), ref) <= TOL, (low, maxops)
+
+
+def test_fused_c64(qa, orc):
+    from qpp_amd import workloads
+    n = 16
+    gates = workloads.random_layers(n, 6, seed=5)
+    qc = qa.QCircuit.from_gates(n, gates)
+    psi = np.zeros(2 ** n, dtype=complex)
+    psi[0] = 1
+    st = qa.State.from_array(psi, [2] * n, dtype=qa.C64)
+    qa.Program(qc, dtype=qa.C64).run(st)
+    ref = oracle_run(orc, psi, qc)
+    out = st.download().astype(np.complex128)
+    # FP32 rounding over ~100 gates: amplitude error ~1e-7, infidelity ~1e-8
+    # (norm drift included); the complex64 path's bar is 1e-5
+    infid = 1 - abs(np.vdot(ref, out)) ** 2
+    assert maxabs(out, ref) <= 5e-6 and infid <= 1e-6
