@@ -274,13 +274,15 @@ int qs_program_destroy(qs_program* p);
 
 typedef struct qs_engine qs_engine;
 
-/* Engine over a circuit (a deep copy is taken).  noise_p < 0: QEngine;
+/* Engine over a circuit (a deep copy is taken).  kind QS_KET is
+ * QEngine (QEngineT<ket>), QS_RHO is QEngineT<cmat>.  noise_p < 0: no noise;
  * noise_p >= 0: QNoisyEngine<QubitDepolarizingNoise(noise_p)>
  * (classes/qengine.hpp:77-1516, classes/qnoisy_engine.hpp:59-178,
  * classes/noise.hpp:388-408).  The engine's std::mt19937 is seeded with
- * `seed`; it draws uniforms exactly as qpp's engines do. */
-int qs_engine_create(const qs_circuit* qc, qs_dtype dtype, double noise_p,
-                     uint64_t seed, qs_engine** out);
+ * `seed`; it draws uniforms exactly as qpp's engines do (same number, same
+ * order, same use). */
+int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype,
+                     double noise_p, uint64_t seed, qs_engine** out);
 int qs_engine_destroy(qs_engine* e);
 /* QEngineT::execute(reps)  qengine.hpp:1341-1405 (QNoisyEngine: 129-140). */
 int qs_engine_execute(qs_engine* e, uint64_t reps);

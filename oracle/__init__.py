@@ -229,6 +229,35 @@ def measure_seq(A, target, dims=2, destructive=True, u=0.5):
     return [int(x) for x in res[: len(t)]], [float(x) for x in pr[: len(t)]], st
 
 
+def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1 << 16):
+    """QEngine / QNoisyEngine execute(reps) on a qpp_amd.QCircuit; returns
+    (stats dict, dits, probs, final state)."""
+    cs = qc.struct()
+    nc = qc.nc
+    n_out = np.zeros(1, dtype=np.uint64)
+    sd = np.zeros(max(1, max_outcomes * nc), dtype=np.uint64)
+    sc = np.zeros(max_outcomes, dtype=np.uint64)
+    dits = np.zeros(max(1, nc), dtype=np.uint64)
+    probs = np.zeros(max(1, nc))
+    D = qc.d ** qc.nq
+    st = np.zeros(D * D if is_rho else D, dtype=np.complex128)
+    olen = np.zeros(1, dtype=np.uint64)
+    lib().orc_engine_run(ctypes.cast(ctypes.byref(cs), c_void_p), int(is_rho), float(noise_p),
+                         int(seed), int(reps), max_outcomes,
+                         n_out.ctypes.data_as(POINTER(c_uint64)), sd.ctypes.data_as(POINTER(c_uint64)),
+                         sc.ctypes.data_as(POINTER(c_uint64)), dits.ctypes.data_as(POINTER(c_uint64)),
+                         probs.ctypes.data_as(POINTER(c_double)), st.ctypes.data_as(POINTER(c_double)),
+                         olen.ctypes.data_as(POINTER(c_uint64)))
+    n = int(n_out[0])
+    stats = {tuple(int(x) for x in sd[i * nc:(i + 1) * nc]): int(sc[i]) for i in range(min(n, max_outcomes))}
+    L_ = int(olen[0])
+    state = st[:L_]
+    if is_rho:
+        Dn = int(round(np.sqrt(L_)))
+        state = state.reshape((Dn, Dn), order="F")
+    return stats, [int(x) for x in dits[:nc]], [float(x) for x in probs[:nc]], state
+
+
 class Rng:
     """The oracle's std::mt19937 + std::generate_canonical<double, 53>."""
 

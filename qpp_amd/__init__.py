@@ -671,6 +671,76 @@ class Program:
     def kernel_name(self):
         return "k_tile<double,16>"
 
+
+class QEngine:
+    """qpp::QEngineT / QNoisyEngineT (classes/qengine.hpp,
+    classes/qnoisy_engine.hpp) over a device-resident state.  noise_p >= 0
+    selects QNoisyEngine<QubitDepolarizingNoise(noise_p)>.  Draws come from
+    an std::mt19937 seeded with `seed`, in qpp's order."""
+
+    def __init__(self, qc, kind=KET, dtype=C128, noise_p=-1.0, seed=5489):
+        self.circuit = qc
+        self._cs = qc.struct()
+        L = lib()
+        L.qs_engine_create.argtypes = [POINTER(_Circuit), c_int, c_int, c_double, c_uint64,
+                                       POINTER(c_void_p)]
+        L.qs_engine_destroy.argtypes = [c_void_p]
+        L.qs_engine_execute.argtypes = [c_void_p, c_uint64]
+        L.qs_engine_num_outcomes.argtypes = [c_void_p, POINTER(c_uint64)]
+        L.qs_engine_get_stats.argtypes = [c_void_p, POINTER(c_uint64), POINTER(c_uint64)]
+        L.qs_engine_get_dits.argtypes = [c_void_p, POINTER(c_uint64)]
+        L.qs_engine_get_probs.argtypes = [c_void_p, POINTER(c_double)]
+        L.qs_engine_get_state.argtypes = [c_void_p, POINTER(c_void_p)]
+        L.qs_engine_set_state.argtypes = [c_void_p, c_void_p]
+        L.qs_engine_reset.argtypes = [c_void_p]
+        self._h = c_void_p()
+        _check(L.qs_engine_create(ctypes.byref(self._cs), kind, dtype, float(noise_p), int(seed),
+                                  ctypes.byref(self._h)))
+
+    def execute(self, reps=1):
+        _check(lib().qs_engine_execute(self._h, int(reps)))
+        return self
+
+    def get_stats(self):
+        n = c_uint64()
+        _check(lib().qs_engine_num_outcomes(self._h, byref(n)))
+        nc = self.circuit.nc
+        d = np.zeros(max(1, n.value * nc), dtype=np.uint64)
+        c = np.zeros(max(1, n.value), dtype=np.uint64)
+        _check(lib().qs_engine_get_stats(self._h, d.ctypes.data_as(POINTER(c_uint64)),
+                                         c.ctypes.data_as(POINTER(c_uint64))))
+        return {tuple(int(x) for x in d[i * nc:(i + 1) * nc]): int(c[i]) for i in range(n.value)}
+
+    def get_dits(self):
+        d = np.zeros(max(1, self.circuit.nc), dtype=np.uint64)
+        _check(lib().qs_engine_get_dits(self._h, d.ctypes.data_as(POINTER(c_uint64))))
+        return [int(x) for x in d[: self.circuit.nc]]
+
+    def get_probs(self):
+        p = np.zeros(max(1, self.circuit.nc))
+        _check(lib().qs_engine_get_probs(self._h, p.ctypes.data_as(POINTER(c_double))))
+        return [float(x) for x in p[: self.circuit.nc]]
+
+    def get_state(self):
+        h = c_void_p()
+        _check(lib().qs_engine_get_state(self._h, byref(h)))
+        return State._wrap(h)
+
+    def set_state(self, st):
+        _check(lib().qs_engine_set_state(self._h, st.handle))
+        return self
+
+    def reset(self):
+        _check(lib().qs_engine_reset(self._h))
+        return self
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().qs_engine_destroy(self._h)
+        except Exception:
+            pass
+
     def __del__(self):
         try:
             if self._h:

@@ -1,0 +1,591 @@
+// qs_engine.hip -- QEngine / QNoisyEngine execute loop over device states.
+//
+// Restates the step interpreter of qpp's engines (classes/qengine.hpp,
+// classes/qnoisy_engine.hpp) on a resident device state:
+//   execute(reps)        qengine.hpp:1341-1405 (canonical form for reps > 1,
+//                        steps before the first measurement run once, then
+//                        sampling or per-rep re-execution)
+//   sampling tail        qengine.hpp:374-481 (rep 0 draws and discards one
+//                        sample, executes the measurements; later reps
+//                        sample the pre-measurement state)
+//   retries              qengine.hpp:316-345 (state restored between reps,
+//                        the first rep's state kept)
+//   destructive measure  qengine.hpp:489-503, 514-526 (relative positions)
+//   noise                qnoisy_engine.hpp:104-140, noise.hpp:131-160
+//                        (before every step, one draw per live qubit)
+// Every uniform comes from the engine's std::mt19937 through
+// std::generate_canonical<double, 53>, in qpp's order.  Runs of gate steps
+// between measurements are compiled into tile-fused programs (cached) when
+// the state is a qubit state of at least 12 index bits, and applied with the
+// gate kernels otherwise.  A depolarizing event applies its Pauli operator;
+// the identity event and the renormalisation after a Pauli (noise.hpp:159,
+// a division by a norm equal to 1 up to rounding) are not launched.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <random>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "qs_internal.hpp"
+
+namespace {
+
+using qs::fail;
+constexpr uint64_t kMeasured = std::numeric_limits<uint64_t>::max();
+
+struct EStep {
+    uint32_t type = 0, gate = 0;
+    std::vector<uint64_t> ctrl, target, shift;
+    bool has_shift = false;
+    uint64_t c_reg = 0;
+    bool destructive = true;
+};
+
+bool is_meas(const EStep& s) {
+    return s.type == QS_STEP_MEASURE || s.type == QS_STEP_MEASURE_MANY;
+}
+
+std::vector<uint64_t> c_regs_of(const EStep& s) {
+    std::vector<uint64_t> c;
+    if (is_meas(s)) {
+        c.resize(s.target.size());
+        std::iota(c.begin(), c.end(), s.c_reg);
+    }
+    return c;
+}
+
+// can_swap, classes/qcircuit.hpp:7465-7492
+bool can_swap(const EStep& a, const EStep& b) {
+    auto overlap = [](const std::vector<uint64_t>& x, const std::vector<uint64_t>& y) {
+        return std::find_first_of(x.begin(), x.end(), y.begin(), y.end()) != x.end();
+    };
+    return !overlap(a.ctrl, b.ctrl) && !overlap(a.target, b.target) &&
+           !overlap(c_regs_of(a), c_regs_of(b));
+}
+
+// canonical_form, classes/qcircuit.hpp:7556-7612
+std::vector<EStep> canonical_form(const std::vector<EStep>& steps) {
+    auto first = std::find_if(steps.begin(), steps.end(), is_meas);
+    std::vector<EStep> before(steps.begin(), first), rest(first, steps.end());
+    for (std::ptrdiff_t ri = static_cast<std::ptrdiff_t>(rest.size()) - 1; ri >= 0; --ri) {
+        if (!is_meas(rest[ri])) continue;
+        for (size_t after = ri + 1; after < rest.size(); ++after) {
+            if (can_swap(rest[after - 1], rest[after]) && !is_meas(rest[after]))
+                std::swap(rest[after - 1], rest[after]);
+            else
+                break;
+        }
+    }
+    before.insert(before.end(), rest.begin(), rest.end());
+    return before;
+}
+
+// one gate application in relative positions; gate < 0 selects a Pauli
+// (-1 X, -2 Y, -3 Z) of a noise event
+struct GateApp {
+    uint32_t type;
+    int gate;
+    std::vector<uint64_t> ctrl, target, shift;
+    bool has_shift;
+};
+
+const double kPauli[3][8] = {
+    {0, 0, 1, 0, 1, 0, 0, 0},    // X, column-major (re, im)
+    {0, 0, 0, 1, 0, -1, 0, 0},   // Y = [[0, -i], [i, 0]]
+    {1, 0, 0, 0, 0, 0, -1, 0}};  // Z
+
+}  // namespace
+
+struct qs_engine {
+    uint32_t nq = 0, nc = 0, d = 2;
+    qs_kind kind = QS_KET;
+    qs_dtype dtype = QS_C128;
+    double noise_p = -1;
+    std::vector<EStep> steps;
+    std::vector<uint64_t> mat_dim, mat_off;
+    std::vector<double> mats;
+    std::mt19937 gen;
+    qs_state* st = nullptr;
+    std::vector<uint64_t> dits, subsys;
+    std::vector<double> probs;
+    std::map<std::vector<uint64_t>, uint64_t> stats;
+    std::map<std::string, qs_program*> progs;
+
+    double canonical() { return std::generate_canonical<double, 53>(gen); }
+
+    bool was_measured(uint64_t i) const { return subsys[i] == kMeasured; }
+
+    std::vector<uint64_t> rel(const std::vector<uint64_t>& v) const {
+        std::vector<uint64_t> r = v;
+        for (auto& x : r) x = subsys[x];
+        return r;
+    }
+
+    std::vector<uint64_t> live() const {
+        std::vector<uint64_t> l;
+        for (uint64_t i = 0; i < nq; ++i)
+            if (!was_measured(i)) l.push_back(i);
+        return l;
+    }
+
+    uint64_t live_dim() const {
+        uint64_t D = 1;
+        for (uint64_t i = 0; i < nq; ++i)
+            if (!was_measured(i)) D *= d;
+        return D;
+    }
+
+    // set_measured_destructively_, qengine.hpp:489-503
+    void set_measured(uint64_t i) {
+        subsys[i] = kMeasured;
+        for (uint64_t m = i; m < nq; ++m)
+            if (!was_measured(m)) --subsys[m];
+    }
+
+    const double* mat(int g, uint64_t& dim) const {
+        if (g < 0) {
+            dim = 2;
+            return kPauli[-g - 1];
+        }
+        dim = mat_dim[g];
+        return mats.data() + 2 * mat_off[g];
+    }
+
+    int run_gate_by_gate(const std::vector<GateApp>& gl) {
+        for (const GateApp& a : gl) {
+            uint64_t dim;
+            const double* m = mat(a.gate, dim);
+            const uint64_t* sh = a.has_shift ? a.shift.data() : nullptr;
+            int rc = QS_OK;
+            switch (a.type) {
+                case QS_STEP_GATE:
+                    rc = qs_apply(st, m, dim, uint32_t(a.target.size()), a.target.data());
+                    break;
+                case QS_STEP_FAN:
+                    for (uint64_t t : a.target)
+                        if ((rc = qs_apply(st, m, dim, 1, &t))) break;
+                    break;
+                case QS_STEP_CTRL:
+                    rc = qs_apply_ctrl(st, m, dim, uint32_t(a.ctrl.size()), a.ctrl.data(),
+                                       uint32_t(a.target.size()), a.target.data(), sh);
+                    break;
+                case QS_STEP_CTRL_FAN:
+                    rc = qs_apply_ctrl_fan(st, m, dim, uint32_t(a.ctrl.size()), a.ctrl.data(),
+                                           uint32_t(a.target.size()), a.target.data(), sh);
+                    break;
+                default:
+                    break;
+            }
+            if (rc) return rc;
+        }
+        return QS_OK;
+    }
+
+    // a run of gate applications: one fused program when the state allows
+    int run_gates(const std::vector<GateApp>& gl, bool cacheable) {
+        if (gl.empty()) return QS_OK;
+        const uint64_t nl = live().size();
+        const uint64_t nbits = (kind == QS_RHO ? 2 : 1) * nl;
+        bool fusible = d == 2 && nbits >= 12;
+        for (const GateApp& a : gl) {
+            const bool fan = a.type == QS_STEP_FAN || a.type == QS_STEP_CTRL_FAN;
+            if (!fan && a.target.size() > 2) fusible = false;
+        }
+        if (!fusible) return run_gate_by_gate(gl);
+        std::string key;
+        if (cacheable) {
+            key.reserve(gl.size() * 16);
+            for (const GateApp& a : gl) {
+                key += std::to_string(a.type) + ":" + std::to_string(a.gate) + ":";
+                for (uint64_t c : a.ctrl) key += std::to_string(c) + ",";
+                key += "|";
+                for (uint64_t t : a.target) key += std::to_string(t) + ",";
+                key += "|";
+                if (a.has_shift)
+                    for (uint64_t s : a.shift) key += std::to_string(s) + ",";
+                key += ";";
+            }
+            key += "n" + std::to_string(nl);
+            auto it = progs.find(key);
+            if (it != progs.end()) return qs_program_run(it->second, st);
+        }
+        // lower to a qs_circuit over the live qudits
+        std::vector<qs_step> ss(gl.size());
+        std::vector<uint64_t> pool, mdim = mat_dim, moff = mat_off;
+        std::vector<double> mm = mats;
+        for (int p = 0; p < 3; ++p) {
+            mdim.push_back(2);
+            moff.push_back(mm.size() / 2);
+            mm.insert(mm.end(), kPauli[p], kPauli[p] + 8);
+        }
+        const uint32_t npre = static_cast<uint32_t>(mat_dim.size());
+        for (size_t i = 0; i < gl.size(); ++i) {
+            const GateApp& a = gl[i];
+            qs_step& s = ss[i];
+            std::memset(&s, 0, sizeof(s));
+            s.type = a.type;
+            s.gate = a.gate >= 0 ? uint32_t(a.gate) : npre + uint32_t(-a.gate - 1);
+            s.nctrl = uint32_t(a.ctrl.size());
+            s.ntarget = uint32_t(a.target.size());
+            s.ctrl_off = uint32_t(pool.size());
+            pool.insert(pool.end(), a.ctrl.begin(), a.ctrl.end());
+            s.target_off = uint32_t(pool.size());
+            pool.insert(pool.end(), a.target.begin(), a.target.end());
+            if (a.has_shift) {
+                s.shift_off = uint32_t(pool.size());
+                pool.insert(pool.end(), a.shift.begin(), a.shift.end());
+            } else {
+                s.shift_off = UINT32_MAX;
+            }
+        }
+        pool.push_back(0);
+        qs_circuit qc{};
+        qc.nq = uint32_t(nl);
+        qc.nc = nc;
+        qc.d = d;
+        qc.nsteps = uint32_t(ss.size());
+        qc.steps = ss.data();
+        qc.nmats = uint32_t(mdim.size());
+        qc.mat_dim = mdim.data();
+        qc.mat_off = moff.data();
+        qc.mats = mm.data();
+        qc.pool = pool.data();
+        qs_program* prog = nullptr;
+        int rc = qs_program_create(&qc, 0, qc.nsteps, kind, dtype, nullptr, &prog);
+        if (rc == QS_ERR_NOT_IMPLEMENTED) return run_gate_by_gate(gl);
+        if (rc) return rc;
+        rc = qs_program_run(prog, st);
+        if (cacheable && !rc)
+            progs[key] = prog;
+        else
+            qs_program_destroy(prog);
+        return rc;
+    }
+
+    GateApp app_of(const EStep& s) const {
+        GateApp a;
+        a.type = s.type;
+        a.gate = int(s.gate);
+        a.ctrl = rel(s.ctrl);
+        a.target = rel(s.target);
+        a.shift = s.shift;
+        a.has_shift = s.has_shift;
+        return a;
+    }
+
+    // visit_measurement_step_, qengine.hpp:639-734, with its draw u
+    int measure(const EStep& s, double u) {
+        const std::vector<uint64_t> t = rel(s.target);
+        std::vector<uint64_t> res(t.size());
+        std::vector<double> ps(t.size());
+        if (int rc = qs_measure_seq(st, uint32_t(t.size()), t.data(), s.destructive ? 1 : 0, u,
+                                    res.data(), ps.data()))
+            return rc;
+        for (size_t i = 0; i < t.size(); ++i) {
+            dits[s.c_reg + i] = res[i];
+            probs[s.c_reg + i] = ps[i];
+        }
+        if (s.destructive)
+            for (uint64_t q : s.target) set_measured(q);
+        return QS_OK;
+    }
+
+    // noise before a step (qnoisy_engine.hpp:108-116): one draw per live
+    // qubit in ascending order, k = lower_bound(cp{1-p, p/3, p/3, p/3}, u)
+    void noise_apps(std::vector<GateApp>& pending) {
+        const std::vector<double> w{1 - noise_p, noise_p / 3, noise_p / 3, noise_p / 3};
+        const double sum = std::accumulate(w.begin(), w.end(), 0.0);
+        std::vector<double> q(4), cp(4);
+        for (int i = 0; i < 4; ++i) q[i] = w[i] / sum;
+        std::partial_sum(q.begin(), q.end(), cp.begin());
+        cp.back() = 1.0;
+        for (uint64_t qb : live()) {
+            const double u = canonical();
+            const int k = int(std::lower_bound(cp.begin(), cp.end(), u) - cp.begin());
+            if (k == 0) continue;
+            GateApp a;
+            a.type = QS_STEP_GATE;
+            a.gate = -k;
+            a.target = {subsys[qb]};
+            a.has_shift = false;
+            pending.push_back(a);
+        }
+    }
+
+    // executes steps [pos, end) once; noise before every step when noisy
+    int execute_steps_once(const std::vector<EStep>& s, size_t pos, bool& measured) {
+        measured = false;
+        std::vector<GateApp> pending;
+        bool cacheable = noise_p < 0;
+        for (size_t i = pos; i < s.size(); ++i) {
+            if (noise_p >= 0) noise_apps(pending);
+            if (is_meas(s[i])) {
+                measured = true;
+                if (int rc = run_gates(pending, cacheable && !pending.empty())) return rc;
+                pending.clear();
+                const double u = live_dim() >= 2 ? canonical() : 0.0;
+                if (int rc = measure(s[i], u)) return rc;
+            } else if (s[i].type != QS_STEP_NOP) {
+                pending.push_back(app_of(s[i]));
+            }
+        }
+        return run_gates(pending, cacheable);
+    }
+
+    struct Snap {
+        qs_state* st = nullptr;
+        std::vector<uint64_t> dits, subsys;
+        std::vector<double> probs;
+    };
+
+    int snapshot(Snap& sn) const {
+        sn.dits = dits;
+        sn.subsys = subsys;
+        sn.probs = probs;
+        return qs_state_clone(st, &sn.st);
+    }
+
+    void restore(Snap& sn, bool take) {
+        qs_state_destroy(st);
+        if (take) {
+            st = sn.st;
+            sn.st = nullptr;
+        } else {
+            qs_state_clone(sn.st, &st);
+        }
+        dits = sn.dits;
+        subsys = sn.subsys;
+        probs = sn.probs;
+    }
+
+    // execute_steps_with_postselection_retries_, qengine.hpp:316-345
+    int execute_with_retries(const std::vector<EStep>& s, size_t pos, uint64_t reps) {
+        Snap copy, first;
+        if (int rc = snapshot(copy)) return rc;
+        for (uint64_t rep = 0; rep < reps; ++rep) {
+            bool measured = false;
+            if (int rc = execute_steps_once(s, pos, measured)) return rc;
+            if (measured) ++stats[dits];
+            if (rep == 0 && reps > 1)
+                if (int rc = snapshot(first)) return rc;
+            if (rep + 1 < reps) restore(copy, false);
+        }
+        if (reps > 1) restore(first, true);
+        qs_state_destroy(copy.st);
+        return QS_OK;
+    }
+
+    // execute_measurement_steps_with_sampling_and_postselection_retries_,
+    // qengine.hpp:374-481 (projective measurements, no post-selection)
+    int execute_sampling(const std::vector<EStep>& s, size_t pos, uint64_t reps) {
+        std::map<uint64_t, uint64_t> c_q;
+        bool any = false;
+        for (size_t i = pos; i < s.size(); ++i)
+            if (is_meas(s[i])) {
+                any = true;
+                const auto cr = c_regs_of(s[i]);
+                for (size_t q = 0; q < s[i].target.size(); ++q) c_q[cr[q]] = s[i].target[q];
+            }
+        if (!any) return QS_OK;
+        std::set<uint64_t> sset;
+        for (auto& kv : c_q) sset.insert(kv.second);
+        const std::vector<uint64_t> from(sset.begin(), sset.end());
+        const uint64_t D = live_dim();
+        qs_state* pre = nullptr;
+        if (int rc = qs_state_clone(st, &pre)) return rc;
+        // rep 0: one sample drawn and discarded, then the measurement steps
+        if (D >= 2) (void)canonical();
+        bool measured = false;
+        int rc = execute_steps_once(s, pos, measured);
+        if (!rc && measured) ++stats[dits];
+        // reps 1..: one draw each, sampled from the pre-measurement state
+        std::vector<double> u;
+        for (uint64_t rep = 1; rep < reps && !rc; ++rep) u.push_back(D >= 2 ? canonical() : 0.0);
+        if (!rc && !u.empty()) {
+            const std::vector<uint64_t> frel = from;  // every qudit is live at pos
+            std::vector<uint64_t> dig(u.size() * frel.size());
+            rc = qs_sample(pre, uint32_t(frel.size()), frel.data(), u.size(), u.data(), nullptr,
+                           dig.data(), nullptr);
+            for (size_t r = 0; r < u.size() && !rc; ++r) {
+                std::vector<uint64_t> res = dits;
+                size_t i = 0;
+                for (auto& kv : c_q) res[kv.first] = dig[r * frel.size() + i++];
+                ++stats[res];
+            }
+        }
+        qs_state_destroy(pre);
+        return rc;
+    }
+
+    void reset_bookkeeping() {
+        dits.assign(nc, 0);
+        probs.assign(nc, 0.0);
+        subsys.resize(nq);
+        std::iota(subsys.begin(), subsys.end(), uint64_t{0});
+    }
+
+    int execute(uint64_t reps) {
+        if (noise_p >= 0) return execute_with_retries(steps, 0, reps);
+        // reset(qstate, false): keep the state and the statistics
+        if (!live().empty() && live().size() != nq)
+            return fail(QS_ERR_DIMS_NOT_EQUAL, "qpp::QEngineT::execute()", "state");
+        reset_bookkeeping();
+        const std::vector<EStep> s = reps > 1 ? canonical_form(steps) : steps;
+        if (s.empty()) return QS_OK;
+        const size_t pos = std::find_if(s.begin(), s.end(), is_meas) - s.begin();
+        bool can_sample = true;
+        for (size_t i = pos; i < s.size(); ++i)
+            if (!is_meas(s[i])) can_sample = false;
+        std::vector<GateApp> pre;
+        for (size_t i = 0; i < pos; ++i)
+            if (s[i].type != QS_STEP_NOP) pre.push_back(app_of(s[i]));
+        if (int rc = run_gates(pre, true)) return rc;
+        if (reps > 1 && can_sample) return execute_sampling(s, pos, reps);
+        return execute_with_retries(s, pos, reps);
+    }
+
+    ~qs_engine() {
+        for (auto& kv : progs) qs_program_destroy(kv.second);
+        if (st) qs_state_destroy(st);
+    }
+};
+
+extern "C" {
+
+int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double noise_p,
+                     uint64_t seed, qs_engine** out) {
+    constexpr const char* W = "qpp::QEngineT::QEngineT()";
+    if (!qc || !out) return fail(QS_ERR_INVALID_HANDLE, W, "circuit");
+    *out = nullptr;
+    if (noise_p > 1) return fail(QS_ERR_OUT_OF_RANGE, "qpp::QubitDepolarizingNoise()", "p");
+    if (noise_p >= 0 && qc->d != 2)
+        return fail(QS_ERR_NOT_IMPLEMENTED, W, "depolarizing noise on qudits");
+    auto* e = new qs_engine;
+    e->nq = qc->nq;
+    e->nc = qc->nc;
+    e->d = qc->d;
+    e->kind = kind;
+    e->dtype = dtype;
+    e->noise_p = noise_p;
+    e->gen.seed(static_cast<std::mt19937::result_type>(seed));
+    for (uint32_t m = 0; m < qc->nmats; ++m) {
+        e->mat_dim.push_back(qc->mat_dim[m]);
+        e->mat_off.push_back(e->mats.size() / 2);
+        const double* p = qc->mats + 2 * qc->mat_off[m];
+        e->mats.insert(e->mats.end(), p, p + 2 * qc->mat_dim[m] * qc->mat_dim[m]);
+    }
+    for (uint32_t i = 0; i < qc->nsteps; ++i) {
+        const qs_step& s = qc->steps[i];
+        EStep t;
+        t.type = s.type;
+        t.gate = s.gate;
+        t.ctrl.assign(qc->pool + s.ctrl_off, qc->pool + s.ctrl_off + s.nctrl);
+        t.target.assign(qc->pool + s.target_off, qc->pool + s.target_off + s.ntarget);
+        if (s.shift_off != UINT32_MAX) {
+            t.has_shift = true;
+            t.shift.assign(qc->pool + s.shift_off, qc->pool + s.shift_off + s.nctrl);
+        }
+        t.c_reg = s.c_reg;
+        t.destructive = s.destructive != 0;
+        if (is_meas(t) && t.c_reg + t.target.size() > qc->nc) {
+            delete e;
+            return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::measure()", "c_reg");
+        }
+        e->steps.push_back(t);
+    }
+    std::vector<uint64_t> dims(qc->nq, qc->d);
+    if (int rc = qs_state_create(dtype, kind, qc->nq, dims.data(), &e->st)) {
+        delete e;
+        return rc;
+    }
+    e->reset_bookkeeping();
+    *out = e;
+    return QS_OK;
+}
+
+int qs_engine_destroy(qs_engine* e) {
+    delete e;
+    return QS_OK;
+}
+
+int qs_engine_execute(qs_engine* e, uint64_t reps) {
+    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::execute()");
+    qs::clear_error();
+    if (reps == 0) return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()", "reps");
+    return e->execute(reps);
+}
+
+int qs_engine_num_outcomes(const qs_engine* e, uint64_t* n) {
+    if (!e || !n) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_stats()");
+    *n = e->stats.size();
+    return QS_OK;
+}
+
+int qs_engine_get_stats(const qs_engine* e, uint64_t* dits, uint64_t* counts) {
+    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_stats()");
+    size_t k = 0;
+    for (const auto& kv : e->stats) {
+        if (dits) std::copy(kv.first.begin(), kv.first.end(), dits + k * e->nc);
+        if (counts) counts[k] = kv.second;
+        ++k;
+    }
+    return QS_OK;
+}
+
+int qs_engine_get_dits(const qs_engine* e, uint64_t* dits) {
+    if (!e || !dits) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_dits()");
+    std::copy(e->dits.begin(), e->dits.end(), dits);
+    return QS_OK;
+}
+
+int qs_engine_get_probs(const qs_engine* e, double* probs) {
+    if (!e || !probs) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_probs()");
+    std::copy(e->probs.begin(), e->probs.end(), probs);
+    return QS_OK;
+}
+
+int qs_engine_get_state(const qs_engine* e, qs_state** out) {
+    if (!e || !out) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_state()");
+    return qs_state_clone(e->st, out);
+}
+
+int qs_engine_set_state(qs_engine* e, const qs_state* st) {
+    if (!e || !st) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::set_state()");
+    qs_dtype dt;
+    qs_kind kd;
+    uint32_t nd = 0;
+    uint64_t ne = 0;
+    std::vector<uint64_t> dims(128);
+    if (int rc = qs_state_info(st, &dt, &kd, &nd, dims.data(), &ne)) return rc;
+    uint64_t Dl = e->live_dim();
+    const uint64_t D = kd == QS_KET ? ne : static_cast<uint64_t>(std::llround(std::sqrt(double(ne))));
+    if (dt != e->dtype || kd != e->kind || D != Dl)
+        return fail(QS_ERR_DIMS_NOT_EQUAL, "qpp::QEngineT::set_state()", "state");
+    qs_state* c = nullptr;
+    if (int rc = qs_state_clone(st, &c)) return rc;
+    qs_state_destroy(e->st);
+    e->st = c;
+    return QS_OK;
+}
+
+int qs_engine_reset(qs_engine* e) {
+    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::reset()");
+    std::vector<uint64_t> dims(e->nq, e->d);
+    qs_state* s = nullptr;
+    if (int rc = qs_state_create(e->dtype, e->kind, e->nq, dims.data(), &s)) return rc;
+    qs_state_destroy(e->st);
+    e->st = s;
+    e->reset_bookkeeping();
+    e->stats.clear();
+    return QS_OK;
+}
+
+}  // extern "C"
