@@ -180,6 +180,19 @@ int qs_norm2(const qs_state* st, double* out);
 int qs_normalize(qs_state* st);
 
 /* ------------------------------------------------------------------------ */
+/* Precision (complex64 path)                                                */
+/* ------------------------------------------------------------------------ */
+/* No single qpp counterpart: qpp computes in one dtype.  These report how
+ * far a complex64 result is from a complex128 one, accumulated in FP64:
+ * out[5] = { Re<a|b>, Im<a|b>, <a|a>, <b|b>, max_i |a_i - b_i| }
+ * (for density matrices <a|b> = tr(a^dagger b)).  The infidelity of pure
+ * states is 1 - |<a|b>|^2 / (<a|a><b|b>) and their trace distance
+ * sqrt(infidelity). */
+int qs_compare(const qs_state* a, const qs_state* b, double* out);
+/* A copy of st stored as dtype (complex128 <-> complex64). */
+int qs_state_convert(const qs_state* st, qs_dtype dtype, qs_state** out);
+
+/* ------------------------------------------------------------------------ */
 /* Measurement                                                               */
 /* ------------------------------------------------------------------------ */
 /* Z-basis probabilities |psi_i|^2 (ket, re*re + im*im as std::norm) or

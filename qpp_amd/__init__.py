@@ -124,6 +124,8 @@ def lib():
             "qs_syspermute": (c_int, [vp, u64p, c_int, POINTER(vp)]),
             "qs_norm2": (c_int, [vp, dp]),
             "qs_normalize": (c_int, [vp]),
+            "qs_compare": (c_int, [vp, vp, dp]),
+            "qs_state_convert": (c_int, [vp, c_int, POINTER(vp)]),
             "qs_probs": (c_int, [vp, dp, c_uint64]),
             "qs_sample": (c_int, [vp, c_uint32, u64p, c_uint64, dp, u64p, u64p, dp]),
             "qs_measure_seq": (c_int, [vp, c_uint32, u64p, c_int, c_double, u64p, dp]),
@@ -325,6 +327,30 @@ class State:
     def normalize(self):
         _check(lib().qs_normalize(self._h))
         return self
+
+    def astype(self, dtype):
+        """A device copy stored as C128 or C64."""
+        h = c_void_p()
+        _check(lib().qs_state_convert(self._h, dtype, byref(h)))
+        return State._wrap(h)
+
+    def compare(self, other):
+        """FP64 comparison with another state of the same shape (any dtype):
+        overlap, norms, max-abs amplitude difference and, for kets, the
+        infidelity 1 - |<a|b>|^2/(<a|a><b|b>) and pure-state trace distance
+        sqrt(infidelity)."""
+        out = (c_double * 5)()
+        _check(lib().qs_compare(self._h, other._h, out))
+        ov = complex(out[0], out[1])
+        r = {"overlap": ov, "norm2_a": out[2], "norm2_b": out[3], "max_abs": out[4]}
+        if self.kind == KET:
+            inf = max(0.0, 1.0 - abs(ov) ** 2 / (out[2] * out[3]))
+            r["infidelity"] = inf
+            r["trace_distance"] = math.sqrt(inf)
+        else:
+            # Hilbert-Schmidt distance; trace distance <= sqrt(rank) / 2 times it
+            r["hs_distance"] = math.sqrt(max(0.0, out[2] + out[3] - 2 * out[0]))
+        return r
 
     def probs(self):
         dt, kd, dims, ne = self.info()

@@ -21,6 +21,7 @@ SOURCES = [
     "qs_fused.hip",
     "qs_program.hip",
     "qs_engine.hip",
+    "qs_precision.hip",
 ]
 
 FLAGS = [
