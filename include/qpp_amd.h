@@ -312,6 +312,42 @@ int qs_engine_set_state(qs_engine* e, const qs_state* st);
 /* QEngine::reset(): |0..0>, stats cleared. */
 int qs_engine_reset(qs_engine* e);
 
+/* ------------------------------------------------------------------------ */
+/* Batched noisy trajectories (one persistent launch)                        */
+/* ------------------------------------------------------------------------ */
+/* Trajectory i of a run computes what
+ *   QNoisyEngine<QubitDepolarizingNoise(noise_p)>(circuits[circuit_of[i]],
+ *                                                 seed = seeds[i]).execute(1)
+ * computes (classes/qnoisy_engine.hpp:104-140, classes/noise.hpp:131-160,
+ * 388-408; noise_p < 0: QEngine(..).execute(1), qengine.hpp:1341-1405): the
+ * same std::mt19937 draws in the same order, and the final measurement
+ * outcome drawn from the trajectory's last uniform.  Circuits: qubits, gate
+ * steps whose gates are dense 1-qubit gates or Clifford gates on at most 2
+ * qubits (CNOT, CZ, X, Z, ...), ending in one MEASURE_MANY of every qubit.
+ * Circuits may have different qubit counts (>= 8 + reg_bits).
+ * out_index[i]: basis index of the outcome (measured digit of target j =
+ * bit (n-1-target[j])); out_prob[i]: its probability. */
+typedef struct qs_batch qs_batch;
+int qs_batch_create(const qs_circuit* const* circuits, uint32_t ncircuits,
+                    qs_dtype dtype, const qs_fuse_opts* opts, qs_batch** out);
+int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of,
+                 const uint64_t* seeds, double noise_p, uint64_t* out_index,
+                 double* out_prob);
+/* Program size of one circuit; statistics of the last run (shots decided by
+ * the exact sequential replay, host planning and kernel times in ms). */
+int qs_batch_info(const qs_batch* b, uint32_t circuit, uint64_t* num_passes,
+                  uint64_t* num_ops, uint64_t* last_exact, double* last_host_ms,
+                  double* last_device_ms);
+/* Host only: the Pauli-frame plan of one trajectory -- the circuit steps
+ * whose 1-qubit matrix is replaced (U X^x Z^z, column-major, 8 doubles
+ * each; up to cap entries), the frame left at the measurement
+ * (i^phase X^xmask Z^zmask on index bits) and the measurement uniform. */
+int qs_batch_plan(const qs_batch* b, uint32_t circuit, uint64_t seed,
+                  double noise_p, uint64_t cap, uint64_t* n_ovr,
+                  uint32_t* ovr_step, double* ovr_mats, uint64_t* xmask,
+                  uint64_t* zmask, uint32_t* phase, double* u);
+int qs_batch_destroy(qs_batch* b);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -697,6 +697,13 @@ class Program:
     def kernel_name(self):
         return "k_tile<double,16>"
 
+    def __del__(self):
+        try:
+            if self._h:
+                lib().qs_program_destroy(self._h)
+        except Exception:
+            pass
+
 
 class QEngine:
     """qpp::QEngineT / QNoisyEngineT (classes/qengine.hpp,
@@ -767,9 +774,86 @@ class QEngine:
         except Exception:
             pass
 
+
+class Batch:
+    """Batched noisy trajectories in one persistent launch.  Trajectory i is
+    QNoisyEngine<QubitDepolarizingNoise(noise_p)>(circuits[circuit_of[i]],
+    seed=seeds[i]).execute(1) (noise_p < 0: QEngine(...).execute(1));
+    every circuit ends in one measurement of all its qubits."""
+
+    def __init__(self, circuits, dtype=C64, reg_bits=4, low_bits=5, max_ops=48):
+        if isinstance(circuits, QCircuit):
+            circuits = [circuits]
+        self.circuits = list(circuits)
+        self._cs = [qc.struct() for qc in self.circuits]
+        L = lib()
+        L.qs_batch_create.argtypes = [POINTER(POINTER(_Circuit)), c_uint32, c_int,
+                                      POINTER(_FuseOpts), POINTER(c_void_p)]
+        L.qs_batch_run.argtypes = [c_void_p, c_uint64, POINTER(c_uint32), POINTER(c_uint64),
+                                   c_double, POINTER(c_uint64), POINTER(c_double)]
+        L.qs_batch_info.argtypes = [c_void_p, c_uint32, POINTER(c_uint64), POINTER(c_uint64),
+                                    POINTER(c_uint64), POINTER(c_double), POINTER(c_double)]
+        L.qs_batch_plan.argtypes = [c_void_p, c_uint32, c_uint64, c_double, c_uint64,
+                                    POINTER(c_uint64), POINTER(c_uint32), POINTER(c_double),
+                                    POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint32),
+                                    POINTER(c_double)]
+        L.qs_batch_destroy.argtypes = [c_void_p]
+        arr = (POINTER(_Circuit) * len(self._cs))(*[ctypes.pointer(c) for c in self._cs])
+        o = _FuseOpts(reg_bits, low_bits, max_ops, 0)
+        self._h = c_void_p()
+        _check(L.qs_batch_create(arr, len(self._cs), dtype, ctypes.byref(o), ctypes.byref(self._h)))
+
+    def run(self, circuit_of, seeds, noise_p):
+        """Outcome basis indices and their probabilities, one per trajectory."""
+        c = np.ascontiguousarray(circuit_of, dtype=np.uint32)
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        if c.shape != s.shape:
+            raise SizeMismatch("[WHERE: qpp_amd::batch] Size mismatch! [CONTEXT: seeds]")
+        idx = np.zeros(max(1, len(c)), dtype=np.uint64)
+        pr = np.zeros(max(1, len(c)))
+        _check(lib().qs_batch_run(self._h, len(c), c.ctypes.data_as(POINTER(c_uint32)),
+                                  s.ctypes.data_as(POINTER(c_uint64)), float(noise_p),
+                                  idx.ctypes.data_as(POINTER(c_uint64)),
+                                  pr.ctypes.data_as(POINTER(c_double))))
+        return idx[: len(c)], pr[: len(c)]
+
+    def info(self, circuit=0):
+        v = [c_uint64() for _ in range(3)]
+        h, d = c_double(), c_double()
+        _check(lib().qs_batch_info(self._h, circuit, byref(v[0]), byref(v[1]), byref(v[2]),
+                                   byref(h), byref(d)))
+        return {"passes": v[0].value, "ops": v[1].value, "exact_replays": v[2].value,
+                "host_ms": h.value, "device_ms": d.value}
+
+    def plan(self, circuit, seed, noise_p, cap=4096):
+        n = c_uint64()
+        st = np.zeros(cap, dtype=np.uint32)
+        m = np.zeros(8 * cap)
+        xm, zm, ph, u = c_uint64(), c_uint64(), c_uint32(), c_double()
+        _check(lib().qs_batch_plan(self._h, circuit, int(seed), float(noise_p), cap, byref(n),
+                                   st.ctypes.data_as(POINTER(c_uint32)),
+                                   m.ctypes.data_as(POINTER(c_double)), byref(xm), byref(zm),
+                                   byref(ph), byref(u)))
+        k = min(n.value, cap)
+        mats = [np.array([m[8 * i + 2 * j] + 1j * m[8 * i + 2 * j + 1] for j in range(4)])
+                .reshape(2, 2, order="F") for i in range(k)]
+        return {"steps": [int(x) for x in st[:k]], "mats": mats, "xmask": xm.value,
+                "zmask": zm.value, "phase": ph.value, "u": u.value}
+
+    def dits(self, circuit, index):
+        """The engine's classical dits for an outcome index (the final
+        MEASURE_MANY writes target j's digit to c_reg + j)."""
+        qc = self.circuits[circuit]
+        tp, g, ctrl, tg, shift, creg, dest = [s for s in qc.steps if s[0] == STEP_MEASURE_MANY][-1]
+        d = [0] * qc.nc
+        n = qc.nq
+        for j, q in enumerate(tg):
+            d[creg + j] = (int(index) >> (n - 1 - q)) & 1
+        return tuple(d)
+
     def __del__(self):
         try:
             if self._h:
-                lib().qs_program_destroy(self._h)
+                lib().qs_batch_destroy(self._h)
         except Exception:
             pass

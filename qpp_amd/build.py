@@ -22,6 +22,7 @@ SOURCES = [
     "qs_program.hip",
     "qs_engine.hip",
     "qs_precision.hip",
+    "qs_batch.hip",
 ]
 
 FLAGS = [

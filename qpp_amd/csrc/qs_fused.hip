@@ -29,246 +29,11 @@
 
 #include "qs_fused.hpp"
 #include "qs_internal.hpp"
+#include "qs_tile_ops.hpp"
 
 namespace {
 
 using qs::fail;
-
-template <typename R>
-struct cx;
-template <>
-struct cx<double> {
-    using T = double2;
-};
-template <>
-struct cx<float> {
-    using T = float2;
-};
-template <typename R>
-using cxt = typename cx<R>::T;
-
-template <typename R>
-__device__ __forceinline__ cxt<R> mk(R x, R y) {
-    cxt<R> r;
-    r.x = x;
-    r.y = y;
-    return r;
-}
-
-// u0 * a + u1 * b
-template <typename R>
-__device__ __forceinline__ cxt<R> mac2(cxt<R> u0, cxt<R> a, cxt<R> u1, cxt<R> b) {
-    cxt<R> r;
-    r.x = u0.x * a.x - u0.y * a.y + u1.x * b.x - u1.y * b.y;
-    r.y = u0.x * a.y + u0.y * a.x + u1.x * b.y + u1.y * b.x;
-    return r;
-}
-
-template <typename R>
-__device__ __forceinline__ cxt<R> cmul(cxt<R> u, cxt<R> a) {
-    return mk<R>(u.x * a.x - u.y * a.y, u.x * a.y + u.y * a.x);
-}
-
-template <typename R>
-__device__ __forceinline__ cxt<R> sel(bool c, cxt<R> a, cxt<R> b) {
-    return mk<R>(c ? a.x : b.x, c ? a.y : b.y);
-}
-
-template <typename R>
-__device__ __forceinline__ cxt<R> opm(const FOp* op, int i) {
-    return mk<R>(static_cast<R>(op->m[2 * i]), static_cast<R>(op->m[2 * i + 1]));
-}
-
-constexpr int kThreads = 256;
-
-template <int NR>
-struct RBof;
-template <>
-struct RBof<8> {
-    static constexpr int v = 3;
-};
-template <>
-struct RBof<16> {
-    static constexpr int v = 4;
-};
-
-// ---------------------------------------------------------------------------
-// ops on the register array x[NR]
-// ---------------------------------------------------------------------------
-template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_u1(cxt<R>* x, const FOp* op, bool tok) {
-    const cxt<R> u00 = opm<R>(op, 0), u10 = opm<R>(op, 1);
-    const cxt<R> u01 = opm<R>(op, 2), u11 = opm<R>(op, 3);
-    const uint32_t regok = op->regok;
-#pragma unroll
-    for (int rho = 0; rho < NR; ++rho) {
-        if ((rho >> J) & 1) continue;
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const int r1 = rho | (1 << J);
-        const cxt<R> a = x[rho], b = x[r1];
-        const cxt<R> y0 = mac2<R>(u00, a, u01, b);
-        const cxt<R> y1 = mac2<R>(u10, a, u11, b);
-        x[rho] = CT ? sel<R>(tok, y0, a) : y0;
-        x[r1] = CT ? sel<R>(tok, y1, b) : y1;
-    }
-}
-
-template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_x(cxt<R>* x, const FOp* op, bool tok) {
-    const uint32_t regok = op->regok;
-#pragma unroll
-    for (int rho = 0; rho < NR; ++rho) {
-        if ((rho >> J) & 1) continue;
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const int r1 = rho | (1 << J);
-        const cxt<R> a = x[rho], b = x[r1];
-        x[rho] = CT ? sel<R>(tok, b, a) : b;
-        x[r1] = CT ? sel<R>(tok, a, b) : a;
-    }
-}
-
-template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_diag_r(cxt<R>* x, const FOp* op, bool tok) {
-    const cxt<R> d0 = opm<R>(op, 0), d1 = opm<R>(op, 1);
-    const uint32_t regok = op->regok;
-#pragma unroll
-    for (int rho = 0; rho < NR; ++rho) {
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const cxt<R> y = cmul<R>(((rho >> J) & 1) ? d1 : d0, x[rho]);
-        x[rho] = CT ? sel<R>(tok, y, x[rho]) : y;
-    }
-}
-
-// diagonal factor d (per lane or uniform) on every element
-template <typename R, int NR, bool CT>
-__device__ __forceinline__ void do_scale(cxt<R>* x, const FOp* op, cxt<R> d, bool tok) {
-    const uint32_t regok = op->regok;
-#pragma unroll
-    for (int rho = 0; rho < NR; ++rho) {
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const cxt<R> y = cmul<R>(d, x[rho]);
-        x[rho] = CT ? sel<R>(tok, y, x[rho]) : y;
-    }
-}
-
-template <typename R, int NR, int J0, int J1, bool CT>
-__device__ __forceinline__ void do_u2(cxt<R>* x, const FOp* op, const cxt<R>* __restrict__ mats,
-                                      bool tok) {
-    // column-major 4x4; gate index bit 0 <-> register bit J0, bit 1 <-> J1
-    const cxt<R>* M = mats + op->moff;
-    cxt<R> m[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) m[i] = M[i];
-    const uint32_t regok = op->regok;
-#pragma unroll
-    for (int rho = 0; rho < NR; ++rho) {
-        if (((rho >> J0) & 1) || ((rho >> J1) & 1)) continue;
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const int i0 = rho, i1 = rho | (1 << J0), i2 = rho | (1 << J1),
-                  i3 = rho | (1 << J0) | (1 << J1);
-        const cxt<R> v[4] = {x[i0], x[i1], x[i2], x[i3]};
-        cxt<R> o[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            cxt<R> s = mac2<R>(m[r], v[0], m[r + 4], v[1]);
-            const cxt<R> t2 = mac2<R>(m[r + 8], v[2], m[r + 12], v[3]);
-            s.x += t2.x;
-            s.y += t2.y;
-            o[r] = s;
-        }
-        if (CT) {
-            x[i0] = sel<R>(tok, o[0], v[0]);
-            x[i1] = sel<R>(tok, o[1], v[1]);
-            x[i2] = sel<R>(tok, o[2], v[2]);
-            x[i3] = sel<R>(tok, o[3], v[3]);
-        } else {
-            x[i0] = o[0];
-            x[i1] = o[1];
-            x[i2] = o[2];
-            x[i3] = o[3];
-        }
-    }
-}
-
-#define QS_J_CASES(F, ...)                                         \
-    switch (op->j0) {                                              \
-        case 0: F<R, NR, 0, CT>(__VA_ARGS__); break;               \
-        case 1: F<R, NR, 1, CT>(__VA_ARGS__); break;               \
-        case 2: F<R, NR, 2, CT>(__VA_ARGS__); break;               \
-        case 3: if constexpr (NR > 8) F<R, NR, 3, CT>(__VA_ARGS__); break; \
-        default: break;                                            \
-    }
-
-template <typename R, int NR, bool CT>
-__device__ __forceinline__ void run_op_ct(cxt<R>* x, const FOp* op, const cxt<R>* mats,
-                                          uint32_t t, uint64_t base, bool tok) {
-    switch (op->kind) {
-        case OP_U1:
-            QS_J_CASES(do_u1, x, op, tok)
-            break;
-        case OP_X:
-            QS_J_CASES(do_x, x, op, tok)
-            break;
-        case OP_DIAG_R:
-            QS_J_CASES(do_diag_r, x, op, tok)
-            break;
-        case OP_DIAG_T: {
-            const bool one = (t & op->tbit) != 0;
-            const cxt<R> d = sel<R>(one, opm<R>(op, 1), opm<R>(op, 0));
-            do_scale<R, NR, CT>(x, op, d, tok);
-            break;
-        }
-        case OP_DIAG_O: {
-            const cxt<R> d = (base & op->dbit) ? opm<R>(op, 1) : opm<R>(op, 0);
-            do_scale<R, NR, CT>(x, op, d, tok);
-            break;
-        }
-        case OP_U2: {
-            const int key = op->j0 * 8 + op->j1;
-            switch (key) {
-                case 0 * 8 + 1: do_u2<R, NR, 0, 1, CT>(x, op, mats, tok); break;
-                case 0 * 8 + 2: do_u2<R, NR, 0, 2, CT>(x, op, mats, tok); break;
-                case 1 * 8 + 2: do_u2<R, NR, 1, 2, CT>(x, op, mats, tok); break;
-                case 0 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 0, 3, CT>(x, op, mats, tok); break;
-                case 1 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 1, 3, CT>(x, op, mats, tok); break;
-                case 2 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 2, 3, CT>(x, op, mats, tok); break;
-                default: break;
-            }
-            break;
-        }
-        default:
-            break;
-    }
-}
-
-template <typename R, int NR>
-__device__ __forceinline__ void run_op(cxt<R>* x, const FOp* op, const cxt<R>* mats,
-                                       uint32_t t, uint64_t base) {
-    if ((base & op->omask) != op->oval) return;  // out-of-tile control, uniform
-    if (op->ctrl) {
-        const bool tok = (t & op->tmask) == op->tval;
-        run_op_ct<R, NR, true>(x, op, mats, t, base, tok);
-    } else {
-        run_op_ct<R, NR, false>(x, op, mats, t, base, true);
-    }
-}
-
-// thread part of a phase's global index / swizzled LDS index
-__device__ __forceinline__ uint64_t thr_g(const FPhase* ph, uint32_t t) {
-    uint64_t g = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if ((t >> j) & 1u) g |= ph->g_thr[j];
-    return g;
-}
-
-__device__ __forceinline__ uint32_t thr_s(const FPhase* ph, uint32_t t) {
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if ((t >> j) & 1u) s ^= ph->s_thr[j];
-    return s;
-}
 
 // One pass over all tiles of the state.
 template <typename R, int NR>
@@ -326,7 +91,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
                 cur = ph;
             }
             const int ob = ph->op_begin, oe = ph->op_end;
-            for (int o = ob; o < oe; ++o) run_op<R, NR>(x, ops + o, mats, t, base);
+            for (int o = ob; o < oe; ++o) run_op<R, NR>(x, ops + o, ops[o].m, mats, t, base);
         }
         if (!cur->coalesced) {
             const uint32_t sw = thr_s(cur, t);
@@ -549,6 +314,7 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
     }
     std::vector<size_t> todo(gates.size());
     for (size_t i = 0; i < todo.size(); ++i) todo[i] = i;
+    P.gate_op.assign(gates.size(), -1);
     std::vector<C> mats;
     while (!todo.empty()) {
         std::set<uint32_t> S;
@@ -600,7 +366,10 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
                 P.phases.push_back(ldp);
             }
             ph.op_begin = static_cast<int>(P.ops.size());
-            for (size_t pi : ptake) P.ops.push_back(lower_op(gates[take[pi]], ph, RB, Sv, mats));
+            for (size_t pi : ptake) {
+                P.gate_op[take[pi]] = static_cast<int>(P.ops.size());
+                P.ops.push_back(lower_op(gates[take[pi]], ph, RB, Sv, mats));
+            }
             ph.op_end = static_cast<int>(P.ops.size());
             P.phases.push_back(ph);
             std::vector<size_t> rem;

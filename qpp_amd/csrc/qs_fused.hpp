@@ -81,6 +81,7 @@ struct FusedProgram {
     std::vector<FOp> ops;
     std::vector<std::complex<double>> mats;  // OP_U2 matrices
     std::vector<size_t> gates_per_pass;
+    std::vector<int> gate_op;  // gate index -> index of its op in `ops`
     qs_dtype dtype = QS_C128;
     void* dblob = nullptr;
     FPass* d_passes = nullptr;
@@ -95,6 +96,9 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
 int fused_upload(FusedProgram& P, qs_dtype dtype);
 void fused_free(FusedProgram& P);
 int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap);
+// circuit gate steps [b, e) -> bit gates (qs_program.hip)
+int lower_steps(const qs_circuit* qc, uint32_t b, uint32_t e, bool rho,
+                std::vector<BitGate>& out);
 }  // namespace qs
 
 #endif  // QPP_AMD_QS_FUSED_HPP_

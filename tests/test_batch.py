@@ -1,0 +1,181 @@
+"""Batched noisy trajectories (qs_batch_*).
+
+CPU: the host plan (Pauli frame absorbed into 1-qubit gates, conjugated
+through Clifford gates) reproduces an explicit simulation of the same
+trajectory with the errors applied where QNoisyEngine applies them, drawn
+from the same std::mt19937 stream (oracle Rng).  GPU: every trajectory's
+outcome equals the oracle QNoisyEngine's for the same seed."""
+import numpy as np
+import pytest
+
+from helpers import CNOT, CZ, H, I2, X, Y, Z, haar, t_apply
+
+import qpp_amd as qa
+
+PAULI = [I2, X, Y, Z]
+
+
+def noisy_circuit(n, depth, seed, cz=False, extra=False):
+    """Haar 1-qubit gates on every qubit, then a CNOT ladder (i, i+1)
+    (optionally CZ / X / Z / H Clifford gates too), ending in one
+    measurement of every qubit."""
+    rng = np.random.default_rng(seed)
+    qc = qa.QCircuit(n, n)
+    for _ in range(depth):
+        for q in range(n):
+            qc.gate(haar(2, rng), q)
+        for i in range(n - 1):
+            qc.CTRL(Z if cz and i % 3 == 2 else X, i, i + 1)
+        if extra:
+            qc.gate(X, 0).gate(Z, n - 1).gate(H, n // 2)
+            qc.CTRL(X, n - 1, 0, shift=[1])
+            qc.gate(CZ, 1, 3)
+    qc.measure(list(range(n)), 0)
+    return qc
+
+
+def explicit_trajectory(orc, qc, seed, p):
+    """Statevector of one QNoisyEngine trajectory with explicit Pauli errors
+    before every step (qnoisy_engine.hpp:104-121) and the measurement
+    uniform; numpy tensor contractions, no frame algebra."""
+    n = qc.nq
+    psi = np.zeros(2 ** n, dtype=complex)
+    psi[0] = 1
+    r = orc.Rng(seed)
+    w = [1 - p, p / 3, p / 3, p / 3]
+    for tp, g, ctrl, tg, shift, creg, dest in qc.steps:
+        for q in range(n):
+            k = orc.discrete_draw(w, r.canonical())
+            if k:
+                psi = t_apply(psi, PAULI[k], [q], [2] * n)
+        if tp == qa.STEP_MEASURE_MANY:
+            return psi, r.canonical()
+        U = qc.mats[g]
+        if tp == qa.STEP_GATE:
+            psi = t_apply(psi, U, tg, [2] * n)
+        elif tp == qa.STEP_CTRL:
+            psi = orc.applyCTRL(psi, U, ctrl, tg, [2] * n, shift=shift)
+    raise AssertionError("no measurement")
+
+
+def framed_trajectory(orc, qc, plan):
+    """The same trajectory from the plan: overridden 1-qubit matrices, then
+    the final frame i^phase X^x Z^z."""
+    n = qc.nq
+    ovr = dict(zip(plan["steps"], plan["mats"]))
+    psi = np.zeros(2 ** n, dtype=complex)
+    psi[0] = 1
+    for s, (tp, g, ctrl, tg, shift, creg, dest) in enumerate(qc.steps):
+        if tp == qa.STEP_MEASURE_MANY:
+            break
+        U = ovr.get(s, qc.mats[g])
+        if tp == qa.STEP_GATE:
+            psi = t_apply(psi, U, tg, [2] * n)
+        elif tp == qa.STEP_CTRL:
+            psi = orc.applyCTRL(psi, U, ctrl, tg, [2] * n, shift=shift)
+    idx = np.arange(2 ** n)
+    zsign = np.where(np.array([bin(int(i) & plan["zmask"]).count("1") % 2 for i in idx]) == 1, -1, 1)
+    psi = psi * zsign
+    psi = psi[idx ^ plan["xmask"]]
+    return psi * (1j ** plan["phase"])
+
+
+@pytest.mark.parametrize("extra", [False, True])
+def test_plan_matches_explicit_trajectory(orc, extra):
+    n = 12
+    qc = noisy_circuit(n, 2, seed=5, cz=True, extra=extra)
+    b = qa.Batch([qc], dtype=qa.C128)
+    nerr = 0
+    for seed in range(6):
+        p = 0.06
+        plan = b.plan(0, seed, p)
+        ref, u = explicit_trajectory(orc, qc, seed, p)
+        got = framed_trajectory(orc, qc, plan)
+        assert plan["u"] == u
+        assert np.max(np.abs(got - ref)) < 1e-12
+        nerr += len(plan["steps"])
+    assert nerr > 10  # the noise actually fired
+
+
+def test_plan_noiseless_draws_only_the_measurement(orc):
+    qc = noisy_circuit(12, 1, seed=1)
+    b = qa.Batch([qc])
+    plan = b.plan(0, 77, -1.0)
+    assert plan["steps"] == [] and plan["xmask"] == 0 and plan["phase"] == 0
+    assert plan["u"] == orc.Rng(77).canonical()
+
+
+def test_batch_rejects_unsupported_circuits():
+    rng = np.random.default_rng(0)
+    qc = qa.QCircuit(12, 12)
+    qc.gate(haar(4, rng), 0, 1)  # dense non-Clifford 2-qubit gate
+    qc.measure(list(range(12)), 0)
+    with pytest.raises(qa.NotImplemented_):
+        qa.Batch([qc])
+    qc2 = qa.QCircuit(12, 12)
+    qc2.gate(H, 0).measure([0, 1], 0)  # not every qubit measured
+    with pytest.raises(qa.NotImplemented_):
+        qa.Batch([qc2])
+    small = qa.QCircuit(6, 6)
+    small.gate(H, 0).measure(list(range(6)), 0)
+    with pytest.raises(qa.NotImplemented_):
+        qa.Batch([small])
+
+
+# ---------------------------------------------------------------------------
+# device
+# ---------------------------------------------------------------------------
+gpu = pytest.mark.gpu
+
+
+def oracle_outcome(orc, qc, seed, p):
+    stats, dits, probs, st = orc.engine_run(qc, reps=1, noise_p=p, seed=seed)
+    return tuple(dits)
+
+
+@gpu
+@pytest.mark.parametrize("p", [0.01, 1e-3, -1.0])
+def test_batch_vs_oracle_engine_mixed_sizes(orc, p):
+    circs = [noisy_circuit(12, 3, seed=11, cz=True), noisy_circuit(13, 2, seed=12, extra=True)]
+    b = qa.Batch(circs, dtype=qa.C128)
+    rng = np.random.default_rng(3)
+    ntr = 40
+    cof = np.array([i % 2 for i in range(ntr)], dtype=np.uint32)
+    seeds = rng.integers(0, 2 ** 32, size=ntr, dtype=np.uint64)
+    idx, prob = b.run(cof, seeds, p)
+    print(b.info(0), b.info(1))
+    for i in range(ntr):
+        c = int(cof[i])
+        got = b.dits(c, idx[i])
+        assert got == oracle_outcome(orc, circs[c], int(seeds[i]), p), (i, c, int(seeds[i]))
+        assert 0 < prob[i] <= 1
+
+
+@gpu
+def test_batch_c64_agrees_with_c128():
+    circs = [noisy_circuit(12, 4, seed=21), noisy_circuit(16, 3, seed=22), noisy_circuit(20, 2, seed=23)]
+    rng = np.random.default_rng(4)
+    ntr = 300
+    cof = rng.integers(0, 3, size=ntr).astype(np.uint32)
+    seeds = rng.integers(0, 2 ** 32, size=ntr, dtype=np.uint64)
+    i64, _ = qa.Batch(circs, dtype=qa.C128).run(cof, seeds, 1e-3)
+    b32 = qa.Batch(circs, dtype=qa.C64)
+    i32, _ = b32.run(cof, seeds, 1e-3)
+    same = np.mean(i64 == i32)
+    print("c64 == c128 outcomes:", same, b32.info(2))
+    assert same >= 0.97
+
+
+@gpu
+def test_batch_rerun_is_deterministic_and_order_free():
+    circs = [noisy_circuit(12, 2, seed=31), noisy_circuit(14, 2, seed=32)]
+    b = qa.Batch(circs, dtype=qa.C128)
+    cof = np.array([0, 1, 1, 0, 1, 0, 0, 1], dtype=np.uint32)
+    seeds = np.arange(100, 108, dtype=np.uint64)
+    i1, p1 = b.run(cof, seeds, 0.02)
+    i2, p2 = b.run(cof[::-1].copy(), seeds[::-1].copy(), 0.02)
+    assert np.array_equal(i1, i2[::-1]) and np.array_equal(p1, p2[::-1])
+    # one trajectory alone gives the same outcome as inside the batch
+    for k in (0, 5):
+        ia, pa = b.run(cof[k:k + 1], seeds[k:k + 1], 0.02)
+        assert ia[0] == i1[k] and pa[0] == p1[k]
