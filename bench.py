@@ -96,6 +96,204 @@ def run_gates(qa, st, gates):
     apply_gates(st, gates)
 
 
+def bench_batch(args, qa):
+    """Batched noisy trajectories: 4096 x 16 qubits + 512 x {12, 20, 22}
+    qubits, depth 30 (Haar 1q on every qubit + CNOT ladder), depolarizing
+    p = 1e-3 before every step on every qubit (QNoisyEngine), measurement of
+    every qubit; one persistent launch per run."""
+    from qpp_amd import workloads
+    sizes = [(16, 4096), (12, 512), (20, 512), (22, 512)]
+    scale = args.traj_scale
+    depth = args.traj_depth
+    p = 1e-3
+    dt = qa.C64 if args.dtype == "c64" else qa.C128
+    circs = [workloads.trajectory_circuit(qa, n, depth, 300 + n) for n, _ in sizes]
+    cof = np.concatenate([np.full(max(1, int(c * scale)), i, dtype=np.uint32)
+                          for i, (_, c) in enumerate(sizes)])
+    seeds = np.arange(len(cof), dtype=np.uint64) + 1000
+    ntraj = len(cof)
+    es = 8 if dt == qa.C64 else 16
+    b = qa.Batch(circs, dtype=dt)
+    passes = [b.info(i)["passes"] for i in range(len(circs))]
+    algo_bytes = float(sum(passes[c] * 2 * es * 2 ** sizes[c][0] for c in cof))
+    for w in range(args.warmup):
+        b.run(cof, seeds, p)
+        log(f"warmup {w + 1}/{args.warmup}: {b.info(0)}")
+    t0 = time.perf_counter()
+    dev_ms = []
+    for _ in range(args.steps):
+        idx, prob = b.run(cof, seeds, p)
+        dev_ms.append(b.info(0)["device_ms"])
+    wall = (time.perf_counter() - t0) / args.steps
+    inf = b.info(0)
+    log(f"batch: {ntraj} trajectories, wall {wall * 1e3:.1f} ms/run, kernel {np.mean(dev_ms):.1f} ms, "
+        f"host plan {inf['host_ms']:.1f} ms, exact replays {inf['exact_replays']}")
+
+    # the plain GPU path on a sample: one QNoisyEngine per trajectory (c128)
+    plain = {}
+    for i, (n, c) in enumerate(sizes):
+        k = 2 if n >= 20 else 4
+        tt = time.perf_counter()
+        for j in range(k):
+            qa.QEngine(circs[i], noise_p=p, seed=int(seeds[np.nonzero(cof == i)[0][j]])).execute(1)
+        qa.synchronize()
+        plain[n] = (time.perf_counter() - tt) / k
+        log(f"plain engine {n}q: {plain[n] * 1e3:.1f} ms/trajectory")
+    plain_sec = sum(plain[sizes[c][0]] for c in cof)
+    # precision: the noiseless circuits in c64 vs c128 (FP64 comparison)
+    prec = {}
+    for i, (n, _) in enumerate(sizes):
+        gates = workloads.ladder_layers(n, depth, 300 + n)
+        s64 = qa.State([2] * n)
+        qa.Program(n, gates).run(s64)
+        s32 = qa.State([2] * n, dtype=qa.C64)
+        qa.Program(n, gates, dtype=qa.C64).run(s32)
+        r = s64.compare(s32)
+        prec[f"{n}q"] = {"infidelity": r["infidelity"], "trace_distance": r["trace_distance"],
+                         "max_abs": r["max_abs"]}
+    # outcome agreement with the c128 batch on the same seeds
+    sub = min(ntraj, 1024)
+    ref_idx, _ = qa.Batch(circs, dtype=qa.C128).run(cof[:sub], seeds[:sub], p)
+    agree = float(np.mean(ref_idx == idx[:sub]))
+    # oracle QNoisyEngine on a few 12-qubit trajectories (same seeds)
+    import oracle
+    i12 = [j for j in range(ntraj) if cof[j] == 1][: args.oracle_traj]
+    om = sum(1 for j in i12
+             if tuple(oracle.engine_run(circs[1], 1, p, int(seeds[j]))[1]) == b.dits(1, idx[j]))
+    achieved = algo_bytes / (np.mean(dev_ms) / 1e3) / 1e9
+    return {
+        "metric": "batched noisy trajectories/s (mixed 16/12/20/22 qubits)",
+        "value": round(ntraj / wall, 2),
+        "unit": "trajectories/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64" if dt == qa.C64 else "complex128",
+        "data": "synthetic",
+        "config": {"workload": f"{ntraj} QNoisyEngine trajectories "
+                               f"({', '.join(f'{max(1, int(c * scale))}x{n}q' for n, c in sizes)}), "
+                               f"depth {depth}, depolarizing p={p}, measure all",
+                   "passes_per_circuit": dict(zip([f"{n}q" for n, _ in sizes], passes))},
+        "plain_gpu_trajectories_per_s": round(ntraj / plain_sec, 3),
+        "speedup_vs_plain": round(plain_sec / wall, 2),
+        "host_plan_ms": round(inf["host_ms"], 2),
+        "kernel_ms": round(float(np.mean(dev_ms)), 3),
+        "exact_replays": inf["exact_replays"],
+        "precision_noiseless_c64_vs_c128": prec,
+        "outcomes_equal_c128_batch": agree,
+        "oracle_outcomes_matched": f"{om}/{len(i12)} (12-qubit trajectories)",
+        "roofline": {"bound": "hbm", "kernel": "k_traj", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(float(np.mean(dev_ms)), 3)},
+        "cpu_baseline": None,
+    }
+
+
+def bench_serve(args, qa):
+    """Serving: Poisson arrivals of random circuits (n ~ U{10..26}, depth ~
+    U{10..100}, layers as the plain workload, one measurement of all qubits),
+    1024 shots each (QEngine.execute(1024)), dtype complex128/complex64 50/50,
+    admitted by HBM footprint; rate = 0.8 x the capacity measured by a
+    closed-loop calibration batch."""
+    from qpp_amd import workloads
+    rng = np.random.default_rng(args.seed)
+    nreq = args.requests
+
+    def make(k, r):
+        n = int(r.integers(10, 27))
+        depth = int(r.integers(10, 101))
+        dt = qa.C64 if r.random() < 0.5 else qa.C128
+        return {"n": n, "depth": depth, "dtype": dt, "seed": int(r.integers(1, 2 ** 31)),
+                "gseed": 7000 + k}
+
+    reqs = [make(k, rng) for k in range(nreq)]
+    circs = [workloads.measured_circuit(qa, q["n"], q["depth"], q["gseed"]) for q in reqs]
+    srv = qa.Server(lanes=args.lanes)
+    # closed-loop calibration on a separate sample of the same distribution
+    crng = np.random.default_rng(args.seed + 1)
+    cal = [make(10000 + k, crng) for k in range(args.cal_requests)]
+    ccircs = [workloads.measured_circuit(qa, q["n"], q["depth"], q["gseed"]) for q in cal]
+    t0 = time.perf_counter()
+    ts = [srv.submit(c, q["dtype"], 1024, q["seed"]) for c, q in zip(ccircs, cal)]
+    for t in ts:
+        srv.wait(t)
+        srv.release(t)
+    cap = len(cal) / (time.perf_counter() - t0)
+    lam = 0.8 * cap
+    log(f"serve: capacity {cap:.1f} circuits/s (closed loop, {len(cal)} requests); rate {lam:.1f}/s")
+    gaps = rng.exponential(1.0 / lam, size=nreq)
+    arrivals = np.cumsum(gaps)
+    tickets = []
+    start = time.perf_counter()
+    for k, (c, q) in enumerate(zip(circs, reqs)):
+        now = time.perf_counter() - start
+        if arrivals[k] > now:
+            time.sleep(arrivals[k] - now)
+        tickets.append(srv.submit(c, q["dtype"], 1024, q["seed"]))
+        if (k + 1) % 50 == 0:
+            log(f"serve: submitted {k + 1}/{nreq}, {srv.info()}")
+    lat, stats = [], []
+    for t in tickets:
+        lat.append(srv.wait(t)[2])
+        stats.append(srv.stats(t))
+    wall = time.perf_counter() - start
+    info = srv.info()
+    srv.close()
+    lat = np.array(lat)
+    log(f"serve: {nreq} requests in {wall:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms, {info}")
+    # each result alone == in the mix (a few requests re-run alone)
+    same = 0
+    chk = list(range(0, nreq, max(1, nreq // 6)))[:6]
+    for k in chk:
+        e = qa.QEngine(circs[k], dtype=reqs[k]["dtype"], seed=reqs[k]["seed"]).execute(1024)
+        same += int(e.get_stats() == stats[k][0])
+    # precision of the complex64 requests: the gate part in c64 vs c128
+    ok64, n64, worst = 0, 0, 0.0
+    pass_bytes = 0.0
+    for q in reqs:
+        gates = workloads.random_layers(q["n"], q["depth"], q["gseed"])
+        p = qa.Program(q["n"], gates, dtype=q["dtype"])
+        pass_bytes += p.num_passes() * 2 * (8 if q["dtype"] == qa.C64 else 16) * 2 ** q["n"]
+        if q["dtype"] != qa.C64:
+            continue
+        s32 = qa.State([2] * q["n"], dtype=qa.C64)
+        p.run(s32)
+        s64 = qa.State([2] * q["n"])
+        qa.Program(q["n"], gates).run(s64)
+        inf = s64.compare(s32)["infidelity"]
+        worst = max(worst, inf)
+        n64 += 1
+        ok64 += int(inf <= 1e-5)
+    frac = (ok64 + (nreq - n64)) / nreq  # complex128 requests: exact to rounding
+    return {
+        "metric": "served circuits/s + p99 latency (1024 shots each)",
+        "value": round(nreq / wall, 3),
+        "unit": "circuits/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(wall * 1e3 / nreq, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128+complex64", "data": "synthetic",
+        "config": {"workload": f"{nreq} Poisson-arriving random circuits, n~U{{10..26}}, "
+                               f"depth~U{{10..100}}, 1024 shots, c128/c64 50/50, "
+                               f"{args.lanes} lanes",
+                   "arrival_rate_per_s": round(lam, 3), "capacity_per_s": round(cap, 3)},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2),
+                       "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
+        "fraction_at_infidelity_le_1e-5": round(frac, 4),
+        "c64_requests": n64, "c64_worst_infidelity": worst,
+        "alone_equals_mix": f"{same}/{len(chk)}",
+        "server": info,
+        "roofline": {"bound": "hbm", "kernel": "k_tile (all requests)",
+                     "achieved": round(pass_bytes / wall / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(pass_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,7 +302,18 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=30)
-    ap.add_argument("--mode", choices=["fused", "gates"], default="fused")
+    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve"], default="fused")
+    ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
+    ap.add_argument("--cal-requests", type=int, default=40,
+                    help="serve mode: closed-loop calibration requests")
+    ap.add_argument("--lanes", type=int, default=8, help="serve mode: concurrent lanes")
+    ap.add_argument("--dtype", choices=["c128", "c64"], default="c64",
+                    help="batch mode: complex dtype of the trajectories")
+    ap.add_argument("--traj-scale", type=float, default=1.0,
+                    help="batch mode: fraction of the 4096/512/512/512 trajectories")
+    ap.add_argument("--traj-depth", type=int, default=30)
+    ap.add_argument("--oracle-traj", type=int, default=16,
+                    help="batch mode: 12-qubit trajectories checked against the oracle engine")
     ap.add_argument("--cpu-gates", type=int, default=6,
                     help="gates in the CPU baseline sample (0 = skip)")
     ap.add_argument("--reg-bits", type=int, default=4)
@@ -123,6 +332,12 @@ def main():
 
     info = qa.device_info()
     log("device:", info)
+    if args.mode == "batch":
+        print(json.dumps(bench_batch(args, qa)), flush=True)
+        return
+    if args.mode == "serve":
+        print(json.dumps(bench_serve(args, qa)), flush=True)
+        return
     n, depth = args.qubits, args.depth
     gates = workloads.random_layers(n, depth, args.seed)
     n1q = sum(1 for g in gates if g[0] == "u")

@@ -348,6 +348,38 @@ int qs_batch_plan(const qs_batch* b, uint32_t circuit, uint64_t seed,
                   uint64_t* zmask, uint32_t* phase, double* u);
 int qs_batch_destroy(qs_batch* b);
 
+/* ------------------------------------------------------------------------ */
+/* Serving                                                                   */
+/* ------------------------------------------------------------------------ */
+/* No qpp counterpart (qpp runs one engine per caller thread): a request
+ * server that admits circuits continuously onto the GPU.  Each request is
+ * QEngine(qc, seed).execute(shots) (classes/qengine.hpp:1341-1405) in
+ * complex128 or complex64, with the engine's statistics as result.  `lanes`
+ * worker threads each own a HIP stream; a free lane takes the oldest queued
+ * request whose device footprint (state, its pre-measurement copy and the
+ * FP64 sampling vectors) fits what is left of `hbm_budget` bytes (0 = 90% of
+ * the free device memory), so requests run concurrently and new ones are
+ * admitted as lanes and memory free up.  Results are identical to running
+ * the request alone. */
+typedef struct qs_server qs_server;
+int qs_server_create(uint32_t lanes, uint64_t hbm_budget, qs_server** out);
+/* Non-blocking; the circuit is copied.  QS_ERR_OUT_OF_MEMORY if the request
+ * can never fit the budget. */
+int qs_server_submit(qs_server* s, const qs_circuit* qc, qs_dtype dtype,
+                     uint64_t shots, uint64_t seed, uint64_t* ticket);
+/* Blocks until done; times_ms[3] (optional): queued, running, total. */
+int qs_server_wait(qs_server* s, uint64_t ticket, double* times_ms);
+/* Results of a finished request: outcomes (n x nc dits, QEngine::get_stats
+ * order), their counts and the last classical dits (nc).  Pass NULL arrays
+ * to read n_outcomes first. */
+int qs_server_stats(qs_server* s, uint64_t ticket, uint64_t* n_outcomes,
+                    uint64_t* dits, uint64_t* counts, uint64_t* last_dits);
+int qs_server_release(qs_server* s, uint64_t ticket);
+/* info[6]: completed, queued, running, peak running, bytes held, peak bytes */
+int qs_server_info(qs_server* s, uint64_t* info);
+/* Runs what is queued to completion, then stops the lanes. */
+int qs_server_destroy(qs_server* s);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -857,3 +857,74 @@ class Batch:
                 lib().qs_batch_destroy(self._h)
         except Exception:
             pass
+
+
+class Server:
+    """Request server (qs_server_*): each request is QEngine(qc, seed)
+    .execute(shots) in complex128 or complex64, admitted onto the GPU by
+    device footprint as lanes and memory free up; `lanes` requests run
+    concurrently on their own HIP streams."""
+
+    def __init__(self, lanes=8, hbm_budget=0):
+        L = lib()
+        L.qs_server_create.argtypes = [c_uint32, c_uint64, POINTER(c_void_p)]
+        L.qs_server_submit.argtypes = [c_void_p, POINTER(_Circuit), c_int, c_uint64, c_uint64,
+                                       POINTER(c_uint64)]
+        L.qs_server_wait.argtypes = [c_void_p, c_uint64, POINTER(c_double)]
+        L.qs_server_stats.argtypes = [c_void_p, c_uint64, POINTER(c_uint64), POINTER(c_uint64),
+                                      POINTER(c_uint64), POINTER(c_uint64)]
+        L.qs_server_release.argtypes = [c_void_p, c_uint64]
+        L.qs_server_info.argtypes = [c_void_p, POINTER(c_uint64)]
+        L.qs_server_destroy.argtypes = [c_void_p]
+        self._h = c_void_p()
+        self._nc = {}
+        _check(L.qs_server_create(int(lanes), int(hbm_budget), byref(self._h)))
+
+    def submit(self, qc, dtype=C128, shots=1024, seed=5489):
+        cs = qc.struct()
+        t = c_uint64()
+        _check(lib().qs_server_submit(self._h, byref(cs), dtype, int(shots), int(seed), byref(t)))
+        self._nc[t.value] = qc.nc
+        return t.value
+
+    def wait(self, ticket):
+        """(queued_ms, run_ms, total_ms) of a finished request."""
+        tm = (c_double * 3)()
+        _check(lib().qs_server_wait(self._h, int(ticket), tm))
+        return tuple(tm)
+
+    def stats(self, ticket):
+        """(statistics dict as QEngine.get_stats, last dits)."""
+        n = c_uint64()
+        _check(lib().qs_server_stats(self._h, int(ticket), byref(n), None, None, None))
+        nc = self._nc[ticket]
+        d = np.zeros(max(1, n.value * nc), dtype=np.uint64)
+        c = np.zeros(max(1, n.value), dtype=np.uint64)
+        last = np.zeros(max(1, nc), dtype=np.uint64)
+        _check(lib().qs_server_stats(self._h, int(ticket), byref(n),
+                                     d.ctypes.data_as(POINTER(c_uint64)),
+                                     c.ctypes.data_as(POINTER(c_uint64)),
+                                     last.ctypes.data_as(POINTER(c_uint64))))
+        st = {tuple(int(x) for x in d[i * nc:(i + 1) * nc]): int(c[i]) for i in range(n.value)}
+        return st, [int(x) for x in last[:nc]]
+
+    def release(self, ticket):
+        _check(lib().qs_server_release(self._h, int(ticket)))
+        self._nc.pop(ticket, None)
+
+    def info(self):
+        v = (c_uint64 * 6)()
+        _check(lib().qs_server_info(self._h, v))
+        keys = ["completed", "queued", "running", "peak_running", "bytes_in_use", "peak_bytes"]
+        return dict(zip(keys, [int(x) for x in v]))
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().qs_server_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

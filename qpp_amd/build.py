@@ -23,6 +23,7 @@ SOURCES = [
     "qs_engine.hip",
     "qs_precision.hip",
     "qs_batch.hip",
+    "qs_serve.hip",
 ]
 
 FLAGS = [

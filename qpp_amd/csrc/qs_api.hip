@@ -29,6 +29,7 @@ thread_local std::string g_last_error;
 std::once_flag g_init_flag;
 int g_init_status = QS_OK;
 hipStream_t g_stream = nullptr;
+thread_local hipStream_t t_stream = nullptr;  // per-thread override (serving lanes)
 }  // namespace
 
 const char* description(int code) {
@@ -85,8 +86,20 @@ int ensure_device() {
             g_init_status = QS_ERR_NO_DEVICE;
             return;
         }
-        if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess)
+        if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess) {
             g_init_status = QS_ERR_DEVICE;
+            return;
+        }
+        // State buffers are stream-ordered allocations from the device's
+        // default pool; keep freed blocks cached so a freed state's memory
+        // serves the next request without a trip through the driver.
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = ~uint64_t{0};
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
     });
     if (g_init_status != QS_OK)
         return fail(g_init_status, "qpp_amd::init",
@@ -95,6 +108,13 @@ int ensure_device() {
 }
 
 hipStream_t default_stream() { return g_stream; }
+
+hipStream_t current_stream() { return t_stream ? t_stream : g_stream; }
+
+void set_thread_stream(hipStream_t s) { t_stream = s; }
+
+StreamScope::StreamScope(hipStream_t s) : old(t_stream) { t_stream = s; }
+StreamScope::~StreamScope() { t_stream = old; }
 
 }  // namespace qs
 
@@ -350,14 +370,14 @@ int qs_state_create(qs_dtype dtype, qs_kind kind, uint32_t ndims,
     st->nelem = kind == QS_KET ? st->D : st->D * st->D;
     st->qubits = std::all_of(dv.begin(), dv.end(), [](uint64_t d) { return d == 2; });
     st->nbits = st->qubits ? static_cast<uint32_t>((kind == QS_KET ? 1 : 2) * dv.size()) : 0;
-    st->stream = qs::default_stream();
-    hipError_t e = hipMalloc(&st->d, st->bytes());
+    st->stream = qs::current_stream();
+    hipError_t e = hipMallocAsync(&st->d, st->bytes(), st->stream);
     if (e != hipSuccess) {
         delete st;
         return qs::fail_hip(e, "qs_state_create");
     }
     if (int rc = set_basis(st, 0)) {
-        (void)hipFree(st->d);
+        (void)hipFreeAsync(st->d, st->stream);
         delete st;
         return rc;
     }
@@ -365,18 +385,18 @@ int qs_state_create(qs_dtype dtype, qs_kind kind, uint32_t ndims,
     return QS_OK;
 }
 
+// The buffer goes back to the pool in stream order after the state's last
+// queued operation; the host does not wait.
 int qs_state_destroy(qs_state* st) {
     if (!st) return QS_OK;
-    if (st->d) {
-        (void)hipStreamSynchronize(st->stream);
-        (void)hipFree(st->d);
-    }
+    if (st->d) (void)hipFreeAsync(st->d, st->stream);
     delete st;
     return QS_OK;
 }
 
 int qs_state_clone(const qs_state* st, qs_state** out) {
     if (!valid(st) || !out) return fail(QS_ERR_INVALID_HANDLE, "qs_state_clone");
+    const qs::StreamScope on(st->stream);  // the copy lives on the source's stream
     if (int rc = qs_state_create(st->dtype, st->kind,
                                  static_cast<uint32_t>(st->dims.size()),
                                  st->dims.data(), out))

@@ -18,8 +18,8 @@
 // All trajectories of a circuit therefore share one compiled tile program
 // and differ only in a short, sorted list of 2x2 matrix overrides.
 //
-// Device side: one kernel, one workgroup per trajectory at a time (static
-// round robin over trajectories sorted by cost, largest first).  The
+// Device side: one kernel, one workgroup per trajectory at a time, taken
+// from a work queue of trajectories sorted by cost, largest first.  The
 // workgroup runs every pass of its circuit's program over its own state
 // slot (tiles streamed through registers/LDS as in k_tile; the first pass
 // synthesises |0..0> instead of loading), then samples the measurement:
@@ -100,13 +100,21 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                                                    const TrajIn* __restrict__ tin,
                                                    const Ovr* __restrict__ ovr,
                                                    const uint32_t* __restrict__ order,
-                                                   uint32_t ntraj, TrajOut* __restrict__ out) {
+                                                   uint32_t ntraj, TrajOut* __restrict__ out,
+                                                   uint32_t* __restrict__ next) {
     extern __shared__ unsigned char smem_raw[];
     cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
     __shared__ double pre[kThreads + 1];
+    __shared__ uint32_t s_k;
     const uint32_t t = threadIdx.x;
     cxt<R>* __restrict__ a = ws + uint64_t(blockIdx.x) * slot;
-    for (uint32_t k = blockIdx.x; k < ntraj; k += gridDim.x) {
+    // work queue: trajectories in `order` (largest first), claimed one at a
+    // time; every workgroup leaves once the queue is exhausted
+    for (;;) {
+        if (t == 0) s_k = atomicAdd(next, 1u);
+        __syncthreads();
+        const uint32_t k = s_k;
+        if (k >= ntraj) break;
         const uint32_t tr = order[k];
         const TrajIn ti = tin[tr];
         const BClass c = cls[ti.cls];
@@ -253,6 +261,8 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         __syncthreads();  // the slot and `pre` are reused by the next trajectory
     }
 }
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0; }
 
 // ---------------------------------------------------------------------------
 // host: Pauli frames
@@ -519,6 +529,7 @@ struct qs_batch {
     size_t c_order = 0;
     void* d_out = nullptr;
     size_t c_out = 0;
+    uint32_t* d_next = nullptr;  // work-queue counter
     void* d_ws = nullptr;
     size_t c_ws = 0;
     // class -> offsets of its ops in the concatenated table
@@ -707,7 +718,8 @@ int qs_batch_create(const qs_circuit* const* circuits, uint32_t ncircuits, qs_dt
 int qs_batch_destroy(qs_batch* b) {
     if (!b) return QS_OK;
     if (b->d_tab || b->d_ws) (void)hipStreamSynchronize(qs::default_stream());
-    for (void* p : {b->d_tab, b->d_tin, b->d_ovr, b->d_order, b->d_out, b->d_ws})
+    for (void* p : {b->d_tab, b->d_tin, b->d_ovr, b->d_order, b->d_out, b->d_ws,
+                    static_cast<void*>(b->d_next)})
         if (p) (void)hipFree(p);
     delete b;
     return QS_OK;
@@ -788,7 +800,7 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
         ti.xmask = plans[i].xmask;
         ti.u = plans[i].u;
     }
-    // largest work first, dealt round robin over the workgroups
+    // queue order: largest work first (greedy longest-processing-time)
     std::vector<uint32_t> order(ntraj);
     std::iota(order.begin(), order.end(), 0u);
     auto cost = [&](uint32_t i) {
@@ -842,6 +854,7 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
         QS_HIP(hipMalloc(&b->d_out, ob), W);
         b->c_out = ob;
     }
+    if (!b->d_next) QS_HIP(hipMalloc(&b->d_next, 256), W);
     hipEvent_t e0, e1;
     QS_HIP(hipEventCreate(&e0), W);
     QS_HIP(hipEventCreate(&e1), W);
@@ -852,24 +865,25 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
     auto* ordp = static_cast<const uint32_t*>(b->d_order);
     auto* outp = static_cast<TrajOut*>(b->d_out);
     const uint32_t nt = static_cast<uint32_t>(ntraj);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, b->d_next);
     if (b->dtype == QS_C128) {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<double, 8>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp);
+                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
         else
             hipLaunchKernelGGL((k_traj<double, 16>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp);
+                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
     } else {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<float, 8>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp);
+                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
         else
             hipLaunchKernelGGL((k_traj<float, 16>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp);
+                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, s);

@@ -580,7 +580,10 @@ int qs_engine_reset(qs_engine* e) {
     if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::reset()");
     std::vector<uint64_t> dims(e->nq, e->d);
     qs_state* s = nullptr;
-    if (int rc = qs_state_create(e->dtype, e->kind, e->nq, dims.data(), &s)) return rc;
+    {
+        const qs::StreamScope on(e->st->stream);
+        if (int rc = qs_state_create(e->dtype, e->kind, e->nq, dims.data(), &s)) return rc;
+    }
     qs_state_destroy(e->st);
     e->st = s;
     e->reset_bookkeeping();

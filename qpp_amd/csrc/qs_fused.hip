@@ -425,8 +425,12 @@ int fused_upload(FusedProgram& P, qs_dtype dtype) {
             mm[i] = make_float2(static_cast<float>(P.mats[i].real()), static_cast<float>(P.mats[i].imag()));
         o_m = put(mm.data(), mm.size() * 8);
     }
-    QS_HIP(hipMalloc(&P.dblob, blob.size()), "qpp_amd::fused");
-    QS_HIP(hipMemcpy(P.dblob, blob.data(), blob.size(), hipMemcpyHostToDevice), "qpp_amd::fused");
+    // stream-ordered: no device-wide synchronisation (serving lanes overlap)
+    P.stream = P.last = qs::current_stream();
+    QS_HIP(hipMallocAsync(&P.dblob, blob.size(), P.stream), "qpp_amd::fused");
+    QS_HIP(hipMemcpyAsync(P.dblob, blob.data(), blob.size(), hipMemcpyHostToDevice, P.stream),
+           "qpp_amd::fused");
+    QS_HIP(hipStreamSynchronize(P.stream), "qpp_amd::fused");  // `blob` is freed on return
     auto* b = static_cast<unsigned char*>(P.dblob);
     P.d_passes = reinterpret_cast<FPass*>(b + o_p);
     P.d_phases = reinterpret_cast<FPhase*>(b + o_h);
@@ -437,7 +441,11 @@ int fused_upload(FusedProgram& P, qs_dtype dtype) {
 }
 
 void fused_free(FusedProgram& P) {
-    if (P.dblob) (void)hipFree(P.dblob);
+    if (P.dblob) {
+        // runs on another stream must finish before the blob returns to the pool
+        if (P.last && P.last != P.stream) (void)hipStreamSynchronize(P.last);
+        (void)hipFreeAsync(P.dblob, P.stream);
+    }
     P.dblob = nullptr;
 }
 
@@ -456,6 +464,11 @@ int launch_pass(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap
 int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap) {
     if (!P.dblob) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::fused", "program not uploaded");
     if (st->dtype != P.dtype) return fail(QS_ERR_TYPE_MISMATCH, "qpp_amd::fused", "dtype");
+    if (st->stream != P.stream) {
+        // the upload finished on P.stream (synchronised); later runs on other
+        // streams are remembered so the free can wait for them
+        P.last = st->stream;
+    }
     if (!st->qubits || st->nbits != P.nb)
         return fail(QS_ERR_DIMS_MISMATCH_CVECTOR, "qpp_amd::fused", "state/program");
     for (size_t p = 0; p < P.passes.size(); ++p) {

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/qpp_amd.h"
 
 enum FOpKind : int {
@@ -83,6 +85,8 @@ struct FusedProgram {
     std::vector<size_t> gates_per_pass;
     std::vector<int> gate_op;  // gate index -> index of its op in `ops`
     qs_dtype dtype = QS_C128;
+    hipStream_t stream = nullptr;       // stream of the upload (and of the free)
+    mutable hipStream_t last = nullptr;  // stream of the last run
     void* dblob = nullptr;
     FPass* d_passes = nullptr;
     FPhase* d_phases = nullptr;

@@ -55,6 +55,17 @@ const char* description(int code);
 // default stream / device bring-up (returns QS_OK or an error code)
 int ensure_device();
 hipStream_t default_stream();
+// stream new states are created on: the calling thread's override if set
+// (a serving lane), else the library stream
+hipStream_t current_stream();
+void set_thread_stream(hipStream_t s);
+// creates states on `s` for the scope (outputs derived from a state live on
+// that state's stream)
+struct StreamScope {
+    hipStream_t old;
+    explicit StreamScope(hipStream_t s);
+    ~StreamScope();
+};
 
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in the .hip files).  `m` is a column-major

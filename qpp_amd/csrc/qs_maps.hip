@@ -246,13 +246,10 @@ uint32_t grid_for(uint64_t work, uint64_t per) {
 
 int make_out(const qs_state* st, qs_kind kind, const std::vector<uint64_t>& dims,
              qs_state** out) {
-    if (int rc = qs_state_create(st->dtype, kind, static_cast<uint32_t>(dims.size()),
-                                 dims.data(), out))
-        return rc;
-    // the new state must not be touched before `st`'s pending work is done:
-    // both live on the library stream
-    (*out)->stream = st->stream;
-    return QS_OK;
+    // created on `st`'s stream: kernels reading `st` write it in stream order
+    const qs::StreamScope on(st->stream);
+    return qs_state_create(st->dtype, kind, static_cast<uint32_t>(dims.size()), dims.data(),
+                           out);
 }
 
 // out[y] = in[G(y)], src[b] = input bit of output bit b

@@ -223,6 +223,15 @@ __global__ __launch_bounds__(256) void k_div(const double* __restrict__ p,
 
 // The reference's cumulative table, replayed in order, for the shots sorted
 // by u (order[] gives the shot of each sorted position).  One block.
+// p[idx[s]] for every shot
+__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ p,
+                                                const uint64_t* __restrict__ idx, uint64_t n,
+                                                double* __restrict__ out) {
+    const uint64_t stride = uint64_t(gridDim.x) * 256;
+    for (uint64_t s = uint64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += stride)
+        out[s] = p[idx[s]];
+}
+
 __global__ __launch_bounds__(256) void k_serial_exact(const double* __restrict__ q,
                                                       uint64_t D,
                                                       const double* __restrict__ u,
@@ -475,10 +484,20 @@ double bound_for(uint64_t N) {
 
 // Indices of nshots draws.  u_dev: device uniforms.  Results in idx_dev.
 int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
-                  std::vector<uint64_t>& out_idx) {
+                  std::vector<uint64_t>& out_idx, double* out_prob_host = nullptr) {
     const uint64_t D = st->D;
     out_idx.assign(nshots, 0);
-    if (D < 2) return QS_OK;  // a one-element table returns 0 (random.tcc:2700)
+    if (D < 2) {  // a one-element table returns 0 (random.tcc:2700)
+        if (out_prob_host) {
+            if (int rc = g_p.need(8)) return rc;
+            if (int rc = probs_device(st, static_cast<double*>(g_p.p))) return rc;
+            QS_HIP(hipMemcpyAsync(out_prob_host, g_p.p, 8, hipMemcpyDeviceToHost, st->stream),
+                   "qpp::sample()");
+            QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+            for (uint64_t s = 1; s < nshots; ++s) out_prob_host[s] = out_prob_host[0];
+        }
+        return QS_OK;
+    }
     const uint64_t nchunks = (D + kChunk - 1) / kChunk;
     // layout in g_aux: sums[nchunks] | prefix[nchunks+1] | u[nshots] |
     // idx[nshots] | certain[nshots] | order[nshots] | s
@@ -517,10 +536,23 @@ int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
     QS_HIP(hipMemcpyAsync(hc.data(), cert, nshots * 4, hipMemcpyDeviceToHost, st->stream),
            "qpp::sample()");
     QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+    // p[index] of every shot (instruments.hpp:808), gathered on the device;
+    // the u slots are free once the search has run
+    auto probs_out = [&]() -> int {
+        if (!out_prob_host) return QS_OK;
+        hipLaunchKernelGGL(k_gather, dim3(grid_for(nshots, 256)), dim3(256), 0, st->stream, p,
+                           idx, nshots, ud);
+        QS_HIP(hipGetLastError(), "qpp::sample()");
+        QS_HIP(hipMemcpyAsync(out_prob_host, ud, nshots * 8, hipMemcpyDeviceToHost,
+                              st->stream),
+               "qpp::sample()");
+        QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+        return QS_OK;
+    };
     std::vector<uint32_t> amb;
     for (uint64_t s = 0; s < nshots; ++s)
         if (!hc[s]) amb.push_back(static_cast<uint32_t>(s));
-    if (amb.empty()) return QS_OK;
+    if (amb.empty()) return probs_out();
     // exact replay for the ambiguous shots, sorted by u
     std::sort(amb.begin(), amb.end(), [&](uint32_t a, uint32_t b) {
         return u_host[a] < u_host[b] || (u_host[a] == u_host[b] && a < b);
@@ -540,7 +572,7 @@ int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
                           st->stream),
            "qpp::sample()");
     QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
-    return QS_OK;
+    return probs_out();
 }
 
 int project(qs_state* st, uint64_t stride, uint64_t dim, uint64_t m) {
@@ -569,7 +601,7 @@ int contract(qs_state* st, uint64_t i, uint64_t m) {
     const int rho = st->kind == QS_RHO;
     const uint64_t nout = rho ? Dn * Dn : Dn;
     void* nbuf = nullptr;
-    QS_HIP(hipMalloc(&nbuf, nout * st->elem_bytes()), "qpp::measure_seq()");
+    QS_HIP(hipMallocAsync(&nbuf, nout * st->elem_bytes(), st->stream), "qpp::measure_seq()");
     if (st->dtype == QS_C128)
         hipLaunchKernelGGL(k_contract<double2>, dim3(grid_for(nout, 256)), dim3(256), 0,
                            st->stream, static_cast<const double2*>(st->d),
@@ -581,8 +613,8 @@ int contract(qs_state* st, uint64_t i, uint64_t m) {
                            static_cast<float2*>(nbuf), nout, Dn, st->D, stride, dim, m,
                            rho);
     QS_HIP(hipGetLastError(), "qpp::measure_seq()");
+    (void)hipFreeAsync(st->d, st->stream);
     QS_HIP(hipStreamSynchronize(st->stream), "qpp::measure_seq()");
-    (void)hipFree(st->d);
     st->d = nbuf;
     st->dims = nd;
     st->D = Dn;
@@ -645,7 +677,7 @@ int qs_sample(const qs_state* st, uint32_t ntarget, const uint64_t* target,
     if (nshots == 0) return fail(QS_ERR_OUT_OF_RANGE, W, "num_samples");
     if (!u) return fail(QS_ERR_ZERO_SIZE, W, "u");
     std::vector<uint64_t> idx;
-    if (int rc = sample_device(st, u, nshots, idx)) return rc;
+    if (int rc = sample_device(st, u, nshots, idx, out_prob)) return rc;
     if (out_index) std::copy(idx.begin(), idx.end(), out_index);
     const uint64_t n = st->dims.size();
     if (out_digits)
@@ -658,14 +690,6 @@ int qs_sample(const qs_state* st, uint32_t ntarget, const uint64_t* target,
             }
             for (uint32_t t = 0; t < ntarget; ++t) out_digits[s * ntarget + t] = mid[tv[t]];
         }
-    if (out_prob) {
-        // p[index] of each shot (instruments.hpp:808)
-        for (uint64_t s = 0; s < nshots; ++s)
-            QS_HIP(hipMemcpyAsync(out_prob + s, static_cast<double*>(g_p.p) + idx[s], 8,
-                                  hipMemcpyDeviceToHost, st->stream),
-                   W);
-        QS_HIP(hipStreamSynchronize(st->stream), W);
-    }
     return QS_OK;
 }
 

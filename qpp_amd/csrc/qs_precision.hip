@@ -147,9 +147,12 @@ int qs_state_convert(const qs_state* st, qs_dtype dtype, qs_state** out) {
     if (!valid(st) || !out) return fail(QS_ERR_INVALID_HANDLE, W);
     if (dtype != QS_C128 && dtype != QS_C64) return fail(QS_ERR_TYPE_MISMATCH, W, "dtype");
     qs_state* r = nullptr;
-    if (int rc = qs_state_create(dtype, st->kind, static_cast<uint32_t>(st->dims.size()),
-                                 st->dims.data(), &r))
-        return rc;
+    {
+        const qs::StreamScope on(st->stream);
+        if (int rc = qs_state_create(dtype, st->kind, static_cast<uint32_t>(st->dims.size()),
+                                     st->dims.data(), &r))
+            return rc;
+    }
     QS_HIP(hipStreamSynchronize(st->stream), W);
     const uint64_t n = st->nelem;
     const unsigned grid = grid_for(n);

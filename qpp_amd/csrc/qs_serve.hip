@@ -1,0 +1,307 @@
+// qs_serve.hip -- request server: continuous admission of circuits onto one
+// GPU, states resident for the life of a request, overlapping execution.
+//
+// A request is QEngine(circuit, seed).execute(shots) (classes/qengine.hpp:
+// 1341-1405) in complex128 or complex64.  The server keeps `lanes` worker
+// threads, each bound to its own HIP stream (default priority); a worker
+// takes the oldest queued request whose device footprint fits the free part
+// of the HBM budget (first fit: a large request waiting for memory does not
+// block small ones behind it), runs it to completion on its stream, releases
+// its memory and takes the next.  Requests that arrive while others run are
+// admitted as soon as a lane and memory are free, so work from several
+// requests is on the device at once.  All device memory of a request is
+// stream-ordered (hipMallocAsync from a caching pool), so admitting or
+// retiring a request never synchronises the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "qs_internal.hpp"
+
+namespace {
+
+using qs::fail;
+using Clock = std::chrono::steady_clock;
+
+// owned copy of a qs_circuit
+struct CircuitCopy {
+    std::vector<qs_step> steps;
+    std::vector<uint64_t> mat_dim, mat_off, pool;
+    std::vector<double> mats;
+    qs_circuit c{};
+    void take(const qs_circuit* q) {
+        steps.assign(q->steps, q->steps + q->nsteps);
+        mat_dim.assign(q->mat_dim, q->mat_dim + q->nmats);
+        mat_off.assign(q->mat_off, q->mat_off + q->nmats);
+        uint64_t nm = 0, np = 0;
+        for (uint32_t i = 0; i < q->nmats; ++i)
+            nm = std::max(nm, 2 * (q->mat_off[i] + q->mat_dim[i] * q->mat_dim[i]));
+        mats.assign(q->mats, q->mats + nm);
+        for (uint32_t s = 0; s < q->nsteps; ++s) {
+            const qs_step& st = q->steps[s];
+            np = std::max<uint64_t>(np, uint64_t(st.ctrl_off) + st.nctrl);
+            np = std::max<uint64_t>(np, uint64_t(st.target_off) + st.ntarget);
+            if (st.shift_off != UINT32_MAX) np = std::max<uint64_t>(np, uint64_t(st.shift_off) + st.nctrl);
+        }
+        pool.assign(q->pool, q->pool + np);
+        c = *q;
+        c.steps = steps.data();
+        c.mat_dim = mat_dim.data();
+        c.mat_off = mat_off.data();
+        c.mats = mats.data();
+        c.pool = pool.data();
+    }
+};
+
+struct Request {
+    uint64_t id = 0;
+    CircuitCopy circ;
+    qs_dtype dtype = QS_C128;
+    uint64_t shots = 1, seed = 0;
+    uint64_t footprint = 0;
+    Clock::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = QS_OK;
+    std::string error;
+    std::vector<uint64_t> stat_dits, stat_counts;  // outcomes (n x nc) and counts
+    std::vector<uint64_t> dits;
+    uint32_t lane = 0;
+};
+
+// Device bytes a request holds at most: the state, the pre-measurement
+// copy the sampling tail keeps (qengine.hpp:414-473), and the FP64
+// probability / exact-replay vectors of the sampler.
+uint64_t footprint_of(const qs_circuit* c, qs_dtype dt) {
+    double D = 1;
+    for (uint32_t i = 0; i < c->nq; ++i) D *= c->d;
+    const double s = dt == QS_C128 ? 16 : 8;
+    return static_cast<uint64_t>(2 * s * D + 16 * D + (1 << 20));
+}
+
+}  // namespace
+
+struct qs_server {
+    uint64_t budget = 0, in_use = 0;
+    uint32_t lanes = 0;
+    std::vector<hipStream_t> streams;
+    std::vector<std::thread> workers;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> all;
+    uint64_t next_id = 1;
+    bool stop = false;
+    Clock::time_point t0 = Clock::now();
+    uint64_t completed = 0, peak_in_use = 0, peak_running = 0, running = 0;
+
+    void work(uint32_t lane) {
+        qs::set_thread_stream(streams[lane]);
+        for (;;) {
+            Request* r = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                // wake for a request that fits, or to leave once stopped
+                // and drained (a queued request always fits once nothing
+                // runs: submit rejects footprints above the budget)
+                cv_work.wait(lk, [&] {
+                    if (stop && queue.empty()) return true;
+                    for (Request* q : queue)
+                        if (in_use + q->footprint <= budget) return true;
+                    return false;
+                });
+                if (queue.empty()) return;
+                auto it = std::find_if(queue.begin(), queue.end(),
+                                       [&](Request* q) { return in_use + q->footprint <= budget; });
+                if (it == queue.end()) continue;
+                r = *it;
+                queue.erase(it);
+                in_use += r->footprint;
+                peak_in_use = std::max(peak_in_use, in_use);
+                ++running;
+                peak_running = std::max(peak_running, running);
+            }
+            r->lane = lane;
+            r->t_start = Clock::now();
+            run(r);
+            r->t_done = Clock::now();
+            {
+                std::lock_guard<std::mutex> lk(m);
+                in_use -= r->footprint;
+                --running;
+                r->done = true;
+                ++completed;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+
+    void run(Request* r) {
+        qs_engine* e = nullptr;
+        int rc = qs_engine_create(&r->circ.c, QS_KET, r->dtype, -1.0, r->seed, &e);
+        if (!rc) rc = qs_engine_execute(e, r->shots);
+        if (!rc) {
+            uint64_t n = 0;
+            rc = qs_engine_num_outcomes(e, &n);
+            const uint32_t nc = r->circ.c.nc;
+            if (!rc) {
+                r->stat_dits.assign(n * nc + 1, 0);
+                r->stat_counts.assign(n + 1, 0);
+                rc = qs_engine_get_stats(e, r->stat_dits.data(), r->stat_counts.data());
+                r->stat_dits.resize(n * nc);
+                r->stat_counts.resize(n);
+            }
+            if (!rc) {
+                r->dits.assign(nc + 1, 0);
+                rc = qs_engine_get_dits(e, r->dits.data());
+                r->dits.resize(nc);
+            }
+        }
+        if (rc) r->error = qs_last_error();
+        if (e) qs_engine_destroy(e);
+        // the request's memory is back in the pool in stream order; wait so
+        // the budget the scheduler sees is the memory actually free
+        (void)hipStreamSynchronize(qs::current_stream());
+        r->status = rc;
+    }
+};
+
+extern "C" {
+
+int qs_server_create(uint32_t lanes, uint64_t hbm_budget, qs_server** out) {
+    constexpr const char* W = "qpp_amd::server";
+    if (!out) return fail(QS_ERR_INVALID_HANDLE, W);
+    *out = nullptr;
+    if (lanes == 0 || lanes > 64) return fail(QS_ERR_OUT_OF_RANGE, W, "lanes");
+    if (int rc = qs::ensure_device()) return rc;
+    auto s = std::make_unique<qs_server>();
+    if (hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        QS_HIP(hipMemGetInfo(&fr, &tot), W);
+        hbm_budget = static_cast<uint64_t>(0.9 * double(fr));
+    }
+    s->budget = hbm_budget;
+    s->lanes = lanes;
+    s->streams.resize(lanes);
+    for (uint32_t i = 0; i < lanes; ++i)
+        QS_HIP(hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking), W);
+    qs_server* sp = s.release();
+    for (uint32_t i = 0; i < lanes; ++i) sp->workers.emplace_back([sp, i] { sp->work(i); });
+    *out = sp;
+    return QS_OK;
+}
+
+// Queues QEngine(circuit, seed).execute(shots); the circuit is copied.
+int qs_server_submit(qs_server* s, const qs_circuit* qc, qs_dtype dtype, uint64_t shots,
+                     uint64_t seed, uint64_t* ticket) {
+    constexpr const char* W = "qpp_amd::server";
+    if (!s || !qc || !ticket) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (dtype != QS_C128 && dtype != QS_C64) return fail(QS_ERR_TYPE_MISMATCH, W, "dtype");
+    if (shots == 0) return fail(QS_ERR_OUT_OF_RANGE, W, "shots");
+    auto r = std::make_unique<Request>();
+    r->circ.take(qc);
+    r->dtype = dtype;
+    r->shots = shots;
+    r->seed = seed;
+    r->footprint = footprint_of(qc, dtype);
+    if (r->footprint > s->budget) return fail(QS_ERR_OUT_OF_MEMORY, W, "request exceeds the HBM budget");
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        r->id = s->next_id++;
+        *ticket = r->id;
+        s->queue.push_back(r.get());
+        s->all.emplace(r->id, std::move(r));
+    }
+    s->cv_work.notify_one();
+    return QS_OK;
+}
+
+// Blocks until the request is done.  times_ms (optional, 3): queued time,
+// run time, submit-to-done latency.  Then qs_server_stats reads the results.
+int qs_server_wait(qs_server* s, uint64_t ticket, double* times_ms) {
+    constexpr const char* W = "qpp_amd::server";
+    if (!s) return fail(QS_ERR_INVALID_HANDLE, W);
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return fail(QS_ERR_OUT_OF_RANGE, W, "ticket");
+    Request* r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (times_ms) {
+        using ms = std::chrono::duration<double, std::milli>;
+        times_ms[0] = ms(r->t_start - r->t_submit).count();
+        times_ms[1] = ms(r->t_done - r->t_start).count();
+        times_ms[2] = ms(r->t_done - r->t_submit).count();
+    }
+    if (r->status) return fail(r->status, W, r->error.c_str());
+    return QS_OK;
+}
+
+// Statistics of a finished request (QEngine::get_stats order) and its
+// classical dits; call with dits/counts NULL to get n_outcomes first.
+int qs_server_stats(qs_server* s, uint64_t ticket, uint64_t* n_outcomes, uint64_t* dits,
+                    uint64_t* counts, uint64_t* last_dits) {
+    constexpr const char* W = "qpp_amd::server";
+    if (!s) return fail(QS_ERR_INVALID_HANDLE, W);
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end() || !it->second->done) return fail(QS_ERR_OUT_OF_RANGE, W, "ticket");
+    const Request* r = it->second.get();
+    if (n_outcomes) *n_outcomes = r->stat_counts.size();
+    if (dits) std::copy(r->stat_dits.begin(), r->stat_dits.end(), dits);
+    if (counts) std::copy(r->stat_counts.begin(), r->stat_counts.end(), counts);
+    if (last_dits) std::copy(r->dits.begin(), r->dits.end(), last_dits);
+    return QS_OK;
+}
+
+// Forgets a finished request (its results).
+int qs_server_release(qs_server* s, uint64_t ticket) {
+    if (!s) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::server");
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->all.find(ticket);
+    if (it != s->all.end() && it->second->done) s->all.erase(it);
+    return QS_OK;
+}
+
+// info[6]: completed, queued, running, peak running, bytes in use, peak bytes
+int qs_server_info(qs_server* s, uint64_t* info) {
+    if (!s || !info) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::server");
+    std::lock_guard<std::mutex> lk(s->m);
+    info[0] = s->completed;
+    info[1] = s->queue.size();
+    info[2] = s->running;
+    info[3] = s->peak_running;
+    info[4] = s->in_use;
+    info[5] = s->peak_in_use;
+    return QS_OK;
+}
+
+// Finishes queued work, stops the workers and frees the server.
+int qs_server_destroy(qs_server* s) {
+    if (!s) return QS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers) t.join();
+    for (hipStream_t st : s->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    delete s;
+    return QS_OK;
+}
+
+}  // extern "C"

@@ -71,3 +71,35 @@ def apply_gates(st, gates):
             st.apply_ctrl(U, ctrl, target)
         else:
             st.apply(U, target)
+
+
+def ladder_layers(n, depth, seed):
+    """Trajectory circuit: `depth` layers of a Haar 1-qubit gate on every
+    qubit followed by a CNOT ladder (i -> i+1, i = 0..n-2)."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(depth):
+        for q in range(n):
+            gates.append(("u", [q], [], haar(2, rng)))
+        for i in range(n - 1):
+            gates.append(("cx", [i + 1], [i], X))
+    return gates
+
+
+def trajectory_circuit(qa, n, depth, seed):
+    """A qpp_amd.QCircuit of ladder_layers ending in one destructive
+    measurement of every qubit into dits 0..n-1 (the noisy-trajectory
+    workload)."""
+    qc = qa.QCircuit.from_gates(n, ladder_layers(n, depth, seed))
+    qc.nc = n
+    qc.measure(list(range(n)), 0)
+    return qc
+
+
+def measured_circuit(qa, n, depth, seed):
+    """random_layers ending in one measurement of every qubit (the serving
+    workload: an odd n leaves one qubit unpaired per layer)."""
+    qc = qa.QCircuit.from_gates(n, random_layers(n, depth, seed))
+    qc.nc = n
+    qc.measure(list(range(n)), 0)
+    return qc

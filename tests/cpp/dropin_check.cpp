@@ -158,7 +158,11 @@ int main() {
                         reinterpret_cast<double*>(post.data()), &plen);
         post.resize(plen);
         auto [mr, mp, mst] = measure_seq(psi, {2, 0, 6}, dims, true, u);
-        CHECK(mr == std::vector<idx>(r3.begin(), r3.end()) && mp == p3 &&
+        // outcomes bit-exact; probabilities and the post-measurement state
+        // come from reductions and agree to 1e-12
+        double pd = 0;
+        for (size_t i = 0; i < p3.size(); ++i) pd = std::max(pd, std::abs(mp[i] - p3[i]));
+        CHECK(mr == std::vector<idx>(r3.begin(), r3.end()) && pd <= 1e-12 &&
                   maxdiff(mst, post) <= 1e-12,
               "measure_seq");
     }

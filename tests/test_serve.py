@@ -1,0 +1,63 @@
+"""Request server (qs_server_*): every request's statistics equal those of
+the same QEngine run alone (and the oracle's, for small kets), whatever the
+mix it ran in."""
+import numpy as np
+import pytest
+
+import qpp_amd as qa
+from qpp_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def mix(seed, count, nmin=10, nmax=18, dmax=12):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        depth = int(rng.integers(2, dmax + 1))
+        dt = qa.C64 if rng.random() < 0.5 else qa.C128
+        qc = workloads.measured_circuit(qa, n, depth, seed * 1000 + i)
+        reqs.append((qc, dt, int(rng.integers(1, 2 ** 31))))
+    return reqs
+
+
+def test_results_alone_equal_results_in_the_mix(orc):
+    reqs = mix(7, 14)
+    srv = qa.Server(lanes=4)
+    tickets = [srv.submit(qc, dt, shots=256, seed=sd) for qc, dt, sd in reqs]
+    mixed = []
+    for t in tickets:
+        srv.wait(t)
+        mixed.append(srv.stats(t))
+    info = srv.info()
+    srv.close()
+    assert info["completed"] == len(reqs)
+    for (qc, dt, sd), (st, last) in zip(reqs, mixed):
+        e = qa.QEngine(qc, dtype=dt, seed=sd).execute(256)
+        assert e.get_stats() == st and e.get_dits() == last
+        assert sum(st.values()) == 256
+        if dt == qa.C128 and qc.nq <= 12:
+            ost, odits, _, _ = orc.engine_run(qc, reps=256, seed=sd)
+            assert ost == st and odits == last
+
+
+def test_admission_respects_the_budget():
+    """A budget that holds one 18-qubit c128 request at a time still runs
+    every request; the peak held never exceeds the budget."""
+    reqs = [(workloads.measured_circuit(qa, 18, 3, 50 + i), qa.C128, 11 + i) for i in range(4)]
+    budget = 2 * 16 * 2 ** 18 + 16 * 2 ** 18 + (1 << 20) + 4096
+    srv = qa.Server(lanes=4, hbm_budget=budget)
+    ts = [srv.submit(qc, dt, shots=64, seed=sd) for qc, dt, sd in reqs]
+    res = [srv.wait(t) for t in ts]
+    info = srv.info()
+    srv.close()
+    assert info["peak_running"] == 1 and info["peak_bytes"] <= budget
+    assert all(r[2] >= r[1] for r in res)
+
+
+def test_request_above_budget_is_refused():
+    srv = qa.Server(lanes=1, hbm_budget=1 << 20)
+    with pytest.raises(qa.OutOfMemory):
+        srv.submit(workloads.measured_circuit(qa, 16, 1, 3), qa.C128)
+    srv.close()
