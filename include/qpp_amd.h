@@ -280,6 +280,27 @@ int qs_program_create(const qs_circuit* qc, uint32_t step_begin,
                       uint32_t step_end, qs_kind kind, qs_dtype dtype,
                       const qs_fuse_opts* opts /* NULL: defaults */,
                       qs_program** out);
+/* A program of bit-level operations on a 2^nb-element state (a ket over nb
+ * qubits, or vec(rho) over 2n bits: rho(i, j) at i + j*2^n, so row qubit q
+ * is bit n-1-q and column qubit q bit 2n-1-q).  Each operation is a dense
+ * 2^nbits x 2^nbits column-major matrix at mats + 2*mat_off (complex128,
+ * interleaved) on index bits `bits` (bits[0] = most significant digit of
+ * the matrix index), applied where (index & cmask) == cval.  A unitary on
+ * rho is U on the row bit and conj(U) on the column bit; a 1-qubit channel
+ * {K} on rho is the superoperator sum_K conj(K) (x) K on (column, row) bits
+ * (the form qs_apply_kraus uses; operations.hpp:476-560).  No single qpp
+ * counterpart: it is what a fused sequence of apply / apply(rho, Ks) calls
+ * computes. */
+typedef struct qs_bitgate {
+    uint32_t nbits;   /* 1 or 2 */
+    uint32_t bits[2];
+    uint32_t pad;
+    uint64_t cmask, cval;
+    uint64_t mat_off; /* complex elements */
+} qs_bitgate;
+int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates,
+                           const double* mats, qs_dtype dtype,
+                           const qs_fuse_opts* opts, qs_program** out);
 int qs_program_run(qs_program* p, qs_state* st);
 int qs_program_info(const qs_program* p, uint64_t* num_gates,
                     uint64_t* num_passes, uint64_t* num_phases);

@@ -705,6 +705,101 @@ class Program:
             pass
 
 
+class _BitGate(ctypes.Structure):
+    _fields_ = [("nbits", c_uint32), ("bits", c_uint32 * 2), ("pad", c_uint32),
+                ("cmask", c_uint64), ("cval", c_uint64), ("mat_off", c_uint64)]
+
+
+class BitProgram:
+    """A fused program of bit-level operations (qs_program_create_bits).
+    On a density matrix of n qubits (vec(rho): row qubit q = bit n-1-q,
+    column qubit q = bit 2n-1-q) `rho_apply`, `rho_ctrl` and `rho_channel`
+    add what qpp's apply(rho, U, ...), applyCTRL(rho, ...) and
+    apply(rho, Ks, {q}) compute; all of them run in the same tile passes."""
+
+    def __init__(self, nb):
+        self.nb = int(nb)
+        self.ops = []  # (bits, matrix, cmask, cval)
+        self._h = None
+
+    def gate(self, U, bits, cmask=0, cval=0):
+        self.ops.append((list(bits), np.asarray(U, dtype=np.complex128), int(cmask), int(cval)))
+        return self
+
+    def rho_apply(self, U, target, n):
+        """U rho U^dagger on qubits `target` (1 or 2)."""
+        U = np.asarray(U, dtype=np.complex128)
+        self.gate(U, [n - 1 - t for t in target])
+        return self.gate(U.conj(), [2 * n - 1 - t for t in target])
+
+    def rho_ctrl(self, U, ctrl, target, n, shift=None):
+        """applyCTRL on rho (qubits): U on target where every control is |1>
+        (|0> where shift is 1), on rows and, conjugated, on columns."""
+        shift = shift or [0] * len(ctrl)
+        for col in (False, True):
+            off = 2 * n - 1 if col else n - 1
+            cm = cv = 0
+            for c, s in zip(ctrl, shift):
+                b = 1 << (off - c)
+                cm |= b
+                if not s:
+                    cv |= b
+            M = np.asarray(U, dtype=np.complex128)
+            self.gate(M.conj() if col else M, [off - t for t in target], cm, cv)
+        return self
+
+    def rho_channel(self, Ks, q, n):
+        """sum_K K rho K^dagger on qubit q: the 4x4 superoperator
+        sum_K conj(K) (x) K on (column bit, row bit)."""
+        S = sum(np.kron(np.asarray(K).conj(), np.asarray(K)) for K in Ks)
+        return self.gate(S, [2 * n - 1 - q, n - 1 - q])
+
+    def compile(self, dtype=C128, reg_bits=4, low_bits=5, max_ops=48, grid_cap=0):
+        L = lib()
+        L.qs_program_create_bits.argtypes = [c_uint32, c_uint32, POINTER(_BitGate),
+                                             POINTER(c_double), c_int, POINTER(_FuseOpts),
+                                             POINTER(c_void_p)]
+        L.qs_program_run.argtypes = [c_void_p, c_void_p]
+        L.qs_program_info.argtypes = [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]
+        L.qs_program_destroy.argtypes = [c_void_p]
+        arr = (_BitGate * max(1, len(self.ops)))()
+        mats = []
+        off = 0
+        for i, (bits, M, cm, cv) in enumerate(self.ops):
+            g = arr[i]
+            g.nbits = len(bits)
+            for k, b in enumerate(bits):
+                g.bits[k] = b
+            g.cmask, g.cval, g.mat_off = cm, cv, off
+            flat = np.asfortranarray(M).reshape(-1, order="F")
+            mats.append(flat)
+            off += len(flat)
+        m = np.ascontiguousarray(np.concatenate(mats) if mats else np.zeros(1, dtype=complex))
+        o = _FuseOpts(reg_bits, low_bits, max_ops, grid_cap)
+        if self._h:
+            L.qs_program_destroy(self._h)
+        self._h = c_void_p()
+        _check(L.qs_program_create_bits(self.nb, len(self.ops), arr,
+                                        m.ctypes.data_as(POINTER(c_double)), dtype,
+                                        ctypes.byref(o), ctypes.byref(self._h)))
+        return self
+
+    def run(self, st):
+        _check(lib().qs_program_run(self._h, st.handle))
+
+    def info(self):
+        g, p, h = c_uint64(), c_uint64(), c_uint64()
+        _check(lib().qs_program_info(self._h, byref(g), byref(p), byref(h)))
+        return {"gates": g.value, "passes": p.value, "phases": h.value}
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().qs_program_destroy(self._h)
+        except Exception:
+            pass
+
+
 class QEngine:
     """qpp::QEngineT / QNoisyEngineT (classes/qengine.hpp,
     classes/qnoisy_engine.hpp) over a device-resident state.  noise_p >= 0

@@ -117,6 +117,58 @@ int qs_program_create(const qs_circuit* qc, uint32_t step_begin, uint32_t step_e
     return QS_OK;
 }
 
+int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates,
+                           const double* mats, qs_dtype dtype, const qs_fuse_opts* opts,
+                           qs_program** out) {
+    constexpr const char* W = "qpp_amd::program";
+    if (!out || (ngates && (!gates || !mats))) return fail(QS_ERR_INVALID_HANDLE, W);
+    *out = nullptr;
+    if (nb == 0 || nb > 48) return fail(QS_ERR_OUT_OF_RANGE, W, "nb");
+    std::vector<BitGate> bg(ngates);
+    for (uint32_t i = 0; i < ngates; ++i) {
+        const qs_bitgate& g = gates[i];
+        if (g.nbits < 1 || g.nbits > 2) return fail(QS_ERR_NOT_IMPLEMENTED, W, "gate width");
+        uint64_t used = 0;
+        for (uint32_t k = 0; k < g.nbits; ++k) {
+            if (g.bits[k] >= nb || ((used >> g.bits[k]) & 1))
+                return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "bits");
+            used |= uint64_t{1} << g.bits[k];
+        }
+        if ((g.cmask & used) || (nb < 64 && (g.cmask >> nb)) || (g.cval & ~g.cmask))
+            return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "control");
+        bg[i].bits.assign(g.bits, g.bits + g.nbits);
+        const uint64_t d = uint64_t{1} << g.nbits;
+        bg[i].m.resize(d * d);
+        for (uint64_t e = 0; e < d * d; ++e)
+            bg[i].m[e] = C(mats[2 * (g.mat_off + e)], mats[2 * (g.mat_off + e) + 1]);
+        bg[i].cmask = g.cmask;
+        bg[i].cval = g.cval;
+    }
+    if (int rc = qs::ensure_device()) return rc;
+    FusedOptions o;
+    uint32_t cap = 2048;
+    if (opts) {
+        if (opts->reg_bits) o.reg_bits = opts->reg_bits;
+        if (opts->low_bits) o.low_bits = opts->low_bits;
+        if (opts->max_ops) o.max_ops = opts->max_ops;
+        if (opts->grid_cap) cap = opts->grid_cap;
+    }
+    if (o.reg_bits != 3 && o.reg_bits != 4) return fail(QS_ERR_OUT_OF_RANGE, W, "reg_bits");
+    auto* p = new qs_program;
+    p->ngates = bg.size();
+    p->grid_cap = cap;
+    if (int rc = qs::fused_compile(bg, nb, o, p->P)) {
+        delete p;
+        return rc;
+    }
+    if (int rc = qs::fused_upload(p->P, dtype)) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return QS_OK;
+}
+
 int qs_program_run(qs_program* p, qs_state* st) {
     if (!p || !st || !st->d) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
     return qs::fused_run(p->P, st, p->grid_cap);

@@ -103,3 +103,58 @@ def measured_circuit(qa, n, depth, seed):
     qc.nc = n
     qc.measure(list(range(n)), 0)
     return qc
+
+
+def depolarizing_kraus(p):
+    """Kraus operators of QubitDepolarizingNoise(p) (noise.hpp:388-408):
+    sqrt(1-p) I, sqrt(p/3) X, sqrt(p/3) Y, sqrt(p/3) Z."""
+    Y = np.array([[0, -1j], [1j, 0]])
+    return [np.sqrt(1 - p) * np.eye(2, dtype=complex), np.sqrt(p / 3) * X,
+            np.sqrt(p / 3) * Y, np.sqrt(p / 3) * Z]
+
+
+def dm_pipeline(n, depth, seed, p=1e-3):
+    """Density-matrix workload: rho = 0.9 |psi><psi| + 0.1 I / 2^n (psi a
+    seeded random ket); `depth` layers of a Haar 1-qubit gate on every qubit
+    and CZ on the pairs (2i, 2i+1), each gate followed by the depolarizing
+    channel on every qubit it touched; then ptrace over the upper half of
+    the qubits, ptranspose on qubits {0, 1, 2} and syspermute by a seeded
+    permutation.  Returns (rho, ops, trace_out, pt_target, perm) with ops
+    ("u", [q], U), ("cz", [c, t], Z), ("kraus", [q], Ks)."""
+    rng = np.random.default_rng(seed)
+    D = 2 ** n
+    psi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    psi /= np.linalg.norm(psi)
+    rho = 0.9 * np.outer(psi, psi.conj()) + 0.1 * np.eye(D) / D
+    Ks = depolarizing_kraus(p)
+    ops = []
+    for _ in range(depth):
+        for q in range(n):
+            ops.append(("u", [q], haar(2, rng)))
+            ops.append(("kraus", [q], Ks))
+        for i in range(0, n - 1, 2):
+            ops.append(("cz", [i, i + 1], Z))
+            ops.append(("kraus", [i], Ks))
+            ops.append(("kraus", [i + 1], Ks))
+    trace_out = list(range(n // 2, n))
+    perm = [int(x) for x in rng.permutation(n)]
+    return rho, ops, trace_out, [0, 1, 2][: min(3, n)], perm
+
+
+def dm_program(qa, n, ops):
+    """The gates and channels of dm_pipeline as one fused BitProgram."""
+    bp = qa.BitProgram(2 * n)
+    for kind, q, M in ops:
+        if kind == "u":
+            bp.rho_apply(M, q, n)
+        elif kind == "cz":
+            bp.rho_ctrl(M, [q[0]], [q[1]], n)
+        else:
+            bp.rho_channel(M, q[0], n)
+    return bp
+
+
+def dm_bytes(n, ops, elem_bytes=16):
+    """Algorithmic bytes of the ops applied one by one: every gate and every
+    channel reads and writes rho once (2 s 4^n)."""
+    return len(ops) * 2 * elem_bytes * 4 ** n
