@@ -406,6 +406,9 @@ def main():
     ap.add_argument("--reg-bits", type=int, default=4)
     ap.add_argument("--low-bits", type=int, default=5)
     ap.add_argument("--max-ops", type=int, default=64)
+    ap.add_argument("--grid-cap", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="fused mode: pick tile options with the autotuner first (1/0)")
     ap.add_argument("--sweep", action="store_true",
                     help="time fused options on the first layers and print them")
     args = ap.parse_args()
@@ -450,9 +453,19 @@ def main():
                             f"passes={npas} phases={p.info()['phases']} {ms:.1f} ms "
                             f"({ms / npas:.2f} ms/pass, {gbs:.0f} GB/s per pass, "
                             f"{len(sub) / ms * 1e3:.0f} gates/s)")
+    tuned = None
+    if mode == "fused" and args.autotune:
+        from qpp_amd import autotune
+        cands = [dict(reg_bits=4, low_bits=lb, max_ops=mo, grid_cap=gc)
+                 for lb in (5, 6) for mo in (16, 24, 32, 48, 64) for gc in (2048, 4096)]
+        tt = time.perf_counter()
+        tuned = autotune.tune(n, gates, candidates=cands, time_budget_s=90)
+        log(f"autotune: {tuned} in {time.perf_counter() - tt:.1f} s")
+        args.reg_bits, args.low_bits = tuned["reg_bits"], tuned["low_bits"]
+        args.max_ops, args.grid_cap = tuned["max_ops"], tuned["grid_cap"]
     if mode == "fused":
         prog = qa.Program(n, gates, reg_bits=args.reg_bits, low_bits=args.low_bits,
-                          max_ops=args.max_ops)
+                          max_ops=args.max_ops, grid_cap=args.grid_cap)
         run = lambda: prog.run(st)  # noqa: E731
         npasses = prog.num_passes()
         pass_bytes = npasses * 2 * 16 * 2 ** n
@@ -511,6 +524,9 @@ def main():
                         f"({n1q} Haar 1q + {ncx} CNOT), {'tile-fused passes' if mode == 'fused' else 'gate by gate'}",
             "qubits": n, "depth": depth, "gates": len(gates), "mode": mode,
             "passes_per_step": npasses, "seed": args.seed,
+            "fuse_opts": {"reg_bits": args.reg_bits, "low_bits": args.low_bits,
+                          "max_ops": args.max_ops, "grid_cap": args.grid_cap,
+                          "autotuned": tuned is not None},
         },
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "gate_apps_per_s": round(len(gates) / sec, 2),

@@ -122,25 +122,22 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         for (int p = c.pass_begin; p < c.pass_end; ++p) {
             const FPass* ps = passes + p;
             const bool zero_in = p == c.pass_begin;  // first pass: |0..0>
-            const int pb = ps->phase_begin, pe = ps->phase_end, nrest = ps->nrest;
+            constexpr int RB = RBof<NR>::v;
+            const int pb = ps->phase_begin, pe = ps->phase_end;
             const uint64_t ntiles = ps->ntiles;
             const FPhase* ld = phases + pb;
             const uint64_t gt_ld = thr_g(ld, t);
-            auto tile_base = [&](uint64_t tile) {
-                uint64_t base = 0;
-                for (int i = 0; i < nrest; ++i) base |= ((tile >> i) & 1) << ps->rest[i];
-                return base;
-            };
+            auto tile_base = [&](uint64_t tile) { return tile_base_of(ps, tile); };
             auto fetch = [&](uint64_t tile, cxt<R>* y) {
                 const uint64_t off = tile_base(tile) | gt_ld;
                 if (zero_in) {
 #pragma unroll
                     for (int rho = 0; rho < NR; ++rho)
-                        y[rho] = mk<R>((off | ld->g_reg[rho]) == 0 ? R(1) : R(0), R(0));
+                        y[rho] = mk<R>((off | greg<RB>(ld, rho)) == 0 ? R(1) : R(0), R(0));
                 } else {
                     const cxt<R>* q = a + off;
 #pragma unroll
-                    for (int rho = 0; rho < NR; ++rho) y[rho] = q[ld->g_reg[rho]];
+                    for (int rho = 0; rho < NR; ++rho) y[rho] = q[greg<RB>(ld, rho)];
                 }
             };
             cxt<R> x[NR], y[NR];
@@ -153,19 +150,20 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                 if (tile + 1 < ntiles) fetch(tile + 1, y);
                 oc = ov;
                 uint32_t next_op = oc < ti.ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
-                const FPhase* cur = ld;
+                int ci = pb;
                 for (int pi = pb; pi < pe; ++pi) {
                     const FPhase* ph = phases + pi;
-                    if (ph != cur) {
+                    if (pi != ci) {
+                        const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
                         const uint32_t sw = thr_s(cur, t);
 #pragma unroll
-                        for (int rho = 0; rho < NR; ++rho) lds[sw ^ cur->s_reg[rho]] = x[rho];
+                        for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
                         __syncthreads();
                         const uint32_t sr = thr_s(ph, t);
 #pragma unroll
-                        for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ ph->s_reg[rho]];
+                        for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ph, rho)];
                         __syncthreads();
-                        cur = ph;
+                        ci = pi;
                     }
                     const int ob = ph->op_begin, oe = ph->op_end;
                     for (int o = ob; o < oe; ++o) {
@@ -178,20 +176,21 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                         run_op<R, NR>(x, ops + o, mu, mats, t, base);
                     }
                 }
+                const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
                 if (!cur->coalesced) {
                     const uint32_t sw = thr_s(cur, t);
 #pragma unroll
-                    for (int rho = 0; rho < NR; ++rho) lds[sw ^ cur->s_reg[rho]] = x[rho];
+                    for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
                     __syncthreads();
                     const uint32_t sr = thr_s(ld, t);
 #pragma unroll
-                    for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ ld->s_reg[rho]];
+                    for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ld, rho)];
                     __syncthreads();
                     cur = ld;
                 }
                 cxt<R>* q = a + (base | (cur == ld ? gt_ld : thr_g(cur, t)));
 #pragma unroll
-                for (int rho = 0; rho < NR; ++rho) q[cur->g_reg[rho]] = x[rho];
+                for (int rho = 0; rho < NR; ++rho) q[greg<RB>(cur, rho)] = x[rho];
             }
             ov = oc;
             __syncthreads();  // the pass's stores are visible to the next pass

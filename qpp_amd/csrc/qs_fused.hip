@@ -47,21 +47,18 @@ __global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
     cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
     const FPass* __restrict__ ps = passes + pass_id;
     const uint32_t t = threadIdx.x;
-    const int pb = ps->phase_begin, pe = ps->phase_end, nrest = ps->nrest;
+    constexpr int RB = RBof<NR>::v;
+    const int pb = ps->phase_begin, pe = ps->phase_end;
     const uint64_t ntiles = ps->ntiles;
     const FPhase* ld = phases + pb;  // load layout (lanes on tile bits 0..5)
     const uint64_t gt_ld = thr_g(ld, t);
-    auto tile_base = [&](uint64_t tile) {
-        uint64_t base = 0;
-        for (int i = 0; i < nrest; ++i) base |= ((tile >> i) & 1) << ps->rest[i];
-        return base;
-    };
+    auto tile_base = [&](uint64_t tile) { return tile_base_of(ps, tile); };
     cxt<R> x[NR], y[NR];
     uint64_t tile = blockIdx.x;
     if (tile < ntiles) {
         const cxt<R>* p = a + (tile_base(tile) | gt_ld);
 #pragma unroll
-        for (int rho = 0; rho < NR; ++rho) y[rho] = p[ld->g_reg[rho]];
+        for (int rho = 0; rho < NR; ++rho) y[rho] = p[greg<RB>(ld, rho)];
     }
     for (; tile < ntiles; tile += gridDim.x) {
         const uint64_t base = tile_base(tile);
@@ -72,41 +69,46 @@ __global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
         if (next < ntiles) {
             const cxt<R>* p = a + (tile_base(next) | gt_ld);
 #pragma unroll
-            for (int rho = 0; rho < NR; ++rho) y[rho] = p[ld->g_reg[rho]];
+            for (int rho = 0; rho < NR; ++rho) y[rho] = p[greg<RB>(ld, rho)];
         }
-        const FPhase* cur = ld;
+        // the current layout is carried as a phase index; readfirstlane makes
+        // its uniformity explicit so its tables are read with scalar loads
+        int ci = pb;
         for (int pi = pb; pi < pe; ++pi) {
             const FPhase* ph = phases + pi;
-            if (ph != cur) {
+            if (pi != ci) {
                 // remap through LDS: write with the current layout, read with
                 // the new one (index = swizzled thread part ^ register part)
+                const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
                 const uint32_t sw = thr_s(cur, t);
 #pragma unroll
-                for (int rho = 0; rho < NR; ++rho) lds[sw ^ cur->s_reg[rho]] = x[rho];
+                for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
                 __syncthreads();
                 const uint32_t sr = thr_s(ph, t);
 #pragma unroll
-                for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ ph->s_reg[rho]];
+                for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ph, rho)];
                 __syncthreads();
-                cur = ph;
+                ci = pi;
             }
             const int ob = ph->op_begin, oe = ph->op_end;
             for (int o = ob; o < oe; ++o) run_op<R, NR>(x, ops + o, ops[o].m, mats, t, base);
         }
+        ci = __builtin_amdgcn_readfirstlane(ci);
+        const FPhase* cur = phases + ci;
         if (!cur->coalesced) {
             const uint32_t sw = thr_s(cur, t);
 #pragma unroll
-            for (int rho = 0; rho < NR; ++rho) lds[sw ^ cur->s_reg[rho]] = x[rho];
+            for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
             __syncthreads();
             const uint32_t sr = thr_s(ld, t);
 #pragma unroll
-            for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ ld->s_reg[rho]];
+            for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ld, rho)];
             __syncthreads();
             cur = ld;
         }
         cxt<R>* p = a + (base | (cur == ld ? gt_ld : thr_g(cur, t)));
 #pragma unroll
-        for (int rho = 0; rho < NR; ++rho) p[cur->g_reg[rho]] = x[rho];
+        for (int rho = 0; rho < NR; ++rho) p[greg<RB>(cur, rho)] = x[rho];
     }
 }
 
@@ -200,6 +202,11 @@ FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
             }
         ph.g_reg[rho] = g;
         ph.s_reg[rho] = swz(e);
+        for (uint32_t k = 0; k < RB; ++k)
+            if (rho == (1u << k)) {
+                ph.g_rb[k] = g;
+                ph.s_rb[k] = swz(e);
+            }
     }
     uint32_t j = 0;
     for (uint32_t b = 0; b < m; ++b)
@@ -332,6 +339,13 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
             if (!S.count(b)) rest.push_back(b);
         ps.nrest = static_cast<int>(rest.size());
         for (size_t i = 0; i < rest.size(); ++i) ps.rest[i] = static_cast<uint8_t>(rest[i]);
+        ps.nruns = 0;
+        for (size_t i = 0; i < rest.size();) {
+            size_t j = i + 1;
+            while (j < rest.size() && rest[j] == rest[j - 1] + 1) ++j;
+            ps.runs[ps.nruns++] = uint32_t(i) | (rest[i] << 8) | (uint32_t(j - i) << 16);
+            i = j;
+        }
         ps.ntiles = uint64_t{1} << rest.size();
         auto tile_bit = [&](uint32_t g) {
             return static_cast<uint32_t>(std::find(Sv.begin(), Sv.end(), g) - Sv.begin());

@@ -47,6 +47,12 @@ struct FPhase {
     uint32_t s_reg[kMaxNR];       // swizzled LDS index of register group rho
     uint8_t regbits[8];           // tile bits held in registers
     uint8_t tbits[16];            // tile bits indexed by the thread number
+    // per register bit k: its global index bit and swizzled LDS offset; both
+    // offsets are XOR-linear in rho, so g_reg[rho] / s_reg[rho] are the XOR
+    // of these over the set bits of rho (what the kernels use: 4 uniform
+    // values per phase instead of 2 x 16 table loads)
+    uint64_t g_rb[4];
+    uint32_t s_rb[4];
     int op_begin, op_end;
     int coalesced;                // lanes on tile bits 0..5: loads/stores coalesce
     int pad;
@@ -57,6 +63,10 @@ struct FPass {
     int nrest;           // bits outside the tile
     uint8_t S[16];       // tile bit i -> global bit S[i] (ascending)
     uint8_t rest[64];    // global bits outside the tile (ascending)
+    // the same as runs of consecutive bits: tile counter bits [src, src+len)
+    // go to global bits [dst, dst+len); run = src | dst << 8 | len << 16
+    int nruns;
+    uint32_t runs[24];
     uint64_t ntiles;
     int phase_begin, phase_end;
 };

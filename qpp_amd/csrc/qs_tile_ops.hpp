@@ -237,6 +237,41 @@ __device__ __forceinline__ void run_op(cxt<R>* x, const FOp* op, const double* m
     }
 }
 
+// Global index bits of tile number `tile` (the pass's out-of-tile bits as
+// runs of consecutive bits; scalar shifts and masks, no table loads).
+__device__ __forceinline__ uint64_t tile_base_of(const FPass* ps, uint64_t tile) {
+    const int nr = ps->nruns;
+    uint64_t base = 0;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        if (i < nr) {
+            const uint32_t r = ps->runs[i];
+            const uint32_t src = r & 255u, dst = (r >> 8) & 255u, len = r >> 16;
+            base |= ((tile >> src) & ((uint64_t{1} << len) - 1)) << dst;
+        }
+    }
+    return base;
+}
+
+// register part of a phase's global index / swizzled LDS index for register
+// group rho (a compile-time constant in the unrolled loops)
+template <int RB>
+__device__ __forceinline__ uint64_t greg(const FPhase* ph, int rho) {
+    uint64_t g = 0;
+#pragma unroll
+    for (int k = 0; k < RB; ++k)
+        if ((rho >> k) & 1) g |= ph->g_rb[k];
+    return g;
+}
+template <int RB>
+__device__ __forceinline__ uint32_t sreg(const FPhase* ph, int rho) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < RB; ++k)
+        if ((rho >> k) & 1) s ^= ph->s_rb[k];
+    return s;
+}
+
 // thread part of a phase's global index / swizzled LDS index
 __device__ __forceinline__ uint64_t thr_g(const FPhase* ph, uint32_t t) {
     uint64_t g = 0;
