@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -112,6 +113,11 @@ hipStream_t default_stream() { return g_stream; }
 hipStream_t current_stream() { return t_stream ? t_stream : g_stream; }
 
 void set_thread_stream(hipStream_t s) { t_stream = s; }
+
+bool force_exact_sampling() {
+    const char* v = std::getenv("QPP_AMD_FORCE_EXACT");
+    return v && v[0] == '1';
+}
 
 StreamScope::StreamScope(hipStream_t s) : old(t_stream) { t_stream = s; }
 StreamScope::~StreamScope() { t_stream = old; }

@@ -41,6 +41,7 @@
 
 #include "qs_fused.hpp"
 #include "qs_internal.hpp"
+#include "qs_serial.hpp"
 #include "qs_tile_ops.hpp"
 
 namespace {
@@ -101,11 +102,13 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                                                    const Ovr* __restrict__ ovr,
                                                    const uint32_t* __restrict__ order,
                                                    uint32_t ntraj, TrajOut* __restrict__ out,
-                                                   uint32_t* __restrict__ next) {
+                                                   uint32_t* __restrict__ next, int force_exact) {
     extern __shared__ unsigned char smem_raw[];
     cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
     __shared__ double pre[kThreads + 1];
     __shared__ uint32_t s_k;
+    __shared__ uint64_t s_idx;
+    __shared__ int s_certain;
     const uint32_t t = threadIdx.x;
     cxt<R>* __restrict__ a = ws + uint64_t(blockIdx.x) * slot;
     // work queue: trajectories in `order` (largest first), claimed one at a
@@ -230,30 +233,42 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
             // side, block sums + prefix + block scan on ours
             const double bound =
                 2.0 * (2.0 * double(N) + 2.0 * double(blk) + 2.0 * kThreads + 8.0) * kUr * total;
-            bool certain = idx < N && (idx == 0 || target - cprev > bound) &&
-                           (idx == N - 1 || cc - target > bound);
-            uint32_t exact = 0;
-            if (!certain) {
-                // libstdc++ discrete_distribution: sum, normalise, partial
-                // sums, last := 1, lower_bound (random.tcc:2656-2712)
-                double S = 0;
-                for (uint64_t i = 0; i < N; ++i) S += pnorm(a[i ^ xm]);
-                double cp = 0;
-                idx = N - 1;
-                for (uint64_t i = 0; i < N; ++i) {
-                    cp = cp + __ddiv_rn(pnorm(a[i ^ xm]), S);
-                    if (i == N - 1) cp = 1.0;
-                    if (cp >= u) {
-                        idx = i;
-                        break;
-                    }
-                }
-                exact = 1;
-            }
+            const bool certain = !force_exact && idx < N &&
+                                 (idx == 0 || target - cprev > bound) &&
+                                 (idx == N - 1 || cc - target > bound);
+            s_idx = idx;
+            s_certain = certain ? 1 : 0;
+        }
+        __syncthreads();
+        if (!s_certain) {
+            // libstdc++ discrete_distribution: sequential sum, normalise,
+            // sequential partial sums, last := 1, lower_bound
+            // (random.tcc:2656-2712), replayed by the whole workgroup with
+            // the tile area of LDS as staging
+            double* buf = reinterpret_cast<double*>(smem_raw);
+            // the tile area holds 2 x (192 * PER) doubles
+            constexpr uint32_t PER = uint32_t(sizeof(cxt<R>) * kThreads * NR / 16) / qs_serial::kLoaders;
+            const double S = qs_serial::sum<PER>(N, buf, [&](uint64_t i) { return pnorm(a[i ^ xm]); });
+            if (t == 0) pre[0] = S;
+            __syncthreads();
+            const double Sv = pre[0];
+            bool pending = true;
+            uint64_t found = N - 1;
+            qs_serial::scan<PER>(
+                N, buf, [&](uint64_t i) { return __ddiv_rn(pnorm(a[i ^ xm]), Sv); },
+                [&]() { return pending ? ti.u : 2.0; },
+                [&](uint64_t k) {
+                    found = k;
+                    pending = false;
+                });
+            if (t == 0) s_idx = found;
+        }
+        if (t == 0) {
+            const uint64_t idx = s_idx;
             TrajOut o;
             o.index = idx;
-            o.prob = pnorm(a[idx ^ xm]) / total;
-            o.exact = exact;
+            o.prob = pnorm(a[idx ^ xm]) / pre[kThreads];
+            o.exact = s_certain ? 0 : 1;
             o.pad = 0;
             out[tr] = o;
         }
@@ -864,25 +879,26 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
     auto* ordp = static_cast<const uint32_t*>(b->d_order);
     auto* outp = static_cast<TrajOut*>(b->d_out);
     const uint32_t nt = static_cast<uint32_t>(ntraj);
+    const int force = qs::force_exact_sampling() ? 1 : 0;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, b->d_next);
     if (b->dtype == QS_C128) {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<double, 8>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
+                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
         else
             hipLaunchKernelGGL((k_traj<double, 16>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
+                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
     } else {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<float, 8>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
+                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
         else
             hipLaunchKernelGGL((k_traj<float, 16>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next);
+                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, s);

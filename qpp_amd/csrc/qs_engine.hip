@@ -281,13 +281,17 @@ struct qs_engine {
         return a;
     }
 
-    // visit_measurement_step_, qengine.hpp:639-734, with its draw u
-    int measure(const EStep& s, double u) {
+    // visit_measurement_step_, qengine.hpp:639-734, with its draw u (or,
+    // when `given`, the basis index that draw selects, already sampled)
+    int measure(const EStep& s, double u, const uint64_t* given = nullptr) {
         const std::vector<uint64_t> t = rel(s.target);
         std::vector<uint64_t> res(t.size());
         std::vector<double> ps(t.size());
-        if (int rc = qs_measure_seq(st, uint32_t(t.size()), t.data(), s.destructive ? 1 : 0, u,
-                                    res.data(), ps.data()))
+        const int dst = s.destructive ? 1 : 0;
+        if (int rc = given ? qs::measure_seq_given(st, uint32_t(t.size()), t.data(), dst, *given,
+                                                   res.data(), ps.data())
+                           : qs_measure_seq(st, uint32_t(t.size()), t.data(), dst, u, res.data(),
+                                            ps.data()))
             return rc;
         for (size_t i = 0; i < t.size(); ++i) {
             dits[s.c_reg + i] = res[i];
@@ -399,6 +403,28 @@ struct qs_engine {
         for (auto& kv : c_q) sset.insert(kv.second);
         const std::vector<uint64_t> from(sset.begin(), sset.end());
         const uint64_t D = live_dim();
+        if (s.size() == pos + 1) {
+            // A single measurement step: its draw and every later rep's draw
+            // sample the same pre-measurement state (sample() reads the full
+            // basis index, instruments.hpp:783-799), so all of them come from
+            // one sampling call and no copy of the state is needed.
+            if (D >= 2) (void)canonical();  // rep 0's discarded sample
+            std::vector<double> u;
+            for (uint64_t rep = 0; rep < reps; ++rep) u.push_back(D >= 2 ? canonical() : 0.0);
+            std::vector<uint64_t> idx(u.size()), dig(u.size() * from.size());
+            if (int rc = qs_sample(st, uint32_t(from.size()), from.data(), u.size(), u.data(),
+                                   idx.data(), dig.data(), nullptr))
+                return rc;
+            if (int rc = measure(s[pos], 0.0, &idx[0])) return rc;
+            ++stats[dits];
+            for (size_t r = 1; r < u.size(); ++r) {
+                std::vector<uint64_t> res = dits;
+                size_t i = 0;
+                for (auto& kv : c_q) res[kv.first] = dig[r * from.size() + i++];
+                ++stats[res];
+            }
+            return QS_OK;
+        }
         qs_state* pre = nullptr;
         if (int rc = qs_state_clone(st, &pre)) return rc;
         // rep 0: one sample drawn and discarded, then the measurement steps

@@ -59,6 +59,12 @@ hipStream_t default_stream();
 // (a serving lane), else the library stream
 hipStream_t current_stream();
 void set_thread_stream(hipStream_t s);
+// test hook: QPP_AMD_FORCE_EXACT=1 sends every sampled shot through the
+// exact sequential replay instead of the error-bounded parallel search
+bool force_exact_sampling();
+// qpp::measure_seq with the sampled basis index given (qs_measure.hip)
+int measure_seq_given(qs_state* st, uint32_t ntarget, const uint64_t* target,
+                      int destructive, uint64_t index, uint64_t* results, double* probs);
 // creates states on `s` for the scope (outputs derived from a state live on
 // that state's stream)
 struct StreamScope {

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "qs_internal.hpp"
+#include "qs_serial.hpp"
 
 namespace {
 
@@ -193,33 +194,11 @@ __global__ __launch_bounds__(256) void k_search(const double* __restrict__ p,
 // q_k = p_k / s with s the sequential sum; one block
 __global__ __launch_bounds__(256) void k_serial_sum(const double* __restrict__ p,
                                                     uint64_t D, double* __restrict__ s_out) {
-    // thread 0 adds in order; the block stages 4096-element tiles in LDS
-    __shared__ double tile[kChunk];
-    double s = 0;
-    const uint64_t ntiles = (D + kChunk - 1) / kChunk;
-    for (uint64_t t = 0; t < ntiles; ++t) {
-        const uint64_t b = t * kChunk;
-        for (uint64_t i = threadIdx.x; i < kChunk; i += 256)
-            tile[i] = (b + i < D) ? p[b + i] : 0.0;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint64_t n = (D - b < kChunk) ? D - b : kChunk;
-            for (uint64_t i = 0; i < n; ++i) s = __dadd_rn(s, tile[i]);
-        }
-        __syncthreads();
-    }
+    __shared__ double buf[2 * kChunk];
+    const double s = qs_serial::sum<kChunk / qs_serial::kLoaders>(D, buf, [&](uint64_t i) { return p[i]; });
     if (threadIdx.x == 0) *s_out = s;
 }
 
-__global__ __launch_bounds__(256) void k_div(const double* __restrict__ p,
-                                             uint64_t D, const double* __restrict__ s,
-                                             double* __restrict__ q) {
-    const double sv = *s;
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < D;
-         i += stride)
-        q[i] = __ddiv_rn(p[i], sv);
-}
 
 // The reference's cumulative table, replayed in order, for the shots sorted
 // by u (order[] gives the shot of each sorted position).  One block.
@@ -232,35 +211,23 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ p,
         out[s] = p[idx[s]];
 }
 
-__global__ __launch_bounds__(256) void k_serial_exact(const double* __restrict__ q,
+// cp_k = cp_{k-1} + p_k / S in order, cp_{D-1} := 1; every shot in `order`
+// (sorted by u) gets the first k with cp_k >= u.  The quotients are formed
+// by the loader threads; the next threshold stays in a register.
+__global__ __launch_bounds__(256) void k_serial_exact(const double* __restrict__ p,
                                                       uint64_t D,
+                                                      const double* __restrict__ S,
                                                       const double* __restrict__ u,
                                                       const uint32_t* __restrict__ order,
                                                       uint32_t nsorted,
                                                       uint64_t* __restrict__ idx) {
-    __shared__ double tile[kChunk];
-    double cp = 0;
+    __shared__ double buf[2 * kChunk];
+    const double sv = *S;
     uint32_t next = 0;
-    const uint64_t ntiles = (D + kChunk - 1) / kChunk;
-    for (uint64_t t = 0; t < ntiles && next < nsorted; ++t) {
-        const uint64_t b = t * kChunk;
-        for (uint64_t i = threadIdx.x; i < kChunk; i += 256)
-            tile[i] = (b + i < D) ? q[b + i] : 0.0;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint64_t n = (D - b < kChunk) ? D - b : kChunk;
-            for (uint64_t i = 0; i < n && next < nsorted; ++i) {
-                const uint64_t k = b + i;
-                cp = __dadd_rn(cp, tile[i]);
-                const double c = (k == D - 1) ? 1.0 : cp;
-                while (next < nsorted && c >= u[order[next]]) {
-                    idx[order[next]] = k;
-                    ++next;
-                }
-            }
-        }
-        __syncthreads();
-    }
+    qs_serial::scan<kChunk / qs_serial::kLoaders>(
+        D, buf, [&](uint64_t i) { return __ddiv_rn(p[i], sv); },
+        [&]() { return next < nsorted ? u[order[next]] : 2.0; },
+        [&](uint64_t k) { idx[order[next++]] = k; });
 }
 
 // ---------------------------------------------------------------------------
@@ -414,7 +381,7 @@ struct Buf {
         return QS_OK;
     }
 };
-thread_local Buf g_p, g_aux, g_q;
+thread_local Buf g_p, g_aux;
 
 int norm_complex(const qs_state* st, double* re, double* im) {
     const int nb = 1024;
@@ -550,22 +517,19 @@ int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
         return QS_OK;
     };
     std::vector<uint32_t> amb;
+    const bool force = qs::force_exact_sampling();
     for (uint64_t s = 0; s < nshots; ++s)
-        if (!hc[s]) amb.push_back(static_cast<uint32_t>(s));
+        if (force || !hc[s]) amb.push_back(static_cast<uint32_t>(s));
     if (amb.empty()) return probs_out();
     // exact replay for the ambiguous shots, sorted by u
     std::sort(amb.begin(), amb.end(), [&](uint32_t a, uint32_t b) {
         return u_host[a] < u_host[b] || (u_host[a] == u_host[b] && a < b);
     });
-    if (int rc = g_q.need(D * 8)) return rc;
-    auto* q = static_cast<double*>(g_q.p);
     QS_HIP(hipMemcpyAsync(order, amb.data(), amb.size() * 4, hipMemcpyHostToDevice,
                           st->stream),
            "qpp::sample()");
     hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(256), 0, st->stream, p, D, sdev);
-    hipLaunchKernelGGL(k_div, dim3(grid_for(D, 256)), dim3(256), 0, st->stream, p, D,
-                       sdev, q);
-    hipLaunchKernelGGL(k_serial_exact, dim3(1), dim3(256), 0, st->stream, q, D, ud,
+    hipLaunchKernelGGL(k_serial_exact, dim3(1), dim3(256), 0, st->stream, p, D, sdev, ud,
                        order, static_cast<uint32_t>(amb.size()), idx);
     QS_HIP(hipGetLastError(), "qpp::sample()");
     QS_HIP(hipMemcpyAsync(out_idx.data(), idx, nshots * 8, hipMemcpyDeviceToHost,
@@ -704,6 +668,20 @@ int qs_measure_seq(qs_state* st, uint32_t ntarget, const uint64_t* target,
     // one sample of all targets
     std::vector<uint64_t> idx;
     if (int rc = sample_device(st, &u, 1, idx)) return rc;
+    return qs::measure_seq_given(st, ntarget, target, destructive, idx[0], results, probs);
+}
+
+}  // extern "C"
+
+// measure_seq with the sampled basis index already known (the engine draws
+// it together with its other shots from the same state)
+int qs::measure_seq_given(qs_state* st, uint32_t ntarget, const uint64_t* target,
+                          int destructive, uint64_t index, uint64_t* results,
+                          double* probs) {
+    constexpr const char* W = "qpp::measure_seq()";
+    std::vector<uint64_t> tv(target, target + ntarget);
+    if (!check_subsys(tv, st->dims)) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    const std::vector<uint64_t> idx{index};
     const uint64_t n0 = st->dims.size();
     std::vector<uint64_t> mid(n0), res(ntarget);
     {
@@ -768,5 +746,3 @@ int qs_measure_seq(qs_state* st, uint32_t ntarget, const uint64_t* target,
     }
     return QS_OK;
 }
-
-}  // extern "C"

@@ -179,3 +179,26 @@ def test_batch_rerun_is_deterministic_and_order_free():
     for k in (0, 5):
         ia, pa = b.run(cof[k:k + 1], seeds[k:k + 1], 0.02)
         assert ia[0] == i1[k] and pa[0] == p1[k]
+
+
+@gpu
+def test_batch_exact_replay_path_vs_oracle(orc, monkeypatch):
+    """Every trajectory's outcome decided by the in-kernel exact sequential
+    replay (forced) still equals the oracle's."""
+    monkeypatch.setenv("QPP_AMD_FORCE_EXACT", "1")
+    circs = [noisy_circuit(12, 2, seed=61), noisy_circuit(13, 2, seed=62)]
+    cof = np.array([i % 2 for i in range(16)], dtype=np.uint32)
+    seeds = np.arange(500, 516, dtype=np.uint64)
+    b = qa.Batch(circs, dtype=qa.C128)
+    idx, _ = b.run(cof, seeds, 0.01)
+    assert b.info(0)["exact_replays"] == 16
+    for i in range(16):
+        c = int(cof[i])
+        assert b.dits(c, idx[i]) == oracle_outcome(orc, circs[c], int(seeds[i]), 0.01)
+    b64 = qa.Batch(circs, dtype=qa.C64)
+    b64.run(cof, seeds, 0.01)
+    assert b64.info(0)["exact_replays"] == 16
+    # the bounded parallel search decides the same outcomes
+    monkeypatch.delenv("QPP_AMD_FORCE_EXACT")
+    i2, _ = qa.Batch(circs, dtype=qa.C128).run(cof, seeds, 0.01)
+    assert np.array_equal(i2, idx)
