@@ -388,6 +388,38 @@ def bench_rho(args, qa):
     }
 
 
+def bench_copy(args, qa):
+    """Reference streaming kernel: a 30-qubit complex128 state (16 GiB) copied
+    into another with qs_state_copy (each byte read once, written once).
+    Used to calibrate memory-traffic counters and as the practical HBM
+    ceiling."""
+    import ctypes
+    n = args.qubits
+    L = qa.lib()
+    L.qs_state_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    src = qa.State([2] * n)
+    dst = qa.State([2] * n)
+    for _ in range(max(1, args.warmup)):
+        qa._check(L.qs_state_copy(dst.handle, src.handle))
+    reps = max(1, args.steps)
+    ms = profile_region(qa, lambda: [qa._check(L.qs_state_copy(dst.handle, src.handle))
+                                     for _ in range(reps)]) / reps
+    nbytes = 2 * 16 * 2 ** n
+    gbs = nbytes / (ms / 1e3) / 1e9
+    return {
+        "metric": "HBM copy GB/s (reference streaming kernel)",
+        "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": reps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": f"copy of a {n}-qubit complex128 state ({nbytes / 2**30:.0f} GiB moved)",
+                   "bytes_per_launch": nbytes},
+        "roofline": {"bound": "hbm", "kernel": "k_convert<double2,double2>", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(ms, 4)},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,7 +428,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=30)
-    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho"], default="fused")
+    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy"], default="fused")
     ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
     ap.add_argument("--cal-requests", type=int, default=40,
                     help="serve mode: closed-loop calibration requests")
@@ -437,6 +469,9 @@ def main():
         return
     if args.mode == "rho":
         print(json.dumps(bench_rho(args, qa)), flush=True)
+        return
+    if args.mode == "copy":
+        print(json.dumps(bench_copy(args, qa)), flush=True)
         return
     n, depth = args.qubits, args.depth
     gates = workloads.random_layers(n, depth, args.seed)
@@ -550,6 +585,17 @@ def main():
         },
         "cpu_baseline": None,
     }
+    # HBM bytes per launch of this kernel at this size, measured with
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by the copy-kernel
+    # calibration (profiles/traffic.json; the summaries beside it hold the
+    # commands and counter values)
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            tr = json.load(f).get(kernel_name)
+        if tr and tr.get("qubits") == n and mode == "fused":
+            out["roofline"]["traffic"] = tr["bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
     if args.cpu_gates > 0:
         log(f"cpu baseline: {args.cpu_gates} gates at {n} qubits ...")
         out["cpu_baseline"] = cpu_baseline(gates, n, args.cpu_gates)

@@ -191,6 +191,9 @@ int qs_normalize(qs_state* st);
 int qs_compare(const qs_state* a, const qs_state* b, double* out);
 /* A copy of st stored as dtype (complex128 <-> complex64). */
 int qs_state_convert(const qs_state* st, qs_dtype dtype, qs_state** out);
+/* dst := src (same shape and dtype), asynchronous on dst's stream, with a
+ * plain streaming kernel. */
+int qs_state_copy(qs_state* dst, const qs_state* src);
 
 /* ------------------------------------------------------------------------ */
 /* Measurement                                                               */

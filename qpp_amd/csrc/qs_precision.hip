@@ -141,6 +141,27 @@ int qs_compare(const qs_state* a, const qs_state* b, double* out) {
     return QS_OK;
 }
 
+// dst := src for two states of the same shape and dtype, with a plain
+// streaming kernel (every byte read once and written once: the reference
+// point for memory-traffic counters).
+int qs_state_copy(qs_state* dst, const qs_state* src) {
+    constexpr const char* W = "qpp_amd::copy()";
+    if (!valid(dst) || !valid(src)) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (dst->dtype != src->dtype || dst->kind != src->kind || dst->dims != src->dims)
+        return fail(QS_ERR_TYPE_MISMATCH, W, "dst/src");
+    if (src->stream != dst->stream) QS_HIP(hipStreamSynchronize(src->stream), W);
+    const uint64_t n = src->nelem;
+    const unsigned grid = grid_for(n);
+    if (src->dtype == QS_C128)
+        hipLaunchKernelGGL((k_convert<double2, double2>), dim3(grid), dim3(kBlock), 0, dst->stream,
+                           static_cast<const double2*>(src->d), static_cast<double2*>(dst->d), n);
+    else
+        hipLaunchKernelGGL((k_convert<float2, float2>), dim3(grid), dim3(kBlock), 0, dst->stream,
+                           static_cast<const float2*>(src->d), static_cast<float2*>(dst->d), n);
+    QS_HIP(hipGetLastError(), W);
+    return QS_OK;
+}
+
 // A copy of `st` stored as `dtype` (complex128 <-> complex64).
 int qs_state_convert(const qs_state* st, qs_dtype dtype, qs_state** out) {
     constexpr const char* W = "qpp_amd::convert()";
