@@ -707,6 +707,7 @@ int qs_batch_create(const qs_circuit* const* circuits, uint32_t ncircuits, qs_dt
     *out = nullptr;
     if (dtype != QS_C128 && dtype != QS_C64) return fail(QS_ERR_TYPE_MISMATCH, W, "dtype");
     FusedOptions opt;
+    opt.bundle = false;  // per-trajectory overrides address single 1-bit ops
     opt.reg_bits = 4;
     opt.low_bits = 5;
     opt.max_ops = 48;

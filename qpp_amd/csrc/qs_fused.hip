@@ -297,6 +297,59 @@ FOp lower_op(const BitGate& g, const FPhase& ph, uint32_t RB, const std::vector<
     return op;
 }
 
+// Replaces each run of >= 2 consecutive uncontrolled dense 1-bit ops on
+// distinct register bits of a phase by one OP_U1B op (ops on distinct bits
+// commute, so their order inside the run is free).  The gate -> op map
+// no longer points at bundled gates.
+void bundle_phase(FusedProgram& P, FPhase& ph, uint32_t RB, std::vector<C>& mats) {
+    auto plain_u1 = [](const FOp& o) {
+        return o.kind == OP_U1 && !o.ctrl && o.omask == 0;
+    };
+    std::vector<FOp> out;
+    std::vector<int> remap(P.ops.size() - ph.op_begin, -1);
+    int i = ph.op_begin;
+    const int e = ph.op_end;
+    while (i < e) {
+        if (!plain_u1(P.ops[i])) {
+            remap[i - ph.op_begin] = ph.op_begin + int(out.size());
+            out.push_back(P.ops[i++]);
+            continue;
+        }
+        int j = i;
+        uint32_t mask = 0;
+        while (j < e && plain_u1(P.ops[j]) && !((mask >> P.ops[j].j0) & 1u)) {
+            mask |= 1u << P.ops[j].j0;
+            ++j;
+        }
+        if (j - i < 2) {
+            remap[i - ph.op_begin] = ph.op_begin + int(out.size());
+            out.push_back(P.ops[i++]);
+            continue;
+        }
+        FOp b{};
+        b.kind = OP_U1B;
+        b.tbit = mask;
+        b.regok = P.ops[i].regok;
+        b.moff = static_cast<uint32_t>(mats.size());
+        for (uint32_t k = 0; k < 4; ++k) {
+            C m[4] = {C(1, 0), C(0, 0), C(0, 0), C(1, 0)};
+            for (int q = i; q < j; ++q)
+                if (uint32_t(P.ops[q].j0) == k)
+                    for (int t = 0; t < 4; ++t)
+                        m[t] = C(P.ops[q].m[2 * t], P.ops[q].m[2 * t + 1]);
+            mats.insert(mats.end(), m, m + 4);
+        }
+        (void)RB;
+        out.push_back(b);
+        i = j;
+    }
+    P.ops.resize(ph.op_begin);
+    P.ops.insert(P.ops.end(), out.begin(), out.end());
+    ph.op_end = static_cast<int>(P.ops.size());
+    for (int& g : P.gate_op)
+        if (g >= ph.op_begin) g = g < e ? remap[g - ph.op_begin] : -1;
+}
+
 }  // namespace
 
 int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOptions& opt,
@@ -385,6 +438,7 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
                 P.ops.push_back(lower_op(gates[take[pi]], ph, RB, Sv, mats));
             }
             ph.op_end = static_cast<int>(P.ops.size());
+            if (opt.bundle) bundle_phase(P, ph, RB, mats);
             P.phases.push_back(ph);
             std::vector<size_t> rem;
             std::set<size_t> tk(ptake.begin(), ptake.end());

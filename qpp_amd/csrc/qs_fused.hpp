@@ -16,7 +16,10 @@ enum FOpKind : int {
     OP_DIAG_R = 2,  // diag(d0, d1) on register bit j0
     OP_DIAG_T = 3,  // diag(d0, d1) on a thread-number bit (tbit)
     OP_DIAG_O = 4,  // diag(d0, d1) on a bit outside the tile (dbit)
-    OP_U2 = 5       // 4x4 dense on register bits j0 < j1 (j1 = gate MSB)
+    OP_U2 = 5,      // 4x4 dense on register bits j0 < j1 (j1 = gate MSB)
+    OP_U1B = 6      // uncontrolled 2x2 dense on each register bit k set in
+                    // tbit, matrices at mats + moff + 4k (commuting ops
+                    // grouped so the kernel applies them in one straight run)
 };
 
 // One gate of a phase.  Its control (global mask, value) is split by where
@@ -83,6 +86,7 @@ struct FusedOptions {
     uint32_t reg_bits = 4;  // log2(elements per thread): tile = 2^(8 + reg_bits)
     uint32_t low_bits = 6;  // low index bits every tile keeps (coalescing)
     size_t max_ops = 64;    // gates per pass at most (balances VALU vs HBM)
+    bool bundle = true;     // group commuting 1-bit ops of a phase (OP_U1B)
 };
 
 struct FusedProgram {

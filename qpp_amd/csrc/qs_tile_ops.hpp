@@ -94,6 +94,20 @@ __device__ __forceinline__ void do_u1(cxt<R>* x, const FOp* op, const double* mu
     }
 }
 
+// uncontrolled 2x2 on register bit J, matrix (column-major) at M
+template <typename R, int NR, int J>
+__device__ __forceinline__ void do_u1m(cxt<R>* x, const cxt<R>* __restrict__ M) {
+    const cxt<R> u00 = M[0], u10 = M[1], u01 = M[2], u11 = M[3];
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) {
+        if ((rho >> J) & 1) continue;
+        const int r1 = rho | (1 << J);
+        const cxt<R> a = x[rho], b = x[r1];
+        x[rho] = mac2<R>(u00, a, u01, b);
+        x[r1] = mac2<R>(u10, a, u11, b);
+    }
+}
+
 template <typename R, int NR, int J, bool CT>
 __device__ __forceinline__ void do_x(cxt<R>* x, const FOp* op, bool tok) {
     const uint32_t regok = op->regok;
@@ -203,6 +217,16 @@ __device__ __forceinline__ void run_op_ct(cxt<R>* x, const FOp* op, const double
         case OP_DIAG_O: {
             const cxt<R> d = (base & op->dbit) ? opm<R>(op, 1) : opm<R>(op, 0);
             do_scale<R, NR, CT>(x, op, d, tok);
+            break;
+        }
+        case OP_U1B: {
+            const cxt<R>* M = mats + op->moff;
+            const uint32_t mask = op->tbit;
+            if (mask & 1u) do_u1m<R, NR, 0>(x, M);
+            if (mask & 2u) do_u1m<R, NR, 1>(x, M + 4);
+            if (mask & 4u) do_u1m<R, NR, 2>(x, M + 8);
+            if constexpr (NR > 8)
+                if (mask & 8u) do_u1m<R, NR, 3>(x, M + 12);
             break;
         }
         case OP_U2: {
