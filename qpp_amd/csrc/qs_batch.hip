@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                     for (int rho = 0; rho < NR; ++rho) y[rho] = q[greg<RB>(ld, rho)];
                 }
             };
-            cxt<R> x[NR], y[NR];
+            cxt<R> x[NR], y[NR], z[NR];
             fetch(0, y);
             uint32_t oc = ov;
             for (uint64_t tile = 0; tile < ntiles; ++tile) {
@@ -168,15 +168,24 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                         __syncthreads();
                         ci = pi;
                     }
+                    // the matrix of op o: its own, or this trajectory's override
+                    auto mu_of = [&](int o) -> const double* {
+                        if (uint32_t(o) != next_op) return ops[o].m;
+                        const double* mu = ovr[oc].m;
+                        ++oc;
+                        next_op = oc < ti.ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
+                        return mu;
+                    };
+                    // consecutive ops alternate between x and z (see run_op)
                     const int ob = ph->op_begin, oe = ph->op_end;
-                    for (int o = ob; o < oe; ++o) {
-                        const double* mu = ops[o].m;
-                        if (uint32_t(o) == next_op) {
-                            mu = ovr[oc].m;
-                            ++oc;
-                            next_op = oc < ti.ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
-                        }
-                        run_op<R, NR>(x, ops + o, mu, mats, t, base);
+                    int o = ob;
+                    for (; o + 1 < oe; o += 2) {
+                        run_op<R, NR>(x, z, ops + o, mu_of(o), mats, t, base);
+                        run_op<R, NR>(z, x, ops + o + 1, mu_of(o + 1), mats, t, base);
+                    }
+                    if (o < oe) {
+                        run_op<R, NR>(x, z, ops + o, mu_of(o), mats, t, base);
+                        copy_regs<R, NR>(z, x);
                     }
                 }
                 const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
@@ -579,7 +588,8 @@ int upload_tables(qs_batch* b) {
             ph.push_back(h);
         }
         for (FOp o : c.P.ops) {
-            if (o.kind == OP_U2) o.moff += mbase;
+            if (o.kind == OP_U2 || o.kind == OP_U1B) o.moff += mbase;
+            seal_op(o);
             op.push_back(o);
         }
         mats.insert(mats.end(), c.P.mats.begin(), c.P.mats.end());

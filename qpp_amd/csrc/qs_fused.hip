@@ -53,13 +53,18 @@ __global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
     const FPhase* ld = phases + pb;  // load layout (lanes on tile bits 0..5)
     const uint64_t gt_ld = thr_g(ld, t);
     auto tile_base = [&](uint64_t tile) { return tile_base_of(ps, tile); };
-    cxt<R> x[NR], y[NR];
+    cxt<R> x[NR], y[NR], z[NR];
     uint64_t tile = blockIdx.x;
     if (tile < ntiles) {
         const cxt<R>* p = a + (tile_base(tile) | gt_ld);
 #pragma unroll
         for (int rho = 0; rho < NR; ++rho) y[rho] = p[greg<RB>(ld, rho)];
     }
+    // Wait for the first tile here, once.  Without this the loop head would
+    // carry the waits for these loads, and since one vector-memory counter
+    // covers loads and stores, every later iteration would stall there until
+    // the previous tile's stores had completed.
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
     for (; tile < ntiles; tile += gridDim.x) {
         const uint64_t base = tile_base(tile);
 #pragma unroll
@@ -90,8 +95,17 @@ __global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
                 __syncthreads();
                 ci = pi;
             }
+            // consecutive ops alternate between x and z (see run_op)
             const int ob = ph->op_begin, oe = ph->op_end;
-            for (int o = ob; o < oe; ++o) run_op<R, NR>(x, ops + o, ops[o].m, mats, t, base);
+            int o = ob;
+            for (; o + 1 < oe; o += 2) {
+                run_op<R, NR>(x, z, ops + o, ops[o].m, mats, t, base);
+                run_op<R, NR>(z, x, ops + o + 1, ops[o + 1].m, mats, t, base);
+            }
+            if (o < oe) {
+                run_op<R, NR>(x, z, ops + o, ops[o].m, mats, t, base);
+                copy_regs<R, NR>(z, x);
+            }
         }
         ci = __builtin_amdgcn_readfirstlane(ci);
         const FPhase* cur = phases + ci;
@@ -321,7 +335,7 @@ void bundle_phase(FusedProgram& P, FPhase& ph, uint32_t RB, std::vector<C>& mats
             mask |= 1u << P.ops[j].j0;
             ++j;
         }
-        if (j - i < 2) {
+        if (mask != (1u << RB) - 1) {  // bundles cover every register bit
             remap[i - ph.op_begin] = ph.op_begin + int(out.size());
             out.push_back(P.ops[i++]);
             continue;
@@ -339,7 +353,6 @@ void bundle_phase(FusedProgram& P, FPhase& ph, uint32_t RB, std::vector<C>& mats
                         m[t] = C(P.ops[q].m[2 * t], P.ops[q].m[2 * t + 1]);
             mats.insert(mats.end(), m, m + 4);
         }
-        (void)RB;
         out.push_back(b);
         i = j;
     }
@@ -481,6 +494,7 @@ int fused_upload(FusedProgram& P, qs_dtype dtype) {
     const size_t o_h = put(P.phases.data(), hb);
     std::vector<FOp> opsv = P.ops;
     if (opsv.empty()) opsv.resize(1);
+    for (FOp& o : opsv) seal_op(o);
     const size_t o_o = put(opsv.data(), ob);
     size_t o_m;
     if (dtype == QS_C128) {

@@ -27,19 +27,31 @@ enum FOpKind : int {
 //   register bits -> regok: bit rho set when register group rho satisfies it
 //   thread bits   -> (t & tmask) == tval on the thread number t
 //   other bits    -> (tile base & omask) == oval, uniform per tile
+// The kernels dispatch on `code` alone (one scalar load, with the matrix in
+// the same 80 leading bytes); the other fields are read only by the cases
+// that need them.  `kind`, `j0`, `j1`, `ctrl` are the host's description,
+// packed into `code` by seal_op().
 struct FOp {
-    int kind;
-    int j0, j1;
+    uint32_t code;      // kind | j0 << 4 | j1 << 6 | kCodeCtrl | kCodeOut
     uint32_t regok;
     uint32_t tmask, tval;
-    uint32_t tbit;
-    uint32_t moff;      // OP_U2: matrix offset (complex elements)
+    double m[8];        // U1: u00 u10 u01 u11 (re, im); DIAG: d0 d1
     uint64_t omask, oval;
     uint64_t dbit;      // OP_DIAG_O: global bit
+    uint32_t tbit;      // OP_DIAG_T: thread bit; OP_U1B: register bits
+    uint32_t moff;      // OP_U2 / OP_U1B: matrix offset (complex elements)
+    int kind;
+    int j0, j1;
     int ctrl;           // 1 when regok / tmask restrict the op per element
-    int pad;
-    double m[8];        // U1: u00 u10 u01 u11 (re, im); DIAG: d0 d1
 };
+
+constexpr uint32_t kCodeCtrl = 1u << 8;  // regok / tmask apply
+constexpr uint32_t kCodeOut = 1u << 9;   // omask / oval apply
+
+inline void seal_op(FOp& o) {
+    o.code = uint32_t(o.kind) | uint32_t(o.j0) << 4 | uint32_t(o.j1) << 6 |
+             (o.ctrl ? kCodeCtrl : 0u) | (o.omask ? kCodeOut : 0u);
+}
 
 constexpr int kMaxNR = 16;
 

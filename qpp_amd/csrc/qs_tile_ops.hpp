@@ -62,6 +62,10 @@ __device__ __forceinline__ cxt<R> opm(const FOp* op, int i) {
 
 constexpr int kThreads = 256;
 
+// s_waitcnt immediate for vmcnt(0) with expcnt / lgkmcnt left at their
+// maxima (gfx9 encoding: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[15:14])
+constexpr int kWaitVm0 = 0x0F70;
+
 template <int NR>
 struct RBof;
 template <>
@@ -74,81 +78,124 @@ struct RBof<16> {
 };
 
 // ---------------------------------------------------------------------------
-// ops on the register array x[NR]
+// ops on the register array: each op reads x[NR] and writes all of y[NR]
+// (elements it leaves alone are copied).  The kernels alternate two arrays
+// between consecutive ops, so an op's results already sit where the next op
+// reads them; an op that updated one loop-carried array in place would make
+// every case of the op switch move its results back into that array's
+// registers (two full-array copies per op, as many VALU instructions as the
+// gate arithmetic of a dense 1-bit op).
 // ---------------------------------------------------------------------------
 template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_u1(cxt<R>* x, const FOp* op, const double* mu, bool tok) {
+__device__ __forceinline__ void do_u1(const cxt<R>* x, cxt<R>* y, const FOp* op,
+                                      const double* mu, bool tok) {
     const cxt<R> u00 = opm<R>(mu, 0), u10 = opm<R>(mu, 1);
     const cxt<R> u01 = opm<R>(mu, 2), u11 = opm<R>(mu, 3);
     const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if ((rho >> J) & 1) continue;
-        if (CT && !((regok >> rho) & 1u)) continue;
         const int r1 = rho | (1 << J);
         const cxt<R> a = x[rho], b = x[r1];
+        if (CT && !((regok >> rho) & 1u)) {
+            y[rho] = a;
+            y[r1] = b;
+            continue;
+        }
         const cxt<R> y0 = mac2<R>(u00, a, u01, b);
         const cxt<R> y1 = mac2<R>(u10, a, u11, b);
-        x[rho] = CT ? sel<R>(tok, y0, a) : y0;
-        x[r1] = CT ? sel<R>(tok, y1, b) : y1;
+        y[rho] = CT ? sel<R>(tok, y0, a) : y0;
+        y[r1] = CT ? sel<R>(tok, y1, b) : y1;
     }
 }
 
 // uncontrolled 2x2 on register bit J, matrix (column-major) at M
 template <typename R, int NR, int J>
-__device__ __forceinline__ void do_u1m(cxt<R>* x, const cxt<R>* __restrict__ M) {
+__device__ __forceinline__ void do_u1m(const cxt<R>* x, cxt<R>* y, const cxt<R>* __restrict__ M) {
     const cxt<R> u00 = M[0], u10 = M[1], u01 = M[2], u11 = M[3];
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if ((rho >> J) & 1) continue;
         const int r1 = rho | (1 << J);
         const cxt<R> a = x[rho], b = x[r1];
-        x[rho] = mac2<R>(u00, a, u01, b);
-        x[r1] = mac2<R>(u10, a, u11, b);
+        y[rho] = mac2<R>(u00, a, u01, b);
+        y[r1] = mac2<R>(u10, a, u11, b);
     }
 }
 
+// a 2x2 on every register bit k, matrix at M + 4k (one straight run)
+template <typename R, int NR>
+__device__ __forceinline__ void do_u1b(const cxt<R>* x, cxt<R>* y, const cxt<R>* __restrict__ M) {
+    cxt<R> s[NR], u[NR];
+    do_u1m<R, NR, 0>(x, s, M);
+    do_u1m<R, NR, 1>(s, u, M + 4);
+    if constexpr (NR > 8) {
+        do_u1m<R, NR, 2>(u, s, M + 8);
+        do_u1m<R, NR, 3>(s, y, M + 12);
+    } else {
+        do_u1m<R, NR, 2>(u, y, M + 8);
+    }
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void copy_regs(const cxt<R>* x, cxt<R>* y) {
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) y[rho] = x[rho];
+}
+
 template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_x(cxt<R>* x, const FOp* op, bool tok) {
+__device__ __forceinline__ void do_x(const cxt<R>* x, cxt<R>* y, const FOp* op, bool tok) {
     const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if ((rho >> J) & 1) continue;
-        if (CT && !((regok >> rho) & 1u)) continue;
         const int r1 = rho | (1 << J);
         const cxt<R> a = x[rho], b = x[r1];
-        x[rho] = CT ? sel<R>(tok, b, a) : b;
-        x[r1] = CT ? sel<R>(tok, a, b) : a;
+        if (CT && !((regok >> rho) & 1u)) {
+            y[rho] = a;
+            y[r1] = b;
+            continue;
+        }
+        y[rho] = CT ? sel<R>(tok, b, a) : b;
+        y[r1] = CT ? sel<R>(tok, a, b) : a;
     }
 }
 
 template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_diag_r(cxt<R>* x, const FOp* op, bool tok) {
-    const cxt<R> d0 = opm<R>(op, 0), d1 = opm<R>(op, 1);
+__device__ __forceinline__ void do_diag_r(const cxt<R>* x, cxt<R>* y, const FOp* op,
+                                          const double* mu, bool tok) {
+    const cxt<R> d0 = opm<R>(mu, 0), d1 = opm<R>(mu, 1);
     const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const cxt<R> y = cmul<R>(((rho >> J) & 1) ? d1 : d0, x[rho]);
-        x[rho] = CT ? sel<R>(tok, y, x[rho]) : y;
+        if (CT && !((regok >> rho) & 1u)) {
+            y[rho] = x[rho];
+            continue;
+        }
+        const cxt<R> v = cmul<R>(((rho >> J) & 1) ? d1 : d0, x[rho]);
+        y[rho] = CT ? sel<R>(tok, v, x[rho]) : v;
     }
 }
 
 // diagonal factor d (per lane or uniform) on every element
 template <typename R, int NR, bool CT>
-__device__ __forceinline__ void do_scale(cxt<R>* x, const FOp* op, cxt<R> d, bool tok) {
+__device__ __forceinline__ void do_scale(const cxt<R>* x, cxt<R>* y, const FOp* op, cxt<R> d,
+                                         bool tok) {
     const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
-        if (CT && !((regok >> rho) & 1u)) continue;
-        const cxt<R> y = cmul<R>(d, x[rho]);
-        x[rho] = CT ? sel<R>(tok, y, x[rho]) : y;
+        if (CT && !((regok >> rho) & 1u)) {
+            y[rho] = x[rho];
+            continue;
+        }
+        const cxt<R> v = cmul<R>(d, x[rho]);
+        y[rho] = CT ? sel<R>(tok, v, x[rho]) : v;
     }
 }
 
 template <typename R, int NR, int J0, int J1, bool CT>
-__device__ __forceinline__ void do_u2(cxt<R>* x, const FOp* op, const cxt<R>* __restrict__ mats,
-                                      bool tok) {
+__device__ __forceinline__ void do_u2(const cxt<R>* x, cxt<R>* y, const FOp* op,
+                                      const cxt<R>* __restrict__ mats, bool tok) {
     // column-major 4x4; gate index bit 0 <-> register bit J0, bit 1 <-> J1
     const cxt<R>* M = mats + op->moff;
     cxt<R> m[16];
@@ -158,10 +205,16 @@ __device__ __forceinline__ void do_u2(cxt<R>* x, const FOp* op, const cxt<R>* __
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if (((rho >> J0) & 1) || ((rho >> J1) & 1)) continue;
-        if (CT && !((regok >> rho) & 1u)) continue;
         const int i0 = rho, i1 = rho | (1 << J0), i2 = rho | (1 << J1),
                   i3 = rho | (1 << J0) | (1 << J1);
         const cxt<R> v[4] = {x[i0], x[i1], x[i2], x[i3]};
+        if (CT && !((regok >> rho) & 1u)) {
+            y[i0] = v[0];
+            y[i1] = v[1];
+            y[i2] = v[2];
+            y[i3] = v[3];
+            continue;
+        }
         cxt<R> o[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -172,92 +225,104 @@ __device__ __forceinline__ void do_u2(cxt<R>* x, const FOp* op, const cxt<R>* __
             o[r] = s;
         }
         if (CT) {
-            x[i0] = sel<R>(tok, o[0], v[0]);
-            x[i1] = sel<R>(tok, o[1], v[1]);
-            x[i2] = sel<R>(tok, o[2], v[2]);
-            x[i3] = sel<R>(tok, o[3], v[3]);
+            y[i0] = sel<R>(tok, o[0], v[0]);
+            y[i1] = sel<R>(tok, o[1], v[1]);
+            y[i2] = sel<R>(tok, o[2], v[2]);
+            y[i3] = sel<R>(tok, o[3], v[3]);
         } else {
-            x[i0] = o[0];
-            x[i1] = o[1];
-            x[i2] = o[2];
-            x[i3] = o[3];
+            y[i0] = o[0];
+            y[i1] = o[1];
+            y[i2] = o[2];
+            y[i3] = o[3];
         }
     }
 }
 
 #define QS_J_CASES(F, ...)                                         \
-    switch (op->j0) {                                              \
+    switch (j0) {                                                  \
         case 0: F<R, NR, 0, CT>(__VA_ARGS__); break;               \
         case 1: F<R, NR, 1, CT>(__VA_ARGS__); break;               \
         case 2: F<R, NR, 2, CT>(__VA_ARGS__); break;               \
-        case 3: if constexpr (NR > 8) F<R, NR, 3, CT>(__VA_ARGS__); break; \
-        default: break;                                            \
+        default:                                                   \
+            if constexpr (NR > 8) F<R, NR, 3, CT>(__VA_ARGS__);    \
+            else copy_regs<R, NR>(x, y);                           \
+            break;                                                 \
     }
 
 template <typename R, int NR, bool CT>
-__device__ __forceinline__ void run_op_ct(cxt<R>* x, const FOp* op, const double* mu,
+__device__ __forceinline__ void run_op_ct(const cxt<R>* x, cxt<R>* y, const FOp* op,
+                                          uint32_t code, const double* mu,
                                           const cxt<R>* mats, uint32_t t, uint64_t base,
                                           bool tok) {
-    switch (op->kind) {
+    const int j0 = (code >> 4) & 3, j1 = (code >> 6) & 3;
+    switch (code & 15u) {
         case OP_U1:
-            QS_J_CASES(do_u1, x, op, mu, tok)
+            QS_J_CASES(do_u1, x, y, op, mu, tok)
             break;
         case OP_X:
-            QS_J_CASES(do_x, x, op, tok)
+            QS_J_CASES(do_x, x, y, op, tok)
             break;
         case OP_DIAG_R:
-            QS_J_CASES(do_diag_r, x, op, tok)
+            QS_J_CASES(do_diag_r, x, y, op, mu, tok)
             break;
         case OP_DIAG_T: {
             const bool one = (t & op->tbit) != 0;
-            const cxt<R> d = sel<R>(one, opm<R>(op, 1), opm<R>(op, 0));
-            do_scale<R, NR, CT>(x, op, d, tok);
+            const cxt<R> d = sel<R>(one, opm<R>(mu, 1), opm<R>(mu, 0));
+            do_scale<R, NR, CT>(x, y, op, d, tok);
             break;
         }
         case OP_DIAG_O: {
-            const cxt<R> d = (base & op->dbit) ? opm<R>(op, 1) : opm<R>(op, 0);
-            do_scale<R, NR, CT>(x, op, d, tok);
+            const cxt<R> d = (base & op->dbit) ? opm<R>(mu, 1) : opm<R>(mu, 0);
+            do_scale<R, NR, CT>(x, y, op, d, tok);
             break;
         }
-        case OP_U1B: {
-            const cxt<R>* M = mats + op->moff;
-            const uint32_t mask = op->tbit;
-            if (mask & 1u) do_u1m<R, NR, 0>(x, M);
-            if (mask & 2u) do_u1m<R, NR, 1>(x, M + 4);
-            if (mask & 4u) do_u1m<R, NR, 2>(x, M + 8);
-            if constexpr (NR > 8)
-                if (mask & 8u) do_u1m<R, NR, 3>(x, M + 12);
+        case OP_U1B:
+            do_u1b<R, NR>(x, y, mats + op->moff);
             break;
-        }
         case OP_U2: {
-            const int key = op->j0 * 8 + op->j1;
-            switch (key) {
-                case 0 * 8 + 1: do_u2<R, NR, 0, 1, CT>(x, op, mats, tok); break;
-                case 0 * 8 + 2: do_u2<R, NR, 0, 2, CT>(x, op, mats, tok); break;
-                case 1 * 8 + 2: do_u2<R, NR, 1, 2, CT>(x, op, mats, tok); break;
-                case 0 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 0, 3, CT>(x, op, mats, tok); break;
-                case 1 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 1, 3, CT>(x, op, mats, tok); break;
-                case 2 * 8 + 3: if constexpr (NR > 8) do_u2<R, NR, 2, 3, CT>(x, op, mats, tok); break;
-                default: break;
+            switch (j0 * 4 + j1) {
+                case 0 * 4 + 1: do_u2<R, NR, 0, 1, CT>(x, y, op, mats, tok); break;
+                case 0 * 4 + 2: do_u2<R, NR, 0, 2, CT>(x, y, op, mats, tok); break;
+                case 1 * 4 + 2: do_u2<R, NR, 1, 2, CT>(x, y, op, mats, tok); break;
+                case 0 * 4 + 3:
+                    if constexpr (NR > 8) do_u2<R, NR, 0, 3, CT>(x, y, op, mats, tok);
+                    else copy_regs<R, NR>(x, y);
+                    break;
+                case 1 * 4 + 3:
+                    if constexpr (NR > 8) do_u2<R, NR, 1, 3, CT>(x, y, op, mats, tok);
+                    else copy_regs<R, NR>(x, y);
+                    break;
+                case 2 * 4 + 3:
+                    if constexpr (NR > 8) do_u2<R, NR, 2, 3, CT>(x, y, op, mats, tok);
+                    else copy_regs<R, NR>(x, y);
+                    break;
+                default: copy_regs<R, NR>(x, y); break;
             }
             break;
         }
         default:
+            copy_regs<R, NR>(x, y);
             break;
     }
 }
 
-// `mu`: the 2x2 matrix an OP_U1 applies (op->m, or a per-trajectory
-// override of it)
+// y := op(x).  `mu`: the 2x2 matrix (or the diagonal) the op applies: op->m,
+// or a per-trajectory override of it.  The dispatch needs one scalar load
+// (`code`); the case then loads its matrix.
 template <typename R, int NR>
-__device__ __forceinline__ void run_op(cxt<R>* x, const FOp* op, const double* mu,
-                                       const cxt<R>* mats, uint32_t t, uint64_t base) {
-    if ((base & op->omask) != op->oval) return;  // out-of-tile control, uniform
-    if (op->ctrl) {
+__device__ __forceinline__ void run_op(const cxt<R>* x, cxt<R>* y, const FOp* op,
+                                       const double* mu, const cxt<R>* mats, uint32_t t,
+                                       uint64_t base) {
+    const uint32_t code = op->code;
+    if ((code & kCodeOut) && (base & op->omask) != op->oval) {  // out-of-tile control
+        copy_regs<R, NR>(x, y);
+        return;
+    }
+    if (code & kCodeCtrl) {
         const bool tok = (t & op->tmask) == op->tval;
-        run_op_ct<R, NR, true>(x, op, mu, mats, t, base, tok);
+        run_op_ct<R, NR, true>(x, y, op, code, mu, mats, t, base, tok);
     } else {
-        run_op_ct<R, NR, false>(x, op, mu, mats, t, base, true);
+        run_op_ct<R, NR, false>(x, y, op, code, mu, mats, t, base, true);
     }
 }
 
