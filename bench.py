@@ -499,9 +499,10 @@ def main():
     if mode == "fused" and args.autotune:
         from qpp_amd import autotune
         cands = [dict(reg_bits=4, low_bits=lb, max_ops=mo, grid_cap=gc)
-                 for lb in (5, 6) for mo in (16, 24, 32, 48, 64) for gc in (2048, 4096)]
+                 for lb in (4, 5, 6) for mo in (24, 32, 48, 64) for gc in (4096, 8192)]
         tt = time.perf_counter()
-        tuned = autotune.tune(n, gates, candidates=cands, time_budget_s=90)
+        tuned = autotune.tune(n, gates, candidates=cands, time_budget_s=90,
+                              prefix_gates=12 * (n + n // 2))
         log(f"autotune: {tuned} in {time.perf_counter() - tt:.1f} s")
         args.reg_bits, args.low_bits = tuned["reg_bits"], tuned["low_bits"]
         args.max_ops, args.grid_cap = tuned["max_ops"], tuned["grid_cap"]

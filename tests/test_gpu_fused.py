@@ -116,6 +116,27 @@ def test_fused_low_bits_variants(qa, orc):
         assert maxabs(st.download(), ref) <= TOL, (low, maxops)
 
 
+@pytest.mark.parametrize("reg", [4, 3])
+def test_fused_many_tiles_per_workgroup(qa, orc, reg):
+    """Few persistent workgroups each walking many tiles (grid_cap), so the
+    prefetch of the next tile, the LDS remaps and the stores of consecutive
+    tiles interleave; several option sets vary the phase layouts, and every
+    run must match the oracle (a race would show as a sporadic wrong
+    amplitude)."""
+    from qpp_amd import workloads
+    n = 18
+    gates = workloads.random_layers(n, 6, seed=11)
+    qc = qa.QCircuit.from_gates(n, gates)
+    rng = np.random.default_rng(12)
+    psi = randket(2 ** n, rng)
+    ref = oracle_run(orc, psi, qc)
+    for low, maxops, grid in ((6, 48, 4), (4, 32, 8), (5, 24, 16), (2, 64, 3)):
+        for rep in range(2):
+            st = qa.State.from_array(psi, [2] * n)
+            qa.Program(qc, reg_bits=reg, low_bits=low, max_ops=maxops, grid_cap=grid).run(st)
+            assert maxabs(st.download(), ref) <= TOL, (reg, low, maxops, grid, rep)
+
+
 def test_fused_c64(qa, orc):
     from qpp_amd import workloads
     n = 16
