@@ -122,6 +122,10 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         const TrajIn ti = tin[tr];
         const BClass c = cls[ti.cls];
         uint32_t ov = ti.ovr_begin;
+        // fields the lambdas below use, as scalars: capturing `ti` itself by
+        // reference would put the record in scratch memory
+        const uint32_t ovr_end = ti.ovr_end;
+        const double u_traj = ti.u;
         for (int p = c.pass_begin; p < c.pass_end; ++p) {
             const FPass* ps = passes + p;
             const bool zero_in = p == c.pass_begin;  // first pass: |0..0>
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                 for (int rho = 0; rho < NR; ++rho) x[rho] = y[rho];
                 if (tile + 1 < ntiles) fetch(tile + 1, y);
                 oc = ov;
-                uint32_t next_op = oc < ti.ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
+                uint32_t next_op = oc < ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
                 int ci = pb;
                 for (int pi = pb; pi < pe; ++pi) {
                     const FPhase* ph = phases + pi;
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                         if (uint32_t(o) != next_op) return ops[o].m;
                         const double* mu = ovr[oc].m;
                         ++oc;
-                        next_op = oc < ti.ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
+                        next_op = oc < ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
                         return mu;
                     };
                     // consecutive ops alternate between x and z (see run_op)
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
             pre[0] = 0;
             for (int j = 1; j <= kThreads; ++j) pre[j] = pre[j - 1] + pre[j];
             const double total = pre[kThreads];
-            const double u = ti.u;
+            const double u = u_traj;
             const double target = u * total;
             int b = 0;
             while (b < kThreads - 1 && pre[b + 1] < target) ++b;
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
             uint64_t found = N - 1;
             qs_serial::scan<PER>(
                 N, buf, [&](uint64_t i) { return __ddiv_rn(pnorm(a[i ^ xm]), Sv); },
-                [&]() { return pending ? ti.u : 2.0; },
+                [&]() { return pending ? u_traj : 2.0; },
                 [&](uint64_t k) {
                     found = k;
                     pending = false;
