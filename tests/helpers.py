@@ -168,3 +168,68 @@ def t_syspermute(state, perm, dims):
     D = int(np.prod(dims))
     T = state.reshape(list(dims) * 2)
     return T.transpose(list(perm) + [n + p for p in perm]).reshape(D, D)
+
+
+# ---------------------------------------------------------------------------
+# golden fixtures (tests/golden/known_answers.json)
+# ---------------------------------------------------------------------------
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def golden_known_answers():
+    import json
+    import os
+    with open(os.path.join(GOLDEN, "known_answers.json")) as f:
+        return json.load(f)["cases"]
+
+
+def golden_gate(name):
+    return {"X": X, "Y": Y, "Z": Z, "H": H, "CNOT": CNOT, "TOF": TOF,
+            "XZ": np.kron(X, Z), "ZZ": np.kron(Z, Z), "Xd3": Xd(3),
+            "Xd3Xd3": np.kron(Xd(3), Xd(3))}[name]
+
+
+def golden_state(spec, dims):
+    """A fixture state: a list of ket terms [digits, re, im], or
+    {"prj_mix": [[w, ket terms], ...]} for sum_i w_i prj(ket_i)."""
+    def ket(terms):
+        v = np.zeros(int(np.prod(dims)), dtype=complex)
+        for digits, re, im in terms:
+            v += complex(re, im) * mket(digits, dims)
+        return v
+    if isinstance(spec, dict):
+        return sum(w * prj(ket(t)) for w, t in spec["prj_mix"])
+    return ket(spec)
+
+
+def golden_run(mod, case, u=None):
+    """Runs one fixture case through `mod` (the oracle or qpp_amd, which share
+    the qpp names) and returns what the case compares."""
+    dims = case["dims"]
+    A = golden_state(case["in"], dims)
+    fn = case["fn"]
+    if fn == "apply":
+        return mod.apply(A, golden_gate(case["gate"]), case["target"], dims)
+    if fn == "applyCTRL":
+        return mod.applyCTRL(A, golden_gate(case["gate"]), case["ctrl"], case["target"], dims,
+                             shift=case["shift"])
+    if fn == "applyCTRL_fan":
+        return mod.applyCTRL_fan(A, golden_gate(case["gate"]), case["ctrl"], case["target"], dims,
+                                 shift=case["shift"])
+    if fn == "measure_seq":
+        return mod.measure_seq(A, case["target"], dims, case["destructive"], u)
+    raise ValueError(fn)
+
+
+def golden_check(mod, case):
+    if case["fn"] == "measure_seq":
+        exp = golden_state(case["out"], case["out_dims"])
+        for u in case["u"]:
+            ms, ps, st = golden_run(mod, case, u)
+            assert [int(m) for m in ms] == case["results"], (case["ref"], u)
+            assert np.allclose(ps, case["probs"], atol=1e-12, rtol=0), (case["ref"], u)
+            assert np.max(np.abs(np.asarray(st) - exp)) <= case["tol"], (case["ref"], u)
+        return
+    out = golden_run(mod, case)
+    exp = golden_state(case["out"], case["dims"])
+    assert np.max(np.abs(np.asarray(out) - exp)) <= case.get("tol", 1e-12), case["ref"]

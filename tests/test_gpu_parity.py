@@ -1,6 +1,6 @@
 """GPU engine vs the oracle, through the C ABI, on seeded inputs.
 
-Tolerances: complex128 amplitudes 1e-12 max-abs; index
+Tolerances: complex128 amplitudes 1e-12 max-abs (the parity bar); index
 maps (ptranspose, syspermute, probabilities, sampled indices) bit-exact;
 complex64 states 2e-5 max-abs.
 """
@@ -9,7 +9,8 @@ import itertools
 import numpy as np
 import pytest
 
-from helpers import CNOT, H, I2, TOF, X, Xd, Y, Z, haar, kets, mket, prj, randket, randrho
+from helpers import (CNOT, H, I2, TOF, X, Xd, Y, Z, golden_check, golden_known_answers, haar,
+                     kets, mket, prj, randket, randrho)
 
 pytestmark = pytest.mark.gpu
 
@@ -18,6 +19,16 @@ TOL = 1e-12
 
 def maxabs(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+_GOLDEN = golden_known_answers()
+
+
+@pytest.mark.parametrize("case", _GOLDEN, ids=[f"{c['fn']}-{i}" for i, c in enumerate(_GOLDEN)])
+def test_known_answers_gpu(qa, case):
+    """qpp's own unit-test known answers (tests/golden/known_answers.json),
+    computed by the GPU engine through the C ABI."""
+    golden_check(qa, case)
 
 
 def test_library_is_in_tree(qa):

@@ -13,9 +13,10 @@ import itertools
 import numpy as np
 import pytest
 
-from helpers import (CNOT, CZ, H, I2, TOF, X, Xd, Y, Z, haar, kets, mket, prj,
-                     randket, randrho, t_apply, t_ctrl_op, t_ptrace,
-                     t_ptranspose, t_syspermute)
+from helpers import (CNOT, CZ, H, I2, TOF, X, Xd, Y, Z, golden_check,
+                     golden_known_answers, haar, kets, mket, prj, randket,
+                     randrho, t_apply, t_ctrl_op, t_ptrace, t_ptranspose,
+                     t_syspermute)
 
 TOL = 1e-12
 
@@ -25,8 +26,19 @@ def close(a, b, tol=TOL):
 
 
 # ---------------------------------------------------------------------------
-# known answers: unit_tests/tests/operations.cpp:16-53
+# known answers transcribed from qpp's unit tests, kept as data in
+# tests/golden/known_answers.json (the same cases run on the GPU engine in
+# test_gpu_parity.py)
 # ---------------------------------------------------------------------------
+_GOLDEN = golden_known_answers()
+
+
+@pytest.mark.parametrize("case", _GOLDEN, ids=[f"{c['fn']}-{i}" for i, c in enumerate(_GOLDEN)])
+def test_known_answers_oracle(orc, case):
+    golden_check(orc, case)
+
+
+# known answers: unit_tests/tests/operations.cpp:16-53
 def test_apply_known_answers_qubits(orc):
     psi = kets("1")
     assert close(orc.apply(psi, X, [0], [2]), kets("0"))
@@ -401,48 +413,29 @@ def test_sample_index_and_edges(orc):
     assert idx == 1 and dig == [1, 0] and abs(pr - 0.5) < 1e-15
 
 
-CPP_DD = r"""
-#include <cstdio>
-#include <random>
-#include <vector>
-int main(int argc, char** argv) {
-    unsigned seed = std::atoi(argv[1]);
-    int K = std::atoi(argv[2]);
-    std::vector<double> p;
-    double x;
-    while (std::scanf("%lf", &x) == 1) p.push_back(x);
-    std::mt19937 gen(seed);
-    std::discrete_distribution<std::size_t> dd(p.begin(), p.end());
-    for (int i = 0; i < K; ++i) std::printf("%zu\n", dd(gen));
-    std::mt19937 g2(seed);
-    for (int i = 0; i < 4; ++i) std::printf("%.17g\n", std::generate_canonical<double, 53>(g2));
-}
-"""
-
-
-def test_discrete_draw_matches_libstdcxx(orc, tmp_path):
-    """The oracle's discrete_distribution restatement, fed the oracle's
-    generate_canonical draws, reproduces libstdc++'s own
-    std::discrete_distribution<size_t>(mt19937) index stream; the Python
-    Rng of the product package reproduces the same uniforms."""
-    import subprocess
+def test_discrete_draw_matches_libstdcxx(orc):
+    """The oracle's discrete_distribution restatement against libstdc++'s own
+    outputs (tests/golden/libstdcxx_draws.json, written by
+    tests/golden/make_libstdcxx_draws.py with g++ 11.4): fed the oracle's
+    generate_canonical draws it reproduces std::discrete_distribution's index
+    stream; its cumulative table is the sequential partial sum of libstdc++'s
+    normalised probabilities() with the last entry 1.0; and both the oracle's
+    and the product package's Rng give libstdc++'s first uniforms."""
+    import json
+    import os
     import qpp_amd
-    src = tmp_path / "dd.cpp"
-    src.write_text(CPP_DD)
-    exe = tmp_path / "dd"
-    subprocess.run(["g++", "-O2", "-o", str(exe), str(src)], check=True)
-    rng = np.random.default_rng(12)
-    for N, seed in ((7, 1), (300, 99), (4096, 12345)):
-        p = rng.random(N) ** 3
-        p[rng.integers(N, size=N // 5)] = 0.0
-        K = 500
-        out = subprocess.run([str(exe), str(seed), str(K)], input=" ".join(repr(float(v)) for v in p),
-                             capture_output=True, text=True, check=True).stdout.split()
-        ref = [int(v) for v in out[:K]]
-        ucan = [float(v) for v in out[K:]]
+    with open(os.path.join(os.path.dirname(__file__), "golden", "libstdcxx_draws.json")) as f:
+        doc = json.load(f)
+    for case in doc["cases"]:
+        p = np.array([float.fromhex(h) for h in case["weights_hex"]])
+        seed, ref = case["seed"], case["draws"]
         r = orc.Rng(seed)
-        got = [orc.discrete_draw(p, r.canonical()) for _ in range(K)]
-        assert got == ref
+        assert [orc.discrete_draw(p, r.canonical()) for _ in ref] == ref
+        q = [float.fromhex(h) for h in case["probabilities_hex"]]
+        cp = list(itertools.accumulate(q))
+        cp[-1] = 1.0
+        assert list(orc.discrete_table(p)) == cp
+        ucan = [float.fromhex(h) for h in case["canonical_hex"]]
         r2 = orc.Rng(seed)
         assert [r2.canonical() for _ in range(4)] == ucan
         py = qpp_amd.Rng(seed)
