@@ -25,7 +25,7 @@ _CACHE = {}
 CANDIDATES = [
     dict(reg_bits=rb, low_bits=lb, max_ops=mo, grid_cap=gc)
     for rb in (3, 4)
-    for lb in (4, 5, 6)
+    for lb in (3, 4, 5, 6)
     for mo in (16, 24, 32, 48, 64)
     for gc in (2048, 4096, 8192)
 ]

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kThreads) void k_tile(cxt<R>* __restrict__ a,
     constexpr int RB = RBof<NR>::v;
     const int pb = ps->phase_begin, pe = ps->phase_end;
     const uint64_t ntiles = ps->ntiles;
-    const FPhase* ld = phases + pb;  // load layout (lanes on tile bits 0..5)
+    const FPhase* ld = phases + pb;  // load layout (a phase the host marked coalesced)
     const uint64_t gt_ld = thr_g(ld, t);
     auto tile_base = [&](uint64_t tile) { return tile_base_of(ps, tile); };
     cxt<R> x[NR], y[NR], z[NR];
@@ -195,15 +195,18 @@ std::vector<size_t> greedy(const std::vector<Item>& items, const std::vector<siz
 
 // Layout of a phase: register tile bits R (ascending; register bit k is
 // R[k]); the thread number enumerates the other tile bits in ascending
-// order.
+// order.  The phase loads and stores HBM directly ("coalesced") when no
+// register bit is below `lc`: lanes 0..lc-1 then sit on tile bits 0..lc-1,
+// which are global bits 0..lc-1 (lc <= low_bits), so every 2^lc lanes access
+// 16 * 2^lc contiguous bytes.
 FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
-                  const std::vector<uint32_t>& Sv) {
+                  const std::vector<uint32_t>& Sv, uint32_t lc) {
     FPhase ph{};
     std::vector<uint32_t> Rv(Rset.begin(), Rset.end());
     ph.coalesced = 1;
     for (uint32_t k = 0; k < RB; ++k) {
         ph.regbits[k] = static_cast<uint8_t>(Rv[k]);
-        if (Rv[k] < 6) ph.coalesced = 0;
+        if (Rv[k] < lc) ph.coalesced = 0;
     }
     const uint32_t NR = 1u << RB;
     for (uint32_t rho = 0; rho < NR; ++rho) {
@@ -373,6 +376,13 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
     const uint32_t m = 8 + RB;
     if (nb < m) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::fused", "state below tile size");
     const uint32_t L = std::min<uint32_t>(opt.low_bits, m);
+    // Any layout whose lanes keep tile bits 0..1 loads and stores HBM
+    // directly (64-B runs): a wave reads the rest of each 128-B line through
+    // its other registers in the same burst, and the halves meet in L2.
+    // Measured on the 30-qubit depth-40 circuit, requiring longer runs
+    // (up to 1 KiB) only adds layout changes: 823 ms per circuit with 64-B
+    // runs against 881 ms with 1-KiB runs (low_bits 3, max_ops 48).
+    const uint32_t lc = std::min<uint32_t>(L, 2);
     // scheduling items: the bits a gate needs in the tile (none for a
     // diagonal 1-bit gate) and every bit it depends on
     std::vector<Item> items(gates.size());
@@ -437,11 +447,11 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
             std::vector<size_t> ptake = greedy(pit, ptodo, R, RB, SIZE_MAX);
             // complete R with unused tile bits, preferring high ones
             for (int b = int(m) - 1; R.size() < RB && b >= 0; --b) R.insert(uint32_t(b));
-            FPhase ph = make_phase(R, m, RB, Sv);
+            FPhase ph = make_phase(R, m, RB, Sv, lc);
             if (first && !ph.coalesced) {
                 std::set<uint32_t> R0;
                 for (uint32_t k = 0; k < RB; ++k) R0.insert(m - RB + k);
-                FPhase ldp = make_phase(R0, m, RB, Sv);
+                FPhase ldp = make_phase(R0, m, RB, Sv, lc);
                 ldp.op_begin = ldp.op_end = static_cast<int>(P.ops.size());
                 P.phases.push_back(ldp);
             }
