@@ -194,8 +194,8 @@ std::vector<size_t> greedy(const std::vector<Item>& items, const std::vector<siz
 }
 
 // Layout of a phase: register tile bits R (ascending; register bit k is
-// R[k]); the thread number enumerates the other tile bits in ascending
-// order.  The phase loads and stores HBM directly ("coalesced") when no
+// R[k]); the thread number enumerates the other tile bits (order below).
+// The phase loads and stores HBM directly ("coalesced") when no
 // register bit is below `lc`: lanes 0..lc-1 then sit on tile bits 0..lc-1,
 // which are global bits 0..lc-1 (lc <= low_bits), so every 2^lc lanes access
 // 16 * 2^lc contiguous bytes.
@@ -225,14 +225,30 @@ FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
                 ph.s_rb[k] = swz(e);
             }
     }
-    uint32_t j = 0;
+    // Thread bits: lanes 0..3 take tile bits with residues 0, 1, 2, 3 mod 4
+    // when there are such bits (lowest first).  A tile bit b moves the
+    // swizzled LDS index in its low nibble bit b mod 4, so 16 lanes that
+    // differ in four bits of distinct residues touch 16 distinct 16-byte
+    // slots of a 256-byte LDS row (no bank conflicts); the lowest-first
+    // choice keeps tile bits 0..1 on lanes 0..1 whenever they are free,
+    // which direct HBM access needs.  The other tile bits follow in
+    // ascending order.
+    std::vector<uint32_t> rest;
     for (uint32_t b = 0; b < m; ++b)
-        if (!Rset.count(b)) {
-            ph.tbits[j] = static_cast<uint8_t>(b);
-            ph.g_thr[j] = uint64_t{1} << Sv[b];
-            ph.s_thr[j] = swz(1u << b);
-            ++j;
-        }
+        if (!Rset.count(b)) rest.push_back(b);
+    std::vector<uint32_t> order;
+    for (uint32_t res = 0; res < 4; ++res) {
+        auto it = std::find_if(rest.begin(), rest.end(), [&](uint32_t b) { return b % 4 == res; });
+        if (it == rest.end()) break;  // keep lanes contiguous in residue order
+        order.push_back(*it);
+        rest.erase(it);
+    }
+    order.insert(order.end(), rest.begin(), rest.end());
+    for (uint32_t j = 0; j < order.size(); ++j) {
+        ph.tbits[j] = static_cast<uint8_t>(order[j]);
+        ph.g_thr[j] = uint64_t{1} << Sv[order[j]];
+        ph.s_thr[j] = swz(1u << order[j]);
+    }
     return ph;
 }
 
