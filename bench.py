@@ -477,6 +477,7 @@ def main():
     gates = workloads.random_layers(n, depth, args.seed)
     n1q = sum(1 for g in gates if g[0] == "u")
     ncx = len(gates) - n1q
+    unfused_bytes = (n1q * 2 + ncx) * 16 * 2 ** n  # dense: read+write; CNOT: half the state
     st = qa.State([2] * n)
 
     mode = args.mode
@@ -499,7 +500,7 @@ def main():
     if mode == "fused" and args.autotune:
         from qpp_amd import autotune
         cands = [dict(reg_bits=4, low_bits=lb, max_ops=mo, grid_cap=gc)
-                 for lb in (3, 4, 5) for mo in (32, 48, 64) for gc in (8192, 16384)]
+                 for lb in (3, 4, 5) for mo in (48, 64, 96) for gc in (8192, 16384)]
         tt = time.perf_counter()
         tuned = autotune.tune(n, gates, candidates=cands, time_budget_s=90,
                               prefix_gates=12 * (n + n // 2))
@@ -574,6 +575,10 @@ def main():
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "gate_apps_per_s": round(len(gates) / sec, 2),
         "circuits_per_s": round(1.0 / sec, 4),
+        # the HBM rate a gate-by-gate engine (one state pass per gate:
+        # 2*16*2^n B per dense gate, half of it per CNOT) would need to
+        # finish the circuit in the same time; context only, not `value`
+        "unfused_equivalent_GBps": round(unfused_bytes / sec / 1e9, 1),
         "roofline": {
             "bound": "hbm",
             "kernel": kernel_name,

@@ -26,7 +26,7 @@ CANDIDATES = [
     dict(reg_bits=rb, low_bits=lb, max_ops=mo, grid_cap=gc)
     for rb in (3, 4)
     for lb in (3, 4, 5, 6)
-    for mo in (16, 24, 32, 48, 64)
+    for mo in (16, 32, 48, 64, 96)
     for gc in (2048, 4096, 8192, 16384)
 ]
 

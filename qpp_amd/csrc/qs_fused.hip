@@ -193,6 +193,49 @@ std::vector<size_t> greedy(const std::vector<Item>& items, const std::vector<siz
     return out;
 }
 
+// The gates a pass over the fixed tile-bit set S (a bit mask) takes: in
+// order, every gate whose needed bits lie in S and that touches no blocked
+// bit; a gate that cannot be taken blocks every bit it touches.
+size_t take_count(const std::vector<uint64_t>& need, const std::vector<uint64_t>& touch,
+                  const std::vector<size_t>& todo, uint64_t S, size_t max_take) {
+    uint64_t blocked = 0;
+    size_t n = 0;
+    for (size_t id : todo) {
+        if (!(touch[id] & blocked) && !(need[id] & ~S) && n < max_take) {
+            ++n;
+            continue;
+        }
+        blocked |= touch[id];
+    }
+    return n;
+}
+
+// Improves a pass's tile-bit set by hill climbing: swap one of its free bits
+// (not among the `fixed` low bits) for a bit outside it while that lets the
+// pass take more gates.
+uint64_t improve_set(const std::vector<uint64_t>& need, const std::vector<uint64_t>& touch,
+                     const std::vector<size_t>& todo, uint64_t S, uint64_t fixed, uint32_t nb,
+                     size_t max_take) {
+    size_t best = take_count(need, touch, todo, S, max_take);
+    for (bool improved = true; improved && best < max_take;) {
+        improved = false;
+        for (uint32_t b = 0; b < nb && !improved; ++b) {
+            if (!((S >> b) & 1) || ((fixed >> b) & 1)) continue;
+            for (uint32_t c = 0; c < nb && !improved; ++c) {
+                if ((S >> c) & 1) continue;
+                const uint64_t S2 = (S & ~(uint64_t{1} << b)) | (uint64_t{1} << c);
+                const size_t v = take_count(need, touch, todo, S2, max_take);
+                if (v > best) {
+                    best = v;
+                    S = S2;
+                    improved = true;
+                }
+            }
+        }
+    }
+    return S;
+}
+
 // Layout of a phase: register tile bits R (ascending; register bit k is
 // R[k]); the thread number enumerates the other tile bits (order below).
 // The phase loads and stores HBM directly ("coalesced") when no
@@ -414,6 +457,12 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
     std::vector<size_t> todo(gates.size());
     for (size_t i = 0; i < todo.size(); ++i) todo[i] = i;
     P.gate_op.assign(gates.size(), -1);
+    std::vector<uint64_t> need_m(items.size()), touch_m(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        for (uint32_t b : items[i].need) need_m[i] |= uint64_t{1} << b;
+        for (uint32_t b : items[i].touch) touch_m[i] |= uint64_t{1} << b;
+    }
+    const uint64_t low_mask = L >= 64 ? ~uint64_t{0} : (uint64_t{1} << L) - 1;
     std::vector<C> mats;
     while (!todo.empty()) {
         std::set<uint32_t> S;
@@ -422,6 +471,18 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
         if (take.empty()) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::fused", "gate wider than a tile");
         // fill S up to m bits with the lowest unused bits
         for (uint32_t b = 0; S.size() < m && b < nb; ++b) S.insert(b);
+        if (opt.improve_sets) {
+            uint64_t Sm = 0;
+            for (uint32_t b : S) Sm |= uint64_t{1} << b;
+            const uint64_t S2 = improve_set(need_m, touch_m, todo, Sm, low_mask, nb, opt.max_ops);
+            if (S2 != Sm) {
+                S.clear();
+                for (uint32_t b = 0; b < nb; ++b)
+                    if ((S2 >> b) & 1) S.insert(b);
+                std::set<uint32_t> full = S;
+                take = greedy(items, todo, full, m, opt.max_ops);
+            }
+        }
         FPass ps{};
         ps.m = static_cast<int>(m);
         std::vector<uint32_t> Sv(S.begin(), S.end());

@@ -99,6 +99,7 @@ struct FusedOptions {
     uint32_t low_bits = 6;  // low index bits every tile keeps (coalescing)
     size_t max_ops = 64;    // gates per pass at most (balances VALU vs HBM)
     bool bundle = true;     // group commuting 1-bit ops of a phase (OP_U1B)
+    bool improve_sets = true;  // hill-climb each pass's tile-bit set (more gates per pass)
 };
 
 struct FusedProgram {
