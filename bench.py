@@ -477,7 +477,6 @@ def main():
     gates = workloads.random_layers(n, depth, args.seed)
     n1q = sum(1 for g in gates if g[0] == "u")
     ncx = len(gates) - n1q
-    unfused_bytes = (n1q * 2 + ncx) * 16 * 2 ** n  # dense: read+write; CNOT: half the state
     st = qa.State([2] * n)
 
     mode = args.mode
@@ -513,13 +512,16 @@ def main():
         run = lambda: prog.run(st)  # noqa: E731
         npasses = prog.num_passes()
         pass_bytes = npasses * 2 * 16 * 2 ** n
-        algo_bytes = pass_bytes
         kernel_name = prog.kernel_name()
     else:
         run = lambda: run_gates(qa, st, gates)  # noqa: E731
         npasses = len(gates)
-        algo_bytes = sum(workloads.gate_bytes(g, n) for g in gates)
+        pass_bytes = sum(workloads.gate_bytes(g, n) for g in gates)
         kernel_name = "k_gate1_hi<double,4>"
+    # gate-apply bytes: every gate application at its own algorithmic bytes
+    # (2*16*2^n per dense gate, 2*16*2^(n-1) per CNOT), whether the engine
+    # fuses the gates into passes or not
+    algo_bytes = sum(workloads.gate_bytes(g, n) for g in gates)
 
     for w in range(args.warmup):
         run()
@@ -573,12 +575,13 @@ def main():
                           "autotuned": tuned is not None},
         },
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "value_definition": "gate applications' algorithmic bytes (2*16*2^n per dense 1q gate, "
+                            "2*16*2^(n-1) per CNOT) / circuit wall time; with fusion this exceeds "
+                            "the HBM rate the passes actually stream, which is fused_pass_GBps and "
+                            "roofline.achieved",
+        "fused_pass_GBps": round(pass_bytes / sec / 1e9, 1),
         "gate_apps_per_s": round(len(gates) / sec, 2),
         "circuits_per_s": round(1.0 / sec, 4),
-        # the HBM rate a gate-by-gate engine (one state pass per gate:
-        # 2*16*2^n B per dense gate, half of it per CNOT) would need to
-        # finish the circuit in the same time; context only, not `value`
-        "unfused_equivalent_GBps": round(unfused_bytes / sec / 1e9, 1),
         "roofline": {
             "bound": "hbm",
             "kernel": kernel_name,
