@@ -84,11 +84,24 @@ def cpu_baseline(gates, n, sample_gates):
         "unit": "GB/s",
         "cores": int(threads),
         "kind": "port",
-        "sample": f"first {len(g)} gates of the same 30-qubit circuit "
+        "sample": f"first {len(g)} gates of the same {n}-qubit circuit "
                   f"({int((kind == 0).sum())} dense 1q + {int((kind == 1).sum())} CNOT), "
                   f"{total:.2f} s, {len(g) / total:.3f} gate-apps/s; qpp 7.0.3 qubit kernels "
                   f"restated (apply_psi_1q / apply_ctrl_psi_1q, out-of-place)",
     }
+
+
+def oracle_threads():
+    """OpenMP threads the oracle runs with (orc_time_gates with no gates
+    only reports them)."""
+    import ctypes
+    from ctypes import POINTER, c_double, c_uint32
+    import oracle
+    L = oracle.lib()
+    L.orc_time_gates.argtypes = [c_uint32, c_uint32, POINTER(c_uint32), POINTER(c_uint32),
+                                 POINTER(c_uint32), POINTER(c_double), POINTER(c_double)]
+    L.orc_time_gates.restype = ctypes.c_int
+    return int(L.orc_time_gates(1, 0, None, None, None, None, None))
 
 
 def run_gates(qa, st, gates):
@@ -158,8 +171,25 @@ def bench_batch(args, qa):
     # oracle QNoisyEngine on a few 12-qubit trajectories (same seeds)
     import oracle
     i12 = [j for j in range(ntraj) if cof[j] == 1][: args.oracle_traj]
+    t_c = time.perf_counter()
     om = sum(1 for j in i12
              if tuple(oracle.engine_run(circs[1], 1, p, int(seeds[j]))[1]) == b.dits(1, idx[j]))
+    t12 = time.perf_counter() - t_c
+    # the oracle's noisy engine (qpp's loop, restated) on 16-qubit trajectories
+    # of the batch: outcome parity and the CPU baseline
+    i16 = [j for j in range(ntraj) if cof[j] == 0][: args.oracle_traj16]
+    t_c = time.perf_counter()
+    om16 = sum(1 for j in i16
+               if tuple(oracle.engine_run(circs[0], 1, p, int(seeds[j]))[1]) == b.dits(0, idx[j]))
+    t16 = time.perf_counter() - t_c
+    cpu = None
+    if i16:
+        cpu = {"value": round(len(i16) / t16, 3), "unit": "trajectories/s (16 qubits)",
+               "cores": oracle_threads(), "kind": "port",
+               "sample": f"{len(i16)} 16-qubit and {len(i12)} 12-qubit trajectories of the same "
+                         f"batch (same seeds) through the oracle's QNoisyEngine (qpp's execute "
+                         f"loop and qubit kernels restated): {t16:.2f} s and {t12:.2f} s; "
+                         f"outcomes equal {om16}/{len(i16)} and {om}/{len(i12)}"}
     achieved = algo_bytes / (np.mean(dev_ms) / 1e3) / 1e9
     return {
         "metric": "batched noisy trajectories/s (mixed 16/12/20/22 qubits)",
@@ -181,12 +211,12 @@ def bench_batch(args, qa):
         "exact_replays": inf["exact_replays"],
         "precision_noiseless_c64_vs_c128": prec,
         "outcomes_equal_c128_batch": agree,
-        "oracle_outcomes_matched": f"{om}/{len(i12)} (12-qubit trajectories)",
+        "oracle_outcomes_matched": f"{om}/{len(i12)} (12-qubit), {om16}/{len(i16)} (16-qubit)",
         "roofline": {"bound": "hbm", "kernel": "k_traj", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_ms": round(float(np.mean(dev_ms)), 3)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -233,9 +263,11 @@ def bench_serve(args, qa):
         tickets.append(srv.submit(c, q["dtype"], 1024, q["seed"]))
         if (k + 1) % 50 == 0:
             log(f"serve: submitted {k + 1}/{nreq}, {srv.info()}")
-    lat, stats = [], []
+    lat, run_ms, stats = [], [], []
     for t in tickets:
-        lat.append(srv.wait(t)[2])
+        w = srv.wait(t)
+        lat.append(w[2])
+        run_ms.append(w[1])
         stats.append(srv.stats(t))
     wall = time.perf_counter() - start
     info = srv.info()
@@ -273,6 +305,27 @@ def bench_serve(args, qa):
         n64 += 1
         ok64 += int(inf <= 1e-5)
     frac = (ok64 + (nreq - n64)) / nreq  # complex128 requests: exact to rounding
+    # CPU baseline: small complex128 requests of the same stream through the
+    # oracle's QEngine (qpp's execute loop and kernels restated), same seeds;
+    # their statistics must equal the served ones
+    cpu = None
+    small = [k for k, q in enumerate(reqs) if q["n"] <= 14 and q["dtype"] == qa.C128][: args.cpu_requests]
+    if small:
+        import oracle
+        t_c = time.perf_counter()
+        eq = sum(int(oracle.engine_run(circs[k], 1024, -1.0, reqs[k]["seed"])[0] == stats[k][0])
+                 for k in small)
+        t_o = time.perf_counter() - t_c
+        g_ms = sum(run_ms[k] for k in small)
+        cpu = {"value": round(len(small) / t_o, 3), "unit": "circuits/s (small requests only)",
+               "cores": oracle_threads(), "kind": "port",
+               "sample": f"{len(small)} complex128 requests of the stream with n <= 14 "
+                         f"(n = {[reqs[k]['n'] for k in small]}, depth = "
+                         f"{[reqs[k]['depth'] for k in small]}) through the oracle's "
+                         f"QEngine.execute(1024): {t_o * 1e3:.1f} ms (the GPU server ran the "
+                         f"same requests in {g_ms:.1f} ms in the mix); statistics equal to the "
+                         f"served ones {eq}/{len(small)}; requests up to 26 qubits and depth "
+                         f"100 are out of reach for a bounded CPU sample"}
     return {
         "metric": "served circuits/s + p99 latency (1024 shots each)",
         "value": round(nreq / wall, 3),
@@ -296,7 +349,7 @@ def bench_serve(args, qa):
                      "achieved": round(pass_bytes / wall / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(pass_bytes / wall / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -363,6 +416,32 @@ def bench_rho(args, qa):
     algo = workloads.dm_bytes(n, ops) + (s * 2 ** (2 * n - len(tr)) + s * 4 ** (n - len(tr))) \
         + 2 * (2 * s * 4 ** n)
     pass_bytes = npass * 2 * s * 4 ** n
+    # CPU baseline: the oracle (qpp's rho kernels restated, out-of-place, as
+    # qpp applies them) on the first ops of the same pipeline, then its maps
+    cpu = None
+    if args.cpu_ops > 0:
+        sub = ops[: args.cpu_ops]
+        r = rho0
+        t_c = time.perf_counter()
+        for kind, q, M in sub:
+            if kind == "u":
+                r = oracle.apply(r, M, q, [2] * n)
+            elif kind == "cz":
+                r = oracle.applyCTRL(r, M, [q[0]], [q[1]], [2] * n)
+            else:
+                r = oracle.apply_kraus(r, M, q, [2] * n)
+        t_ops = time.perf_counter() - t_c
+        t_c = time.perf_counter()
+        oracle.ptrace(r, tr, [2] * n)
+        oracle.ptranspose(r, ptt, [2] * n, semantics=1)
+        oracle.syspermute(r, perm, [2] * n, semantics=1)
+        t_maps = time.perf_counter() - t_c
+        cpu_bytes = workloads.dm_bytes(n, sub) + (algo - workloads.dm_bytes(n, ops))
+        cpu = {"value": round(cpu_bytes / (t_ops + t_maps) / 1e9, 3), "unit": "GB/s",
+               "cores": oracle_threads(), "kind": "port",
+               "sample": f"first {len(sub)} of the {len(ops)} gates/channels ({t_ops:.2f} s, "
+                         f"{t_ops / len(sub) * 1e3:.1f} ms each) and the three maps "
+                         f"({t_maps:.2f} s) through the oracle (qpp's rho kernels restated)"}
     return {
         "metric": "12-qubit density-matrix pipeline GB/s (algorithmic)",
         "value": round(algo / wall / 1e9, 2),
@@ -384,7 +463,7 @@ def bench_rho(args, qa):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(pass_bytes / (fused_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None, "avg_launch_ms": round(fused_ms / npass, 4)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
 
 
@@ -440,8 +519,15 @@ def main():
     ap.add_argument("--traj-depth", type=int, default=30)
     ap.add_argument("--oracle-traj", type=int, default=16,
                     help="batch mode: 12-qubit trajectories checked against the oracle engine")
-    ap.add_argument("--cpu-gates", type=int, default=6,
-                    help="gates in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--oracle-traj16", type=int, default=6,
+                    help="batch mode: 16-qubit trajectories run on the oracle engine (parity + CPU baseline)")
+    ap.add_argument("--cpu-ops", type=int, default=24,
+                    help="rho mode: gates/channels of the pipeline timed on the oracle (CPU baseline)")
+    ap.add_argument("--cpu-requests", type=int, default=4,
+                    help="serve mode: small requests run on the oracle engine (parity + CPU baseline)")
+    ap.add_argument("--cpu-gates", type=int, default=-1,
+                    help="gates in the CPU baseline sample (0 = skip; default: the whole "
+                         "circuit up to 22 qubits, else the first 6 gates)")
     ap.add_argument("--reg-bits", type=int, default=4)
     ap.add_argument("--low-bits", type=int, default=5)
     ap.add_argument("--max-ops", type=int, default=64)
@@ -605,9 +691,10 @@ def main():
             out["roofline"]["traffic"] = tr["bytes_per_launch"]
     except (OSError, ValueError):
         pass
-    if args.cpu_gates > 0:
-        log(f"cpu baseline: {args.cpu_gates} gates at {n} qubits ...")
-        out["cpu_baseline"] = cpu_baseline(gates, n, args.cpu_gates)
+    ncpu = args.cpu_gates if args.cpu_gates >= 0 else (len(gates) if n <= 22 else 6)
+    if ncpu > 0:
+        log(f"cpu baseline: {ncpu} gates at {n} qubits ...")
+        out["cpu_baseline"] = cpu_baseline(gates, n, ncpu)
     print(json.dumps(out), flush=True)
 
 
