@@ -528,6 +528,8 @@ def main():
     ap.add_argument("--cpu-gates", type=int, default=-1,
                     help="gates in the CPU baseline sample (0 = skip; default: the whole "
                          "circuit up to 22 qubits, else the first 6 gates)")
+    ap.add_argument("--precision", type=int, default=1,
+                    help="fused mode: also run the circuit in complex64 and compare with complex128 (1/0)")
     ap.add_argument("--reg-bits", type=int, default=4)
     ap.add_argument("--low-bits", type=int, default=5)
     ap.add_argument("--max-ops", type=int, default=64)
@@ -638,6 +640,31 @@ def main():
     achieved = k_bytes / (avg_ms / 1e3) / 1e9
     log(f"dominant kernel {kernel_name}: {avg_ms:.3f} ms/launch, {achieved:.1f} GB/s")
 
+    # precision split: the same circuit from |0..0> in complex64 (FP32
+    # amplitudes, FP64 reductions) against complex128, compared on the device
+    # in FP64
+    precision = None
+    if mode == "fused" and args.precision:
+        st.set_basis(0)
+        prog.run(st)
+        s64 = qa.State([2] * n, dtype=qa.C64)
+        p64 = qa.Program(n, gates, dtype=qa.C64, reg_bits=args.reg_bits, low_bits=args.low_bits,
+                         max_ops=args.max_ops, grid_cap=args.grid_cap)
+        p64.run(s64)
+        cmp = st.compare(s64)
+        s64.set_basis(0)
+        ms64 = profile_region(qa, lambda: p64.run(s64))
+        precision = {"complex64_ms_per_circuit": round(ms64, 2),
+                     "complex64_circuits_per_s": round(1e3 / ms64, 4),
+                     "complex64_passes": p64.num_passes(),
+                     "infidelity_vs_complex128": cmp["infidelity"],
+                     "trace_distance_vs_complex128": cmp["trace_distance"],
+                     "max_abs_vs_complex128": cmp["max_abs"],
+                     "complex128_norm2": cmp["norm2_a"], "complex64_norm2": cmp["norm2_b"]}
+        log(f"precision: c64 {ms64:.1f} ms/circuit, infidelity {cmp['infidelity']:.3e}, "
+            f"max-abs {cmp['max_abs']:.3e}")
+        del s64, p64
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -668,6 +695,7 @@ def main():
         "fused_pass_GBps": round(pass_bytes / sec / 1e9, 1),
         "gate_apps_per_s": round(len(gates) / sec, 2),
         "circuits_per_s": round(1.0 / sec, 4),
+        "precision": precision,
         "roofline": {
             "bound": "hbm",
             "kernel": kernel_name,
