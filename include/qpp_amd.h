@@ -66,6 +66,7 @@ typedef enum qs_status {
     QS_ERR_QUDIT_ALREADY_MEASURED = 16,  /* exception::QuditAlreadyMeasured */
     QS_ERR_NOT_IMPLEMENTED = 17,         /* exception::NotImplemented */
     QS_ERR_OVERFLOW = 18,                /* std::overflow_error (normalize) */
+    QS_ERR_IO = 19,                      /* std::runtime_error (load/save streams) */
     /* engine-side conditions with no qpp counterpart */
     QS_ERR_INVALID_HANDLE = 100,
     QS_ERR_OUT_OF_MEMORY = 101,          /* device memory exhausted */
@@ -119,6 +120,23 @@ int qs_state_info(const qs_state* st, qs_dtype* dtype, qs_kind* kind,
  * complex64: 8 B each), column-major. */
 int qs_state_upload(qs_state* st, const void* host, uint64_t nbytes);
 int qs_state_download(const qs_state* st, void* host, uint64_t nbytes);
+/* Files in qpp's formats (input_output.hpp), so states travel between qpp
+ * and the engine.  A ket is written as a D x 1 matrix, a density matrix as
+ * D x D; complex64 states are written as complex128 (qpp's cplx).
+ *   QS_FMT_TEXT: qpp::save(A, os) / qpp::load<cmat>(is), input_output.hpp:
+ *     177-211 / 233-262: "rows cols\n", then one line per row of "(re,im)"
+ *     entries separated by single spaces, each real with max_digits10 (17)
+ *     significant digits (internal/util.hpp:440-446).
+ *   QS_FMT_BINARY: qpp::obsolete::save / load, input_output.hpp:318-345 /
+ *     357-390: the 19 bytes "TYPE::Eigen::Matrix", rows and cols as 8-byte
+ *     size_t, then the column-major complex<double> data.
+ * qs_state_load creates a state with the given dims from a file whose
+ * rows x cols must be D x 1 (QS_KET) or D x D (QS_RHO).  Stream errors
+ * return QS_ERR_IO with qpp's std::runtime_error text. */
+typedef enum qs_format { QS_FMT_TEXT = 0, QS_FMT_BINARY = 1 } qs_format;
+int qs_state_save(const qs_state* st, const char* path, qs_format fmt);
+int qs_state_load(const char* path, qs_format fmt, qs_dtype dtype, qs_kind kind,
+                  uint32_t ndims, const uint64_t* dims, qs_state** out);
 /* Basis state |index> or |index><index|. */
 int qs_state_set_basis(qs_state* st, uint64_t index);
 /* Raw device pointer and HIP stream of a handle (for zero-copy callers). */

@@ -121,6 +121,7 @@ inline void check(int rc) {
         case QS_ERR_QUDIT_ALREADY_MEASURED: throw exception::QuditAlreadyMeasured(m);
         case QS_ERR_NOT_IMPLEMENTED: throw exception::NotImplemented(m);
         case QS_ERR_OVERFLOW: throw std::overflow_error("qpp::normalize(): Division by zero!");
+        case QS_ERR_IO: throw std::runtime_error(m);
         default: throw exception::DeviceError(m);
     }
 }
@@ -164,6 +165,23 @@ class State {
     }
     ~State() { qs_state_destroy(h_); }
     qs_state* get() const { return h_; }
+
+    // qpp::save(A, os) (QS_FMT_TEXT) / qpp::obsolete::save (QS_FMT_BINARY)
+    // of this state into a file (input_output.hpp:177-211, 318-345)
+    const State& save(const std::string& path, qs_format fmt = QS_FMT_TEXT) const {
+        check(qs_state_save(h_, path.c_str(), fmt));
+        return *this;
+    }
+    // qpp::load<cmat>(is) / qpp::obsolete::load<cmat>(is) into a device state
+    static State load(const std::string& path, const std::vector<idx>& dims,
+                      qs_kind kind = QS_KET, qs_dtype dt = QS_C128,
+                      qs_format fmt = QS_FMT_TEXT) {
+        std::vector<uint64_t> d(dims.begin(), dims.end());
+        qs_state* h = nullptr;
+        check(qs_state_load(path.c_str(), fmt, dt, kind, static_cast<uint32_t>(d.size()), d.data(),
+                            &h));
+        return State(h);
+    }
 
     cmat download() const {
         qs_dtype dt;

@@ -38,6 +38,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqpp_amd.so")
 
 C128, C64 = 0, 1
+TEXT, BINARY = 0, 1  # qs_format: qpp::save/load text, qpp::obsolete binary
 KET, RHO = 0, 1
 DOCUMENTED, AS_BUILT = 0, 1
 
@@ -74,6 +75,7 @@ SizeMismatch = _mk("SizeMismatch", 15)
 QuditAlreadyMeasured = _mk("QuditAlreadyMeasured", 16)
 NotImplemented_ = _mk("NotImplemented", 17)
 OverflowError_ = _mk("OverflowError", 18)
+RuntimeError_ = _mk("RuntimeError", 19)  # std::runtime_error (load/save streams)
 InvalidHandle = _mk("InvalidHandle", 100)
 OutOfMemory = _mk("OutOfMemory", 101)
 DeviceError = _mk("DeviceError", 102)
@@ -84,8 +86,8 @@ _BY_CODE = {c.code: c for c in [
     MatrixMismatchSubsys, DimsInvalid, DimsNotEqual, DimsMismatchMatrix,
     DimsMismatchCvector, SubsysMismatchDims, PermInvalid, PermMismatchDims,
     OutOfRange, TypeMismatch, SizeMismatch, QuditAlreadyMeasured,
-    NotImplemented_, OverflowError_, InvalidHandle, OutOfMemory, DeviceError,
-    NoDevice]}
+    NotImplemented_, OverflowError_, RuntimeError_, InvalidHandle, OutOfMemory,
+    DeviceError, NoDevice]}
 
 # ---------------------------------------------------------------------------
 # library
@@ -126,6 +128,8 @@ def lib():
             "qs_normalize": (c_int, [vp]),
             "qs_compare": (c_int, [vp, vp, dp]),
             "qs_state_convert": (c_int, [vp, c_int, POINTER(vp)]),
+            "qs_state_save": (c_int, [vp, c_char_p, c_int]),
+            "qs_state_load": (c_int, [c_char_p, c_int, c_int, c_int, c_uint32, u64p, POINTER(vp)]),
             "qs_probs": (c_int, [vp, dp, c_uint64]),
             "qs_sample": (c_int, [vp, c_uint32, u64p, c_uint64, dp, u64p, u64p, dp]),
             "qs_measure_seq": (c_int, [vp, c_uint32, u64p, c_int, c_double, u64p, dp]),
@@ -263,6 +267,22 @@ class State:
 
     def set_basis(self, index):
         _check(lib().qs_state_set_basis(self._h, index))
+
+    def save(self, path, fmt=TEXT):
+        """Write the state to `path` in qpp's format (TEXT: qpp::save,
+        BINARY: qpp::obsolete::save; input_output.hpp:177-211, 318-345)."""
+        _check(lib().qs_state_save(self._h, os.fsencode(path), int(fmt)))
+        return self
+
+    @classmethod
+    def load(cls, path, dims, kind=KET, dtype=C128, fmt=TEXT):
+        """A state with `dims` read from a file in qpp's format (TEXT:
+        qpp::load<cmat>, BINARY: qpp::obsolete::load<cmat>)."""
+        d, dp = _u64(dims)
+        h = c_void_p()
+        _check(lib().qs_state_load(os.fsencode(path), int(fmt), int(dtype), int(kind), len(d), dp,
+                                   byref(h)))
+        return cls(list(dims), kind, dtype, _handle=h)
 
     # in-place operations -------------------------------------------------
     def apply(self, A, target):
@@ -483,6 +503,19 @@ def sample(A, target, dims=2, u=None):
     if np.ndim(u) == 0:
         return [int(x) for x in dig[0]], float(pr[0])
     return dig, pr
+
+
+def save(A, path, dims=2, fmt=TEXT):
+    """qpp::save(A, os) (TEXT) or qpp::obsolete::save (BINARY) of a ket or
+    density matrix, written through the device state."""
+    dims = _dims_of(A, dims)
+    _as_state(A, dims).save(path, fmt)
+
+
+def load(path, dims, kind=KET, fmt=TEXT):
+    """qpp::load<cmat>(is) (TEXT) or qpp::obsolete::load (BINARY) as a numpy
+    array (D for a ket, D x D for a density matrix)."""
+    return State.load(path, dims, kind, C128, fmt).download()
 
 
 def measure_seq(A, target, dims=2, destructive=True, u=None):

@@ -24,6 +24,7 @@ SOURCES = [
     "qs_precision.hip",
     "qs_batch.hip",
     "qs_serve.hip",
+    "qs_io.hip",
 ]
 
 FLAGS = [

@@ -56,6 +56,7 @@ const char* description(int code) {
             return "Qudit was already measured (destructively)";
         case QS_ERR_NOT_IMPLEMENTED: return "Not yet implemented";
         case QS_ERR_OVERFLOW: return "Division by zero";
+        case QS_ERR_IO: return "I/O error";
         case QS_ERR_INVALID_HANDLE: return "Invalid handle";
         case QS_ERR_OUT_OF_MEMORY: return "Out of device memory";
         case QS_ERR_DEVICE: return "HIP runtime error";
@@ -68,6 +69,11 @@ int fail(int code, const char* where, const char* context) {
     // qpp::exception::Exception::what(), classes/exception.hpp:98-113
     g_last_error = std::string("[WHERE: ") + where + "] " + description(code) + "!";
     if (context) g_last_error += std::string(" [CONTEXT: ") + context + "]";
+    return code;
+}
+
+int fail_runtime(int code, const std::string& text) {
+    g_last_error = text;  // a std::runtime_error's what(), verbatim
     return code;
 }
 

@@ -43,6 +43,8 @@ namespace qs {
 // ---------------------------------------------------------------------------
 int fail(int code, const char* where, const char* context = nullptr);
 int fail_hip(hipError_t e, const char* where);
+// an error whose message is a std::runtime_error text (no WHERE/CONTEXT form)
+int fail_runtime(int code, const std::string& text);
 void clear_error();
 const char* description(int code);
 

@@ -5,8 +5,12 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <stdexcept>
+#include <string>
 #include <random>
 #include <vector>
+
+#include <unistd.h>
 
 #include "../../include/qpp_amd.hpp"
 #include "../../oracle/oracle.h"
@@ -182,6 +186,28 @@ int main() {
             threw = true;
         }
         CHECK(threw, "DimsMismatchMatrix");
+    }
+    {  // qpp::save / load and obsolete::save / load through device states
+        const std::string base = "/tmp/qpp_amd_dropin_" + std::to_string(::getpid());
+        bool same = true;
+        for (qs_format fmt : {QS_FMT_TEXT, QS_FMT_BINARY}) {
+            const std::string path = base + (fmt == QS_FMT_TEXT ? ".txt" : ".dat");
+            State(rho, dims).save(path, fmt);
+            same = same && maxdiff(State::load(path, dims, QS_RHO, QS_C128, fmt).download(),
+                                   rho.v) == 0.0;
+            State(psi, dims).save(path, fmt);
+            same = same && maxdiff(State::load(path, dims, QS_KET, QS_C128, fmt).download(),
+                                   psi.v) == 0.0;
+            std::remove(path.c_str());
+        }
+        CHECK(same, "save/load round trips (text, binary)");
+        bool threw = false;
+        try {
+            State::load(base + ".missing", dims);
+        } catch (const std::runtime_error& e) {
+            threw = std::string(e.what()) == "qpp::load(): Error opening input stream!";
+        }
+        CHECK(threw, "load of a missing file throws qpp's runtime_error");
     }
     {  // QCircuit + QEngine against the oracle engine
         QCircuit qc(5, 5);
