@@ -241,6 +241,14 @@ int qs_measure_seq(qs_state* st, uint32_t ntarget, const uint64_t* target,
                    int destructive, double u, uint64_t* results,
                    double* probs);
 
+/* qpp::reset(A, target, dims)  instruments.hpp:1148-1196, in place: a
+ * non-destructive measure_seq of the targets with the caller uniform u,
+ * then Xd(dims[i])^(dims[i] - result[i]) on target[i].  As in qpp the
+ * correction's dimension is dims[i] (the i-th subsystem's), not
+ * dims[target[i]]: where they differ qpp's apply() throws
+ * MatrixMismatchSubsys, which is reported here before anything runs. */
+int qs_reset(qs_state* st, uint32_t ntarget, const uint64_t* target, double u);
+
 /* ------------------------------------------------------------------------ */
 /* Circuits and engines                                                      */
 /* ------------------------------------------------------------------------ */
@@ -254,7 +262,15 @@ typedef enum qs_step_type {
     QS_STEP_CTRL = 3,        /* CTRL: applyCTRL(U, ctrl, target, shift) */
     QS_STEP_CTRL_FAN = 4,    /* CTRL_FAN: applyCTRL_fan */
     QS_STEP_MEASURE = 5,     /* MEASURE / MEASURE_ND (flag) on target[0] */
-    QS_STEP_MEASURE_MANY = 6 /* MEASURE_MANY / _ND on all targets */
+    QS_STEP_MEASURE_MANY = 6, /* MEASURE_MANY / _ND on all targets */
+    QS_STEP_RESET = 7,        /* RESET / RESET_MANY: reset(state, target, d),
+                                 qengine.hpp:804-808 */
+    QS_STEP_DISCARD = 8,      /* DISCARD / DISCARD_MANY: destructive
+                                 measure_seq, results dropped, :810-821 */
+    QS_STEP_POST_SELECT = 9   /* POST_SELECT(_MANY) / _ND (flag): measure_seq
+                                 of all targets, written from c_reg; the run
+                                 goes on only if the results equal the
+                                 post-selection values, :650-733 */
 } qs_step_type;
 
 typedef struct qs_step {
@@ -267,7 +283,7 @@ typedef struct qs_step {
     uint32_t shift_off;   /* offset of the shifts, UINT32_MAX = no shift */
     uint32_t c_reg;       /* first classical dit written by a measurement */
     uint32_t destructive; /* measurement: 1 = destructive (MEASURE), 0 = _ND */
-    uint32_t reserved;
+    uint32_t ps_off;      /* POST_SELECT: offset of ntarget values in the pool */
 } qs_step;
 
 typedef struct qs_circuit {
@@ -353,6 +369,13 @@ int qs_engine_get_state(const qs_engine* e, qs_state** out);
 int qs_engine_set_state(qs_engine* e, const qs_state* st);
 /* QEngine::reset(): |0..0>, stats cleared. */
 int qs_engine_reset(qs_engine* e);
+/* QEngineT::set_ensure_post_selection / set_max_post_selection_reps
+ * (qengine.hpp:1177-1227): with ensure set, a failed POST_SELECT step is
+ * measured again (a new draw) up to max_reps attempts in all.  Defaults:
+ * ensure 0, max_reps UINT64_MAX (internal/classes/qengine_state.hpp:66-72). */
+int qs_engine_set_post_selection(qs_engine* e, int ensure, uint64_t max_reps);
+/* QEngineT::post_select_ok(): 0 after a run whose post-selection failed. */
+int qs_engine_post_select_ok(const qs_engine* e, int* ok);
 
 /* ------------------------------------------------------------------------ */
 /* Batched noisy trajectories (one persistent launch)                        */

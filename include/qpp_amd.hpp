@@ -365,6 +365,19 @@ inline std::tuple<std::vector<idx>, std::vector<double>, cmat> measure_seq(
     return {std::vector<idx>(res.begin(), res.begin() + t.size()),
             std::vector<double>(ps.begin(), ps.begin() + t.size()), s.download()};
 }
+// qpp::reset(A, target, dims) (instruments.hpp:1148-1196) with the uniform
+// its measure_seq draws, or drawn from `gen` as qpp's RandomDevices would
+inline cmat reset(const cmat& A, const std::vector<idx>& target, const std::vector<idx>& dims,
+                  double u) {
+    State s(A, dims);
+    const auto t = detail::u64(target);
+    check(qs_reset(s.get(), static_cast<uint32_t>(t.size()), t.data(), u));
+    return s.download();
+}
+inline cmat reset(const cmat& A, const std::vector<idx>& target, const std::vector<idx>& dims,
+                  std::mt19937& gen) {
+    return reset(A, target, dims, std::generate_canonical<double, 53>(gen));
+}
 
 // ---------------------------------------------------------------------------
 // QCircuit / QEngine (classes/qcircuit.hpp, classes/qengine.hpp)
@@ -399,6 +412,27 @@ class QCircuit {
     }
     QCircuit& measure(const std::vector<idx>& target, idx c_reg = 0, bool destructive = true) {
         return meas(QS_STEP_MEASURE_MANY, target, c_reg, destructive);
+    }
+    // post_select (qcircuit.hpp:3366-3500): measure, keep the run only
+    // when the results equal ps_vals (one value per target)
+    QCircuit& post_select(idx target, idx ps_val, idx c_reg, bool destructive = true) {
+        return post_select(std::vector<idx>{target}, std::vector<idx>{ps_val}, c_reg, destructive);
+    }
+    QCircuit& post_select(const std::vector<idx>& target, const std::vector<idx>& ps_vals,
+                          idx c_reg, bool destructive = true) {
+        if (ps_vals.size() != target.size())
+            throw std::invalid_argument("qpp_amd::QCircuit::post_select(): one value per target");
+        meas(QS_STEP_POST_SELECT, target, c_reg, destructive);
+        steps_.back().ps_off = static_cast<uint32_t>(pool_.size());
+        pool_.insert(pool_.end(), ps_vals.begin(), ps_vals.end());
+        return *this;
+    }
+    // reset / discard (qcircuit.hpp:3745-3940)
+    QCircuit& reset(idx target) { return meas(QS_STEP_RESET, {target}, 0, false); }
+    QCircuit& reset(const std::vector<idx>& target) { return meas(QS_STEP_RESET, target, 0, false); }
+    QCircuit& discard(idx target) { return meas(QS_STEP_DISCARD, {target}, 0, true); }
+    QCircuit& discard(const std::vector<idx>& target) {
+        return meas(QS_STEP_DISCARD, target, 0, true);
     }
     QCircuit& nop() {
         qs_step s{};
@@ -511,10 +545,32 @@ class QEngine {
     }
     QEngine& reset() {
         check(qs_engine_reset(e_));
+        ensure_ = false;
+        max_ps_ = UINT64_MAX;
         return *this;
+    }
+    // qengine.hpp:1177-1227
+    QEngine& set_ensure_post_selection(bool val) {
+        ensure_ = val;
+        check(qs_engine_set_post_selection(e_, ensure_ ? 1 : 0, max_ps_));
+        return *this;
+    }
+    QEngine& set_max_post_selection_reps(idx n) {
+        max_ps_ = n;
+        check(qs_engine_set_post_selection(e_, ensure_ ? 1 : 0, max_ps_));
+        return *this;
+    }
+    bool get_ensure_post_selection() const { return ensure_; }
+    idx get_max_post_selection_reps() const { return max_ps_; }
+    bool post_select_ok() const {
+        int ok = 0;
+        check(qs_engine_post_select_ok(e_, &ok));
+        return ok != 0;
     }
 
   private:
+    bool ensure_ = false;
+    idx max_ps_ = UINT64_MAX;
     idx nc_;
     qs_engine* e_ = nullptr;
 };

@@ -57,6 +57,10 @@ def lib():
         L.orc_rng_free.restype = None
         L.orc_engine_run.argtypes = [c_void_p, c_int, c_double, c_uint64, c_uint64, c_uint64,
                                      u64p, u64p, u64p, u64p, dp, dp, u64p]
+        L.orc_engine_run_ps.argtypes = [c_void_p, c_int, c_double, c_uint64, c_uint64, c_int,
+                                        c_uint64, c_uint64, u64p, u64p, u64p, u64p, dp, dp, u64p,
+                                        POINTER(c_int)]
+        L.orc_reset.argtypes = [dp, c_int, c_uint32, u64p, c_uint32, u64p, c_double, dp]
         _lib = L
     return _lib
 
@@ -229,9 +233,26 @@ def measure_seq(A, target, dims=2, destructive=True, u=0.5):
     return [int(x) for x in res[: len(t)]], [float(x) for x in pr[: len(t)]], st
 
 
-def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1 << 16):
+def reset(A, target, dims=2, u=0.5):
+    """qpp::reset(A, target, dims) with the uniform u."""
+    dims = _dims(A, dims)
+    s, sp, is_rho = _buf(A)
+    d, dp_ = _u64(dims)
+    t, tp = _u64(target)
+    out = np.zeros(len(s), dtype=np.complex128)
+    lib().orc_reset(sp, int(is_rho), len(d), dp_, len(t), tp, float(u),
+                    out.ctypes.data_as(POINTER(c_double)))
+    if is_rho:
+        D = int(round(np.sqrt(len(s))))
+        return out.reshape((D, D), order="F")
+    return out
+
+
+def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1 << 16,
+               ensure_post_selection=False, max_post_selection_reps=None, with_ps_ok=False):
     """QEngine / QNoisyEngine execute(reps) on a qpp_amd.QCircuit; returns
-    (stats dict, dits, probs, final state)."""
+    (stats dict, dits, probs, final state), plus post_select_ok() when
+    with_ps_ok."""
     cs = qc.struct()
     nc = qc.nc
     n_out = np.zeros(1, dtype=np.uint64)
@@ -242,12 +263,19 @@ def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1
     D = qc.d ** qc.nq
     st = np.zeros(D * D if is_rho else D, dtype=np.complex128)
     olen = np.zeros(1, dtype=np.uint64)
-    lib().orc_engine_run(ctypes.cast(ctypes.byref(cs), c_void_p), int(is_rho), float(noise_p),
-                         int(seed), int(reps), max_outcomes,
-                         n_out.ctypes.data_as(POINTER(c_uint64)), sd.ctypes.data_as(POINTER(c_uint64)),
-                         sc.ctypes.data_as(POINTER(c_uint64)), dits.ctypes.data_as(POINTER(c_uint64)),
-                         probs.ctypes.data_as(POINTER(c_double)), st.ctypes.data_as(POINTER(c_double)),
-                         olen.ctypes.data_as(POINTER(c_uint64)))
+    ok = c_int(1)
+    mx = (1 << 64) - 1 if max_post_selection_reps is None else int(max_post_selection_reps)
+    rc = lib().orc_engine_run_ps(ctypes.cast(ctypes.byref(cs), c_void_p), int(is_rho), float(noise_p),
+                            int(seed), int(reps), int(bool(ensure_post_selection)), mx,
+                            max_outcomes,
+                            n_out.ctypes.data_as(POINTER(c_uint64)), sd.ctypes.data_as(POINTER(c_uint64)),
+                            sc.ctypes.data_as(POINTER(c_uint64)), dits.ctypes.data_as(POINTER(c_uint64)),
+                            probs.ctypes.data_as(POINTER(c_double)), st.ctypes.data_as(POINTER(c_double)),
+                            olen.ctypes.data_as(POINTER(c_uint64)), ctypes.byref(ok))
+    if rc != 0:
+        err = RuntimeError(f"oracle engine stopped with qs_status {rc}")
+        err.code = rc
+        raise err
     n = int(n_out[0])
     stats = {tuple(int(x) for x in sd[i * nc:(i + 1) * nc]): int(sc[i]) for i in range(min(n, max_outcomes))}
     L_ = int(olen[0])
@@ -255,7 +283,8 @@ def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1
     if is_rho:
         Dn = int(round(np.sqrt(L_)))
         state = state.reshape((Dn, Dn), order="F")
-    return stats, [int(x) for x in dits[:nc]], [float(x) for x in probs[:nc]], state
+    out = (stats, [int(x) for x in dits[:nc]], [float(x) for x in probs[:nc]], state)
+    return out + (bool(ok.value),) if with_ps_ok else out
 
 
 class Rng:

@@ -21,6 +21,7 @@
 #include <optional>
 #include <random>
 #include <set>
+#include <stdexcept>
 #include <vector>
 
 #include "../include/qpp_amd.h"
@@ -947,6 +948,29 @@ std::vector<idx> vec_of(const uint64_t* p, uint32_t n) {
     return std::vector<idx>(p, p + n);
 }
 
+// Xd(D)^k (gates.hpp:360-373): |j> -> |j+k mod D>.  qpp builds Xd(D > 2)
+// numerically as Fd^-1 Zd Fd; this is its exact value (the two differ by
+// rounding, ~1e-16 per entry).
+Mat shift_pow(idx D, idx k) {
+    Mat M(D, D);
+    for (idx j = 0; j < D; ++j) M((j + k) % D, j) = 1;
+    return M;
+}
+
+// reset, instruments.hpp:1148-1196: measure_seq(A, target, dims, false),
+// then powm(Xd(dims[i]), dims[i] - result[i]) on target[i].  The
+// correction's dimension is dims[i], the i-th subsystem's (as in qpp);
+// callers pass dims where that equals dims[target[i]].
+cvec reset_impl(const cvec& A, bool is_rho, const std::vector<idx>& dims,
+                const std::vector<idx>& target, double u) {
+    const MeasureResult mr = measure_seq_impl(A, is_rho, dims, target, false, u);
+    cvec r = mr.state;
+    for (idx i = 0; i < target.size(); ++i)
+        r = apply_any(r, is_rho, shift_pow(dims[i], dims[i] - mr.results[i]),
+                      {target[i]}, dims);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Engines
 // ---------------------------------------------------------------------------
@@ -956,12 +980,24 @@ struct Step {
     std::optional<std::vector<idx>> shift;
     idx c_reg;
     bool destructive;
+    std::vector<idx> ps;  // POST_SELECT values, one per target
 };
 
+// is_measurement, classes/qcircuit.hpp:7008-7025: the projective
+// measurements and post-selections (reset and discard are not measurements)
 bool is_measurement(const Step& s) {
-    return s.type == QS_STEP_MEASURE || s.type == QS_STEP_MEASURE_MANY;
+    return s.type == QS_STEP_MEASURE || s.type == QS_STEP_MEASURE_MANY ||
+           s.type == QS_STEP_POST_SELECT;
 }
 
+bool is_reset_or_discard(const Step& s) {
+    return s.type == QS_STEP_RESET || s.type == QS_STEP_DISCARD;
+}
+
+// extract_ctrl_target_ps_vals_c_reg, classes/qcircuit.hpp:7371-7449: a
+// measurement gives its targets and c_reg..c_reg+k-1, a gate step its
+// controls and targets, and a reset or discard step nothing at all (it
+// is neither a measurement nor a gate step there).
 std::vector<idx> c_regs_of(const Step& s) {
     std::vector<idx> c;
     if (is_measurement(s)) {
@@ -969,6 +1005,10 @@ std::vector<idx> c_regs_of(const Step& s) {
         std::iota(c.begin(), c.end(), s.c_reg);
     }
     return c;
+}
+
+std::vector<idx> targets_of(const Step& s) {
+    return is_reset_or_discard(s) ? std::vector<idx>{} : s.target;
 }
 
 // can_swap, classes/qcircuit.hpp:7465-7492: controls, targets and classical
@@ -979,7 +1019,7 @@ bool can_swap(const Step& a, const Step& b) {
                x.end();
     };
     if (overlap(a.ctrl, b.ctrl)) return false;
-    if (overlap(a.target, b.target)) return false;
+    if (overlap(targets_of(a), targets_of(b))) return false;
     if (overlap(c_regs_of(a), c_regs_of(b))) return false;
     return true;
 }
@@ -1007,12 +1047,21 @@ std::vector<Step> canonical_form(const std::vector<Step>& steps) {
     return before;
 }
 
+// an engine error, as the qs_status code of the matching qpp exception
+struct EngineError {
+    int code;
+};
+
 struct EngineState {
     cvec qstate;
     std::vector<idx> dims;  // dims of the non-measured qudits
     std::vector<double> probs;
     std::vector<idx> dits;
     std::vector<idx> subsys;  // relative position or SIZE_MAX if measured
+    // internal/classes/qengine_state.hpp:66-72
+    bool ps_ok = true;
+    bool ensure_ps = false;
+    idx max_ps = std::numeric_limits<idx>::max();
 };
 
 struct Engine {
@@ -1045,11 +1094,19 @@ struct Engine {
         st.dits.assign(nc, 0);
         st.subsys.resize(nq);
         std::iota(st.subsys.begin(), st.subsys.end(), idx{0});
+        st.ps_ok = true;
+        st.ensure_ps = false;
+        st.max_ps = std::numeric_limits<idx>::max();
     }
 
+    // get_relative_pos_, qengine.hpp:514-526: QuditAlreadyMeasured for a
+    // destructively measured qudit
     std::vector<idx> rel(const std::vector<idx>& v) const {
         std::vector<idx> r = v;
-        for (auto& x : r) x = st.subsys[x];
+        for (auto& x : r) {
+            if (was_measured(x)) throw EngineError{QS_ERR_QUDIT_ALREADY_MEASURED};
+            x = st.subsys[x];
+        }
         return r;
     }
 
@@ -1089,21 +1146,63 @@ struct Engine {
         }
     }
 
-    // visit_measurement_step_, qengine.hpp:639-734 (MEASURE, MEASURE_MANY and
-    // their _ND forms)
+    // visit_measurement_step_, qengine.hpp:639-734 (MEASURE, MEASURE_MANY,
+    // POST_SELECT, POST_SELECT_MANY and their _ND forms).  A failed
+    // post-selection is measured again from the same state (a new draw)
+    // while ensure is set and fewer than max_ps attempts were made; out of
+    // attempts, the step returns leaving state and dits untouched; without
+    // ensure the failed outcome is kept and ps_ok cleared.
     void measure_step(const Step& s) {
         const auto trel = rel(s.target);
+        idx attempts = 0;
+        for (;;) {
+            const double u = draw_for_sample();
+            MeasureResult mr = measure_seq_impl(st.qstate, is_rho, st.dims,
+                                                trel, s.destructive, u);
+            if (s.type == QS_STEP_POST_SELECT && mr.results != s.ps) {
+                ++attempts;
+                if (!st.ensure_ps) {
+                    st.ps_ok = false;
+                } else if (attempts < st.max_ps) {
+                    // unlimited retries of a result that never comes: give
+                    // up (qpp loops for ever) after 2^16 attempts
+                    if (st.max_ps == std::numeric_limits<idx>::max() &&
+                        attempts >= (idx{1} << 16))
+                        throw EngineError{QS_ERR_OUT_OF_RANGE};
+                    continue;
+                } else {
+                    st.ps_ok = false;
+                    return;
+                }
+            }
+            st.qstate = mr.state;
+            st.dims = mr.dims;
+            for (idx i = 0; i < mr.results.size(); ++i) {
+                st.dits[s.c_reg + i] = mr.results[i];
+                st.probs[s.c_reg + i] = mr.probs[i];
+            }
+            if (s.destructive)
+                for (idx t : s.target) set_measured(t);
+            return;
+        }
+    }
+
+    // RESET / RESET_MANY, qengine.hpp:804-808: qpp::reset on the relative
+    // positions with the engine's uniform dimension
+    void reset_step(const Step& s) {
         const double u = draw_for_sample();
-        MeasureResult mr = measure_seq_impl(st.qstate, is_rho, st.dims, trel,
-                                            s.destructive, u);
+        st.qstate = reset_impl(st.qstate, is_rho, st.dims, rel(s.target), u);
+    }
+
+    // DISCARD / DISCARD_MANY, qengine.hpp:810-821: destructive measure_seq,
+    // results dropped
+    void discard_step(const Step& s) {
+        const double u = draw_for_sample();
+        MeasureResult mr = measure_seq_impl(st.qstate, is_rho, st.dims,
+                                            rel(s.target), true, u);
         st.qstate = mr.state;
         st.dims = mr.dims;
-        for (idx i = 0; i < mr.results.size(); ++i) {
-            st.dits[s.c_reg + i] = mr.results[i];
-            st.probs[s.c_reg + i] = mr.probs[i];
-        }
-        if (s.destructive)
-            for (idx t : s.target) set_measured(t);
+        for (idx t : s.target) set_measured(t);
     }
 
     // A sample over >= 2 basis states consumes one generate_canonical draw;
@@ -1152,46 +1251,82 @@ struct Engine {
         }
         if (is_measurement(s))
             measure_step(s);
+        else if (s.type == QS_STEP_RESET)
+            reset_step(s);
+        else if (s.type == QS_STEP_DISCARD)
+            discard_step(s);
         else if (s.type != QS_STEP_NOP)
             gate_step(s);
     }
 
-    // execute_steps_once_, qengine.hpp:240-294 (no runtime steps)
-    void execute_steps_once(const std::vector<Step>& steps_, idx pos) {
+    // execute_steps_once_, qengine.hpp:240-294 (no runtime steps): stops
+    // after the step whose post-selection failed; statistics only for a
+    // run that measured and got to the end
+    void execute_steps_once(const std::vector<Step>& steps_, idx pos,
+                            bool ensure) {
+        const bool saved_ensure = st.ensure_ps;
+        if (ensure) st.ensure_ps = true;
         bool measured = false;
         for (idx i = pos; i < steps_.size(); ++i) {
             if (is_measurement(steps_[i])) measured = true;
             execute_step(steps_[i]);
+            if (!st.ps_ok) {
+                if (ensure) st.ensure_ps = saved_ensure;
+                return;
+            }
         }
         if (measured) ++stats[st.dits];
+        if (ensure) st.ensure_ps = saved_ensure;
     }
 
-    // execute_steps_with_postselection_retries_, qengine.hpp:316-345 (no
-    // post-selection steps: every rep succeeds; the first rep's state is kept)
+    // execute_steps_with_postselection_retries_, qengine.hpp:316-345: every
+    // rep starts from the saved state; the engine ends in the first rep
+    // whose post-selection held, else in the last rep's state
     void execute_with_retries(const std::vector<Step>& steps_, idx pos,
                               idx reps) {
         EngineState copy = st;
-        EngineState first;
+        EngineState ok;
+        bool found = false;
         for (idx rep = 0; rep < reps; ++rep) {
-            execute_steps_once(steps_, pos);
-            if (rep == 0) first = st;
+            execute_steps_once(steps_, pos, st.ensure_ps);
+            if (st.ps_ok && !found) {
+                found = true;
+                ok = st;
+            }
             if (rep + 1 < reps) st = copy;
         }
-        st = first;
+        if (found) st = ok;
     }
 
     // execute_measurement_steps_with_sampling_and_postselection_retries_,
-    // qengine.hpp:374-481 (projective measurements only, no post-selection)
+    // qengine.hpp:374-481, with build_sampling_maps_ (:150-213).  Every
+    // step from pos on is a projective measurement or post-selection.  Each
+    // attempt samples the saved pre-measurement state (one draw); a sample
+    // whose post-selected qudits differ from their values fails the rep (or,
+    // with ensure, is drawn again up to max_ps attempts).  The first
+    // successful rep runs the steps once (ensure on, no attempt limit) and
+    // fixes the engine's final state; later successes only add statistics.
     void execute_sampling(const std::vector<Step>& steps_, idx pos, idx reps) {
-        std::map<idx, idx> c_q;  // classical dit <- qudit
+        std::map<idx, idx> c_q;       // classical dit <- qudit
+        std::map<idx, idx> q_ps_val;  // post-selected qudit -> value
         bool measured = false;
-        for (idx i = pos; i < steps_.size(); ++i)
-            if (is_measurement(steps_[i])) {
-                measured = true;
-                const auto cr = c_regs_of(steps_[i]);
-                for (idx q = 0; q < steps_[i].target.size(); ++q)
-                    c_q[cr[q]] = steps_[i].target[q];
+        for (idx i = pos; i < steps_.size(); ++i) {
+            if (!is_measurement(steps_[i])) continue;
+            measured = true;
+            const auto cr = c_regs_of(steps_[i]);
+            for (idx q = 0; q < steps_[i].target.size(); ++q) {
+                const idx t = steps_[i].target[q];
+                c_q[cr[q]] = t;
+                if (steps_[i].type != QS_STEP_POST_SELECT) continue;
+                auto it = q_ps_val.find(t);
+                if (it == q_ps_val.end()) {
+                    q_ps_val[t] = steps_[i].ps[q];
+                } else if (it->second != steps_[i].ps[q]) {
+                    st.ps_ok = false;  // incompatible post-selections
+                    return;
+                }
             }
+        }
         if (!measured) return;
         std::set<idx> sset;
         for (auto& kv : c_q) sset.insert(kv.second);
@@ -1200,16 +1335,36 @@ struct Engine {
         EngineState ok = st;
         bool found = false;
         for (idx rep = 0; rep < reps; ++rep) {
-            // sample(engine_state_copy.qstate_, sample_from, d)
-            const idx D = prod(copy.dims);
-            const double u = D >= 2 ? canonical() : 0.0;
-            const SampleResult s =
-                sample_one(copy.qstate, is_rho, copy.dims, sample_from, u);
+            idx attempts = 0;
+            SampleResult s;
+            bool failed;
+            do {
+                ++attempts;
+                // sample(engine_state_copy.qstate_, sample_from, d)
+                const idx D = prod(copy.dims);
+                const double u = D >= 2 ? canonical() : 0.0;
+                s = sample_one(copy.qstate, is_rho, copy.dims, sample_from, u);
+                failed = false;
+                for (idx q = 0; q < sample_from.size(); ++q) {
+                    auto it = q_ps_val.find(sample_from[q]);
+                    if (it != q_ps_val.end() && s.digits[q] != it->second) {
+                        failed = true;
+                        break;
+                    }
+                }
+                if (failed) st.ps_ok = false;
+            } while (failed && st.ensure_ps && attempts < st.max_ps);
+            if (failed) continue;
+            st.ps_ok = true;
             if (!found) {
                 found = true;
-                execute_steps_once(steps_, pos);
+                st.max_ps = std::numeric_limits<idx>::max();
+                execute_steps_once(steps_, pos, true);
+                st.max_ps = copy.max_ps;
                 ok = st;
             } else {
+                // the sampled digits (ascending qudit order) fill the
+                // classical dits in ascending c_reg order
                 std::vector<idx> res = st.dits;
                 idx i = 0;
                 for (auto& kv : c_q) res[kv.first] = s.digits[i++];
@@ -1226,10 +1381,21 @@ struct Engine {
             execute_with_retries(steps, 0, reps);
             return;
         }
-        reset_state(st.qstate);  // keep the statistics
+        // reset(qstate, false): keep state and statistics; post_select_ok
+        // back to true, ensure and max_ps kept
+        const bool ensure = st.ensure_ps;
+        const idx max_ps = st.max_ps;
+        reset_state(st.qstate);
+        st.ensure_ps = ensure;
+        st.max_ps = max_ps;
         const std::vector<Step> s = reps > 1 ? canonical_form(steps) : steps;
         if (s.empty()) return;
-        idx pos = std::find_if(s.begin(), s.end(), is_measurement) - s.begin();
+        // compute_optimize_up_to_pos_ (:87-101) and can_sample_from_
+        // (:122-143): the first measurement / reset / discard step, and
+        // whether only projective measurements follow it
+        const idx pos = std::find_if(s.begin(), s.end(), [](const Step& x) {
+                            return is_measurement(x) || is_reset_or_discard(x);
+                        }) - s.begin();
         bool can_sample = true;
         for (idx i = pos; i < s.size(); ++i)
             if (!is_measurement(s[i])) can_sample = false;
@@ -1406,6 +1572,17 @@ int orc_measure_seq(const double* A, int is_rho, uint32_t ndims,
     return 0;
 }
 
+int orc_reset(const double* A, int is_rho, uint32_t ndims, const uint64_t* dims,
+              uint32_t ntarget, const uint64_t* target, double u,
+              double* out) {
+    const auto dv = vec_of(dims, ndims);
+    const idx D = prod(dv);
+    store(reset_impl(load(A, is_rho ? D * D : D), is_rho, dv,
+                     vec_of(target, ntarget), u),
+          out);
+    return 0;
+}
+
 void* orc_rng_new(uint64_t seed) {
     return new std::mt19937(static_cast<std::mt19937::result_type>(seed));
 }
@@ -1445,6 +1622,18 @@ int orc_engine_run(const qs_circuit* qc, int is_rho, double noise_p,
                    uint64_t* n_outcomes, uint64_t* stats_dits,
                    uint64_t* stats_counts, uint64_t* dits, double* probs,
                    double* out_state, uint64_t* out_len) {
+    return orc_engine_run_ps(qc, is_rho, noise_p, seed, reps, 0,
+                             UINT64_MAX, max_outcomes, n_outcomes, stats_dits,
+                             stats_counts, dits, probs, out_state, out_len,
+                             nullptr);
+}
+
+int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
+                      uint64_t seed, uint64_t reps, int ensure_ps,
+                      uint64_t max_ps, uint64_t max_outcomes,
+                      uint64_t* n_outcomes, uint64_t* stats_dits,
+                      uint64_t* stats_counts, uint64_t* dits, double* probs,
+                      double* out_state, uint64_t* out_len, int* ps_ok) {
     Engine e;
     e.nq = qc->nq;
     e.nc = qc->nc;
@@ -1467,10 +1656,19 @@ int orc_engine_run(const qs_circuit* qc, int is_rho, double noise_p,
                                        qc->pool + s.shift_off + s.nctrl);
         t.c_reg = s.c_reg;
         t.destructive = s.destructive != 0;
+        if (s.type == QS_STEP_POST_SELECT)
+            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.ntarget);
         e.steps.push_back(t);
     }
     e.reset_state(std::nullopt);
-    e.execute(reps);
+    e.st.ensure_ps = ensure_ps != 0;
+    e.st.max_ps = max_ps;
+    try {
+        e.execute(reps);
+    } catch (const EngineError& err) {
+        return err.code;
+    }
+    if (ps_ok) *ps_ok = e.st.ps_ok ? 1 : 0;
     idx k = 0;
     for (auto& kv : e.stats) {
         if (k < max_outcomes) {

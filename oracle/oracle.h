@@ -91,6 +91,11 @@ int orc_measure_seq(const double* A, int is_rho, uint32_t ndims,
                     uint64_t* results, double* probs, double* out_state,
                     uint64_t* out_len);
 
+/* reset(A, target, dims) with the given uniform: instruments.hpp:1148-1196.
+ * out: same size as A. */
+int orc_reset(const double* A, int is_rho, uint32_t ndims, const uint64_t* dims,
+              uint32_t ntarget, const uint64_t* target, double u, double* out);
+
 /* std::mt19937 + std::generate_canonical<double, 53> (the draws qpp's
  * RandomDevices feeds to every distribution; random.h:164-192). */
 void* orc_rng_new(uint64_t seed);
@@ -120,6 +125,20 @@ int orc_engine_run(const struct qs_circuit* qc, int is_rho, double noise_p,
                    uint64_t* n_outcomes, uint64_t* stats_dits,
                    uint64_t* stats_counts, uint64_t* dits, double* probs,
                    double* out_state, uint64_t* out_len);
+/* The same with set_ensure_post_selection(ensure_ps) and
+ * set_max_post_selection_reps(max_ps) (qengine.hpp:1177-1227) before the
+ * run; *ps_ok (may be NULL) receives post_select_ok() after it.  Reset,
+ * discard and post-selection steps: qengine.hpp:639-823.  Returns 0, or the
+ * qs_status code of the error the run stopped at: QS_ERR_QUDIT_ALREADY_MEASURED
+ * for a step on a destructively measured qudit (qengine.hpp:514-526),
+ * QS_ERR_OUT_OF_RANGE for a post-selection retried without limit that
+ * failed 2^16 times. */
+int orc_engine_run_ps(const struct qs_circuit* qc, int is_rho, double noise_p,
+                      uint64_t seed, uint64_t reps, int ensure_ps,
+                      uint64_t max_ps, uint64_t max_outcomes,
+                      uint64_t* n_outcomes, uint64_t* stats_dits,
+                      uint64_t* stats_counts, uint64_t* dits, double* probs,
+                      double* out_state, uint64_t* out_len, int* ps_ok);
 
 #ifdef __cplusplus
 }

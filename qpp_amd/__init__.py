@@ -133,6 +133,7 @@ def lib():
             "qs_probs": (c_int, [vp, dp, c_uint64]),
             "qs_sample": (c_int, [vp, c_uint32, u64p, c_uint64, dp, u64p, u64p, dp]),
             "qs_measure_seq": (c_int, [vp, c_uint32, u64p, c_int, c_double, u64p, dp]),
+            "qs_reset": (c_int, [vp, c_uint32, u64p, c_double]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -402,6 +403,12 @@ class State:
                                     pr.ctypes.data_as(POINTER(c_double))))
         return [int(x) for x in res[: len(t)]], [float(x) for x in pr[: len(t)]]
 
+    def reset(self, target, u):
+        """qpp::reset in place (instruments.hpp:1148-1196) with uniform u."""
+        t, tp = _u64(target)
+        _check(lib().qs_reset(self._h, len(t), tp, float(u)))
+        return self
+
 
 # ---------------------------------------------------------------------------
 # qpp-style value-semantics functions
@@ -518,6 +525,15 @@ def load(path, dims, kind=KET, fmt=TEXT):
     return State.load(path, dims, kind, C128, fmt).download()
 
 
+def reset(A, target, dims=2, u=0.5):
+    """qpp::reset(A, target, dims) (instruments.hpp:1148-1196) -> the reset
+    state; u is the uniform its measure_seq draws."""
+    dims = _dims_of(A, dims)
+    st = _as_state(A, dims)
+    st.reset(target, u)
+    return st.download()
+
+
 def measure_seq(A, target, dims=2, destructive=True, u=None):
     """qpp::measure_seq -> (results, probs, post-measurement state)."""
     dims = _dims_of(A, dims)
@@ -570,13 +586,14 @@ class Rng:
 # ---------------------------------------------------------------------------
 # circuits (qpp::QCircuit lowered to the qs_circuit program format)
 # ---------------------------------------------------------------------------
-STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY = range(7)
+(STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY,
+ STEP_RESET, STEP_DISCARD, STEP_POST_SELECT) = range(10)
 
 
 class _Step(ctypes.Structure):
     _fields_ = [(f, c_uint32) for f in ("type", "gate", "nctrl", "ntarget", "ctrl_off",
                                          "target_off", "shift_off", "c_reg", "destructive",
-                                         "reserved")]
+                                         "ps_off")]
 
 
 class _Circuit(ctypes.Structure):
@@ -588,11 +605,13 @@ class _Circuit(ctypes.Structure):
 
 class QCircuit:
     """Step list of a qpp::QCircuit (classes/qcircuit.hpp): gate / gate_fan /
-    gate_joint / CTRL / CTRL_fan / measure / nop, in program order."""
+    gate_joint / CTRL / CTRL_fan / measure / post_select / reset / discard /
+    nop, in program order."""
 
     def __init__(self, nq, nc=0, d=2):
         self.nq, self.nc, self.d = int(nq), int(nc), int(d)
         self.steps = []       # tuples (type, gate, ctrl, target, shift, c_reg, destructive)
+        self.ps_vals = {}     # step index -> post-selection values
         self.mats = []
         self._mat_index = {}
 
@@ -638,6 +657,32 @@ class QCircuit:
         self.steps.append((kind, 0, [], t, None, c, int(bool(destructive))))
         return self
 
+    def post_select(self, target, ps_vals, c_reg=None, destructive=True):
+        """QCircuit::post_select (qcircuit.hpp:3366-3500): measure the
+        targets into c_reg.. and keep the run only when the results equal
+        ps_vals (one value per target)."""
+        t = [int(target)] if np.isscalar(target) else list(target)
+        v = [int(ps_vals)] if np.isscalar(ps_vals) else list(ps_vals)
+        if len(v) != len(t):
+            raise ValueError("post_select: one value per target")
+        c = t[0] if c_reg is None else int(c_reg)
+        self.ps_vals[len(self.steps)] = v
+        self.steps.append((STEP_POST_SELECT, 0, [], t, None, c, int(bool(destructive))))
+        return self
+
+    def reset(self, target):
+        """QCircuit::reset (qcircuit.hpp:3862-3940): the targets back to |0>."""
+        t = [int(target)] if np.isscalar(target) else list(target)
+        self.steps.append((STEP_RESET, 0, [], t, None, 0, 0))
+        return self
+
+    def discard(self, target):
+        """QCircuit::discard (qcircuit.hpp:3745-3830): measure the targets
+        destructively and drop the results."""
+        t = [int(target)] if np.isscalar(target) else list(target)
+        self.steps.append((STEP_DISCARD, 0, [], t, None, 0, 1))
+        return self
+
     def nop(self):
         self.steps.append((STEP_NOP, 0, [], [], None, 0, 0))
         return self
@@ -669,7 +714,10 @@ class QCircuit:
             else:
                 s.shift_off = len(pool)
                 pool += shift
-            s.c_reg, s.destructive, s.reserved = creg, dest, 0
+            s.c_reg, s.destructive, s.ps_off = creg, dest, 0
+            if i in self.ps_vals:
+                s.ps_off = len(pool)
+                pool += self.ps_vals[i]
         pool_a = np.ascontiguousarray(np.asarray(pool + [0], dtype=np.uint64))
         dims = np.ascontiguousarray(np.asarray([m.shape[0] for m in self.mats] + [0], dtype=np.uint64))
         offs, off = [], 0
@@ -893,7 +941,32 @@ class QEngine:
 
     def reset(self):
         _check(lib().qs_engine_reset(self._h))
+        self._ensure, self._max_ps = False, (1 << 64) - 1
         return self
+
+    def set_ensure_post_selection(self, val):
+        """QEngineT::set_ensure_post_selection (qengine.hpp:1177-1181)."""
+        self._ensure = bool(val)
+        return self._set_ps()
+
+    def set_max_post_selection_reps(self, n):
+        """QEngineT::set_max_post_selection_reps (qengine.hpp:1222-1226)."""
+        self._max_ps = int(n)
+        return self._set_ps()
+
+    def _set_ps(self):
+        L = lib()
+        L.qs_engine_set_post_selection.argtypes = [c_void_p, c_int, c_uint64]
+        _check(L.qs_engine_set_post_selection(self._h, int(getattr(self, "_ensure", False)),
+                                              getattr(self, "_max_ps", (1 << 64) - 1)))
+        return self
+
+    def post_select_ok(self):
+        L = lib()
+        L.qs_engine_post_select_ok.argtypes = [c_void_p, POINTER(c_int)]
+        ok = c_int()
+        _check(L.qs_engine_post_select_ok(self._h, byref(ok)))
+        return bool(ok.value)
 
     def __del__(self):
         try:

@@ -13,6 +13,10 @@
 //   destructive measure  qengine.hpp:489-503, 514-526 (relative positions)
 //   noise                qnoisy_engine.hpp:104-140, noise.hpp:131-160
 //                        (before every step, one draw per live qubit)
+//   reset / discard      qengine.hpp:804-821 (instruments.hpp:1148-1196)
+//   post-selection       qengine.hpp:650-733 (re-measured while ensured),
+//                        240-294 (a failed run stops), 374-481 (sampling
+//                        with post-selected qudits)
 // Every uniform comes from the engine's std::mt19937 through
 // std::generate_canonical<double, 53>, in qpp's order.  Runs of gate steps
 // between measurements are compiled into tile-fused programs (cached) when
@@ -40,6 +44,14 @@ namespace {
 
 using qs::fail;
 constexpr uint64_t kMeasured = std::numeric_limits<uint64_t>::max();
+// Retrying a post-selection without limit (qpp's default, and the first
+// successful rep of the sampling tail) never ends when the selected result
+// has probability 0, e.g. after a measurement that contradicts it; qpp
+// then loops for ever.  Here such a step gives up with an error after
+// kPsGiveUp attempts (a result of probability p fails that often with
+// probability (1-p)^65536, below 1e-28 for p >= 1e-3).
+constexpr uint64_t kUnlimited = std::numeric_limits<uint64_t>::max();
+constexpr uint64_t kPsGiveUp = uint64_t{1} << 16;
 
 struct EStep {
     uint32_t type = 0, gate = 0;
@@ -47,10 +59,18 @@ struct EStep {
     bool has_shift = false;
     uint64_t c_reg = 0;
     bool destructive = true;
+    std::vector<uint64_t> ps;  // POST_SELECT values, one per target
 };
 
+// is_measurement, classes/qcircuit.hpp:7008-7025: measurements and
+// post-selections; reset and discard are not measurements
 bool is_meas(const EStep& s) {
-    return s.type == QS_STEP_MEASURE || s.type == QS_STEP_MEASURE_MANY;
+    return s.type == QS_STEP_MEASURE || s.type == QS_STEP_MEASURE_MANY ||
+           s.type == QS_STEP_POST_SELECT;
+}
+
+bool is_reset_or_discard(const EStep& s) {
+    return s.type == QS_STEP_RESET || s.type == QS_STEP_DISCARD;
 }
 
 std::vector<uint64_t> c_regs_of(const EStep& s) {
@@ -62,13 +82,17 @@ std::vector<uint64_t> c_regs_of(const EStep& s) {
     return c;
 }
 
-// can_swap, classes/qcircuit.hpp:7465-7492
+// can_swap, classes/qcircuit.hpp:7465-7492, on the lists
+// extract_ctrl_target_ps_vals_c_reg (:7371-7449) gives: a reset or discard
+// step yields no controls, targets or dits, so it swaps with anything
 bool can_swap(const EStep& a, const EStep& b) {
     auto overlap = [](const std::vector<uint64_t>& x, const std::vector<uint64_t>& y) {
         return std::find_first_of(x.begin(), x.end(), y.begin(), y.end()) != x.end();
     };
-    return !overlap(a.ctrl, b.ctrl) && !overlap(a.target, b.target) &&
-           !overlap(c_regs_of(a), c_regs_of(b));
+    const std::vector<uint64_t> none;
+    const auto& ta = is_reset_or_discard(a) ? none : a.target;
+    const auto& tb = is_reset_or_discard(b) ? none : b.target;
+    return !overlap(a.ctrl, b.ctrl) && !overlap(ta, tb) && !overlap(c_regs_of(a), c_regs_of(b));
 }
 
 // canonical_form, classes/qcircuit.hpp:7556-7612
@@ -118,10 +142,28 @@ struct qs_engine {
     std::vector<double> probs;
     std::map<std::vector<uint64_t>, uint64_t> stats;
     std::map<std::string, qs_program*> progs;
+    // internal/classes/qengine_state.hpp:66-72
+    bool ps_ok = true, ensure_ps = false;
+    uint64_t max_ps = std::numeric_limits<uint64_t>::max();
 
     double canonical() { return std::generate_canonical<double, 53>(gen); }
 
     bool was_measured(uint64_t i) const { return subsys[i] == kMeasured; }
+
+    // get_relative_pos_, qengine.hpp:514-526: a step on a destructively
+    // measured qudit is QuditAlreadyMeasured
+    bool on_measured(const EStep& x) const {
+        for (const auto* v : {&x.target, &x.ctrl})
+            for (uint64_t q : *v)
+                if (was_measured(q)) return true;
+        return false;
+    }
+
+    // the steps before the failing one run first, as in qpp
+    int already_measured(const std::vector<GateApp>& pending) {
+        run_gates(pending, false);
+        return fail(QS_ERR_QUDIT_ALREADY_MEASURED, "qpp::QEngineT::get_relative_pos_()", "v[i]");
+    }
 
     std::vector<uint64_t> rel(const std::vector<uint64_t>& v) const {
         std::vector<uint64_t> r = v;
@@ -302,6 +344,61 @@ struct qs_engine {
         return QS_OK;
     }
 
+    // POST_SELECT(_MANY)(_ND), qengine.hpp:650-733.  Each attempt draws and
+    // samples the basis index measure_seq would collapse to; a result that
+    // differs from the values is drawn again while ensure_ps is set and
+    // fewer than max_ps attempts were made.  Out of attempts the step
+    // leaves state and dits as they were; without ensure_ps the failed
+    // result is measured and kept.  Either failure clears ps_ok.
+    int post_select(const EStep& s) {
+        const std::vector<uint64_t> t = rel(s.target);
+        std::vector<uint64_t> dig(t.size());
+        uint64_t attempts = 0;
+        for (;;) {
+            double u = live_dim() >= 2 ? canonical() : 0.0;
+            uint64_t index = 0;
+            if (int rc = qs_sample(st, uint32_t(t.size()), t.data(), 1, &u, &index, dig.data(),
+                                   nullptr))
+                return rc;
+            if (dig != s.ps) {
+                ++attempts;
+                if (!ensure_ps) {
+                    ps_ok = false;
+                } else if (attempts < max_ps) {
+                    if (max_ps == kUnlimited && attempts >= kPsGiveUp)
+                        return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
+                                    "post-selection failed 65536 times with unlimited retries");
+                    continue;
+                } else {
+                    ps_ok = false;
+                    return QS_OK;
+                }
+            }
+            return measure(s, 0.0, &index);
+        }
+    }
+
+    // RESET / RESET_MANY, qengine.hpp:804-808: qpp::reset on the relative
+    // positions, one draw for its measure_seq
+    int reset_step(const EStep& s) {
+        const std::vector<uint64_t> t = rel(s.target);
+        const double u = live_dim() >= 2 ? canonical() : 0.0;
+        return qs_reset(st, uint32_t(t.size()), t.data(), u);
+    }
+
+    // DISCARD / DISCARD_MANY, qengine.hpp:810-821: destructive measure_seq
+    // with the results dropped
+    int discard_step(const EStep& s) {
+        const std::vector<uint64_t> t = rel(s.target);
+        std::vector<uint64_t> res(t.size());
+        std::vector<double> ps(t.size());
+        const double u = live_dim() >= 2 ? canonical() : 0.0;
+        if (int rc = qs_measure_seq(st, uint32_t(t.size()), t.data(), 1, u, res.data(), ps.data()))
+            return rc;
+        for (uint64_t q : s.target) set_measured(q);
+        return QS_OK;
+    }
+
     // noise before a step (qnoisy_engine.hpp:108-116): one draw per live
     // qubit in ascending order, k = lower_bound(cp{1-p, p/3, p/3, p/3}, u)
     void noise_apps(std::vector<GateApp>& pending) {
@@ -324,36 +421,64 @@ struct qs_engine {
         }
     }
 
-    // executes steps [pos, end) once; noise before every step when noisy
-    int execute_steps_once(const std::vector<EStep>& s, size_t pos, bool& measured) {
-        measured = false;
+    // execute_steps_once_, qengine.hpp:240-294: steps [pos, end) once,
+    // noise before every step when noisy; stops after a step whose
+    // post-selection failed; a run that measured and got to the end adds
+    // its dits to the statistics.  `ensure` forces ensure_ps on meanwhile.
+    int execute_steps_once(const std::vector<EStep>& s, size_t pos, bool ensure) {
+        const bool saved = ensure_ps;
+        if (ensure) ensure_ps = true;
+        const int rc = steps_once(s, pos);
+        ensure_ps = saved;
+        return rc;
+    }
+
+    int steps_once(const std::vector<EStep>& s, size_t pos) {
+        bool measured = false;
         std::vector<GateApp> pending;
         bool cacheable = noise_p < 0;
         for (size_t i = pos; i < s.size(); ++i) {
             if (noise_p >= 0) noise_apps(pending);
-            if (is_meas(s[i])) {
-                measured = true;
+            const EStep& x = s[i];
+            if (on_measured(x)) return already_measured(pending);
+            if (is_meas(x) || is_reset_or_discard(x)) {
                 if (int rc = run_gates(pending, cacheable && !pending.empty())) return rc;
                 pending.clear();
-                const double u = live_dim() >= 2 ? canonical() : 0.0;
-                if (int rc = measure(s[i], u)) return rc;
-            } else if (s[i].type != QS_STEP_NOP) {
-                pending.push_back(app_of(s[i]));
+                int rc = QS_OK;
+                if (x.type == QS_STEP_POST_SELECT) {
+                    rc = post_select(x);
+                } else if (x.type == QS_STEP_RESET) {
+                    rc = reset_step(x);
+                } else if (x.type == QS_STEP_DISCARD) {
+                    rc = discard_step(x);
+                } else {
+                    const double u = live_dim() >= 2 ? canonical() : 0.0;
+                    rc = measure(x, u);
+                }
+                if (rc) return rc;
+                if (is_meas(x)) measured = true;
+            } else if (x.type != QS_STEP_NOP) {
+                pending.push_back(app_of(x));
             }
+            if (!ps_ok) return run_gates(pending, false);
         }
-        return run_gates(pending, cacheable);
+        if (int rc = run_gates(pending, cacheable)) return rc;
+        if (measured) ++stats[dits];
+        return QS_OK;
     }
 
     struct Snap {
         qs_state* st = nullptr;
         std::vector<uint64_t> dits, subsys;
         std::vector<double> probs;
+        bool ps_ok = true;
     };
 
     int snapshot(Snap& sn) const {
         sn.dits = dits;
         sn.subsys = subsys;
         sn.probs = probs;
+        sn.ps_ok = ps_ok;
         return qs_state_clone(st, &sn.st);
     }
 
@@ -368,40 +493,64 @@ struct qs_engine {
         dits = sn.dits;
         subsys = sn.subsys;
         probs = sn.probs;
+        ps_ok = sn.ps_ok;
     }
 
-    // execute_steps_with_postselection_retries_, qengine.hpp:316-345
+    // execute_steps_with_postselection_retries_, qengine.hpp:316-345: every
+    // rep starts from the saved engine state; the engine ends in the first
+    // rep whose post-selection held, else in the last rep's state
     int execute_with_retries(const std::vector<EStep>& s, size_t pos, uint64_t reps) {
-        Snap copy, first;
-        if (int rc = snapshot(copy)) return rc;
-        for (uint64_t rep = 0; rep < reps; ++rep) {
-            bool measured = false;
-            if (int rc = execute_steps_once(s, pos, measured)) return rc;
-            if (measured) ++stats[dits];
-            if (rep == 0 && reps > 1)
-                if (int rc = snapshot(first)) return rc;
-            if (rep + 1 < reps) restore(copy, false);
+        Snap copy, ok;
+        if (reps > 1)
+            if (int rc = snapshot(copy)) return rc;
+        bool found = false;
+        int rc = QS_OK;
+        for (uint64_t rep = 0; rep < reps && !rc; ++rep) {
+            rc = execute_steps_once(s, pos, ensure_ps);
+            if (!rc && ps_ok && !found) {
+                found = true;
+                if (rep + 1 < reps) rc = snapshot(ok);
+            }
+            if (!rc && rep + 1 < reps) restore(copy, false);
         }
-        if (reps > 1) restore(first, true);
+        if (!rc && ok.st) restore(ok, true);
+        qs_state_destroy(ok.st);
         qs_state_destroy(copy.st);
-        return QS_OK;
+        return rc;
     }
 
     // execute_measurement_steps_with_sampling_and_postselection_retries_,
-    // qengine.hpp:374-481 (projective measurements, no post-selection)
+    // qengine.hpp:374-481, and build_sampling_maps_, :150-213.  The sampled
+    // qudits (ascending) fill the classical dits in ascending c_reg order.
+    // A qudit measured into two dits would make qpp read past its sample
+    // (one digit per qudit); such circuits run rep by rep here.
     int execute_sampling(const std::vector<EStep>& s, size_t pos, uint64_t reps) {
-        std::map<uint64_t, uint64_t> c_q;
-        bool any = false;
+        std::map<uint64_t, uint64_t> c_q, q_ps;
+        bool any = false, has_ps = false;
         for (size_t i = pos; i < s.size(); ++i)
             if (is_meas(s[i])) {
                 any = true;
                 const auto cr = c_regs_of(s[i]);
-                for (size_t q = 0; q < s[i].target.size(); ++q) c_q[cr[q]] = s[i].target[q];
+                for (size_t q = 0; q < s[i].target.size(); ++q) {
+                    const uint64_t t = s[i].target[q];
+                    c_q[cr[q]] = t;
+                    if (s[i].type != QS_STEP_POST_SELECT) continue;
+                    has_ps = true;
+                    auto it = q_ps.find(t);
+                    if (it == q_ps.end()) {
+                        q_ps[t] = s[i].ps[q];
+                    } else if (it->second != s[i].ps[q]) {
+                        ps_ok = false;  // incompatible post-selections
+                        return QS_OK;
+                    }
+                }
             }
         if (!any) return QS_OK;
         std::set<uint64_t> sset;
         for (auto& kv : c_q) sset.insert(kv.second);
         const std::vector<uint64_t> from(sset.begin(), sset.end());
+        if (c_q.size() != from.size()) return execute_with_retries(s, pos, reps);
+        if (has_ps) return execute_sampling_ps(s, pos, reps, c_q, q_ps, from);
         const uint64_t D = live_dim();
         if (s.size() == pos + 1) {
             // A single measurement step: its draw and every later rep's draw
@@ -429,9 +578,7 @@ struct qs_engine {
         if (int rc = qs_state_clone(st, &pre)) return rc;
         // rep 0: one sample drawn and discarded, then the measurement steps
         if (D >= 2) (void)canonical();
-        bool measured = false;
-        int rc = execute_steps_once(s, pos, measured);
-        if (!rc && measured) ++stats[dits];
+        int rc = execute_steps_once(s, pos, true);
         // reps 1..: one draw each, sampled from the pre-measurement state
         std::vector<double> u;
         for (uint64_t rep = 1; rep < reps && !rc; ++rep) u.push_back(D >= 2 ? canonical() : 0.0);
@@ -451,6 +598,90 @@ struct qs_engine {
         return rc;
     }
 
+    // The sampling tail with post-selected qudits (qengine.hpp:414-473).
+    // Every attempt draws once and samples the pre-measurement state; an
+    // attempt whose post-selected qudits differ from their values fails
+    // its rep, or with ensure_ps is drawn again while fewer than max_ps
+    // attempts were made.  The first successful rep runs the steps once
+    // (ensure on, no attempt limit), which fixes the engine's state; later
+    // successes add statistics only.  Until that first success attempts go
+    // one by one, since the steps' own draws follow it; after it no other
+    // draw happens, so the draws come in batches of the remaining rep count
+    // (each remaining rep takes at least one) and are sampled in one call.
+    int execute_sampling_ps(const std::vector<EStep>& s, size_t pos, uint64_t reps,
+                            const std::map<uint64_t, uint64_t>& c_q,
+                            const std::map<uint64_t, uint64_t>& q_ps,
+                            const std::vector<uint64_t>& from) {
+        std::vector<std::pair<size_t, uint64_t>> chk;  // (position in `from`, value)
+        for (size_t q = 0; q < from.size(); ++q) {
+            auto it = q_ps.find(from[q]);
+            if (it != q_ps.end()) chk.push_back({q, it->second});
+        }
+        auto failed = [&](const uint64_t* dig) {
+            for (const auto& c : chk)
+                if (dig[c.first] != c.second) return true;
+            return false;
+        };
+        const uint64_t D = live_dim(), nf = from.size();
+        qs_state* pre = nullptr;
+        if (int rc = qs_state_clone(st, &pre)) return rc;
+        bool found = false;
+        uint64_t rep = 0, attempts = 0;
+        int rc = QS_OK;
+        std::vector<uint64_t> dig(nf);
+        while (!rc && rep < reps && !found) {
+            ++attempts;
+            double u = D >= 2 ? canonical() : 0.0;
+            rc = qs_sample(pre, uint32_t(nf), from.data(), 1, &u, nullptr, dig.data(), nullptr);
+            if (rc) break;
+            if (failed(dig.data())) {
+                ps_ok = false;
+                if (!(ensure_ps && attempts < max_ps)) {
+                    ++rep;
+                    attempts = 0;
+                }
+                continue;
+            }
+            ps_ok = true;
+            found = true;
+            const uint64_t m0 = max_ps;
+            max_ps = std::numeric_limits<uint64_t>::max();
+            rc = execute_steps_once(s, pos, true);
+            max_ps = m0;
+            ++rep;
+            attempts = 0;
+        }
+        while (!rc && rep < reps) {
+            const uint64_t B = reps - rep;
+            std::vector<double> u(B);
+            for (auto& x : u) x = D >= 2 ? canonical() : 0.0;
+            std::vector<uint64_t> dg(B * nf);
+            rc = qs_sample(pre, uint32_t(nf), from.data(), B, u.data(), nullptr, dg.data(), nullptr);
+            for (uint64_t b = 0; b < B && !rc; ++b) {
+                ++attempts;
+                const uint64_t* d = dg.data() + b * nf;
+                if (failed(d)) {
+                    ps_ok = false;
+                    if (!(ensure_ps && attempts < max_ps)) {
+                        ++rep;
+                        attempts = 0;
+                    }
+                    continue;
+                }
+                ps_ok = true;
+                std::vector<uint64_t> res = dits;
+                size_t i = 0;
+                for (auto& kv : c_q) res[kv.first] = d[i++];
+                ++stats[res];
+                ++rep;
+                attempts = 0;
+            }
+        }
+        if (found) ps_ok = true;
+        qs_state_destroy(pre);
+        return rc;
+    }
+
     void reset_bookkeeping() {
         dits.assign(nc, 0);
         probs.assign(nc, 0.0);
@@ -464,15 +695,22 @@ struct qs_engine {
         if (!live().empty() && live().size() != nq)
             return fail(QS_ERR_DIMS_NOT_EQUAL, "qpp::QEngineT::execute()", "state");
         reset_bookkeeping();
+        ps_ok = true;  // ensure_ps and max_ps are kept (qengine.hpp:1354-1361)
         const std::vector<EStep> s = reps > 1 ? canonical_form(steps) : steps;
         if (s.empty()) return QS_OK;
-        const size_t pos = std::find_if(s.begin(), s.end(), is_meas) - s.begin();
+        // compute_optimize_up_to_pos_ and can_sample_from_, qengine.hpp:87-143
+        const size_t pos = std::find_if(s.begin(), s.end(), [](const EStep& x) {
+                               return is_meas(x) || is_reset_or_discard(x);
+                           }) - s.begin();
         bool can_sample = true;
         for (size_t i = pos; i < s.size(); ++i)
             if (!is_meas(s[i])) can_sample = false;
         std::vector<GateApp> pre;
         for (size_t i = 0; i < pos; ++i)
-            if (s[i].type != QS_STEP_NOP) pre.push_back(app_of(s[i]));
+            if (s[i].type != QS_STEP_NOP) {
+                if (on_measured(s[i])) return already_measured(pre);
+                pre.push_back(app_of(s[i]));
+            }
         if (int rc = run_gates(pre, true)) return rc;
         if (reps > 1 && can_sample) return execute_sampling(s, pos, reps);
         return execute_with_retries(s, pos, reps);
@@ -521,9 +759,28 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         t.c_reg = s.c_reg;
         t.destructive = s.destructive != 0;
+        if (s.type > QS_STEP_POST_SELECT) {
+            delete e;
+            return fail(QS_ERR_OUT_OF_RANGE, W, "step type");
+        }
         if (is_meas(t) && t.c_reg + t.target.size() > qc->nc) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::measure()", "c_reg");
+        }
+        bool in_range = true;
+        for (uint64_t q : t.ctrl) in_range = in_range && q < qc->nq;
+        for (uint64_t q : t.target) in_range = in_range && q < qc->nq;
+        if (!in_range) {
+            delete e;
+            return fail(QS_ERR_OUT_OF_RANGE, W, "ctrl/target");
+        }
+        if (t.type == QS_STEP_POST_SELECT) {
+            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.ntarget);
+            for (uint64_t v : t.ps)
+                if (v >= qc->d) {
+                    delete e;
+                    return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::post_select()", "ps_vals");
+                }
         }
         e->steps.push_back(t);
     }
@@ -614,6 +871,22 @@ int qs_engine_reset(qs_engine* e) {
     e->st = s;
     e->reset_bookkeeping();
     e->stats.clear();
+    e->ps_ok = true;
+    e->ensure_ps = false;
+    e->max_ps = std::numeric_limits<uint64_t>::max();
+    return QS_OK;
+}
+
+int qs_engine_set_post_selection(qs_engine* e, int ensure, uint64_t max_reps) {
+    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::set_ensure_post_selection()");
+    e->ensure_ps = ensure != 0;
+    e->max_ps = max_reps;
+    return QS_OK;
+}
+
+int qs_engine_post_select_ok(const qs_engine* e, int* ok) {
+    if (!e || !ok) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::post_select_ok()");
+    *ok = e->ps_ok ? 1 : 0;
     return QS_OK;
 }
 

@@ -671,6 +671,35 @@ int qs_measure_seq(qs_state* st, uint32_t ntarget, const uint64_t* target,
     return qs::measure_seq_given(st, ntarget, target, destructive, idx[0], results, probs);
 }
 
+// qpp::reset, instruments.hpp:1148-1196: measure_seq(A, target, dims, false),
+// then powm(Xd(dims[i]), dims[i] - result[i]) on target[i].  Xd(D) maps
+// |j> to |j+1 mod D> (gates.hpp:360-373), so the correction is the exact
+// permutation |j> -> |j + dims[i] - result[i] mod dims[i]>; the identity
+// (result 0) is not launched.  qpp sizes the correction by dims[i], not
+// dims[target[i]]; its apply() then throws where the two differ, which is
+// checked here before the state changes.
+int qs_reset(qs_state* st, uint32_t ntarget, const uint64_t* target, double u) {
+    constexpr const char* W = "qpp::reset()";
+    if (!valid(st)) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    std::vector<uint64_t> tv(target, target + ntarget);
+    if (!check_subsys(tv, st->dims)) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    for (uint32_t i = 0; i < ntarget; ++i)
+        if (st->dims[i] != st->dims[tv[i]])
+            return fail(QS_ERR_MATRIX_MISMATCH_SUBSYS, "qpp::apply()", "A/dims/target");
+    std::vector<uint64_t> res(ntarget);
+    std::vector<double> ps(ntarget);
+    if (int rc = qs_measure_seq(st, ntarget, target, 0, u, res.data(), ps.data())) return rc;
+    for (uint32_t i = 0; i < ntarget; ++i) {
+        const uint64_t D = st->dims[i], k = (D - res[i]) % D;
+        if (k == 0) continue;
+        std::vector<double> M(2 * D * D, 0.0);  // column-major, interleaved complex
+        for (uint64_t j = 0; j < D; ++j) M[2 * (((j + k) % D) + j * D)] = 1.0;
+        if (int rc = qs_apply(st, M.data(), D, 1, &tv[i])) return rc;
+    }
+    return QS_OK;
+}
+
 }  // extern "C"
 
 // measure_seq with the sampled basis index already known (the engine draws

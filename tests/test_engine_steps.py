@@ -1,0 +1,327 @@
+"""reset, discard and post-selection: qpp::reset (instruments.hpp:1148-1196),
+the engine's RESET / DISCARD / POST_SELECT steps (qengine.hpp:650-821) and
+its post-selection retries (qengine.hpp:240-481, 1177-1227).
+
+The oracle is pinned on the CPU by closed forms and by draw sequences worked
+out by hand from qpp's loops for small circuits; the device engine is then
+compared with the oracle on the same seeds (statistics, dits, probabilities,
+post_select_ok and final state)."""
+import numpy as np
+import pytest
+
+from helpers import H, X, Xd, haar, randket, randrho
+
+import qpp_amd as qa
+
+INF = (1 << 64) - 1
+
+
+def bell_ps_circuit(ps_val=1, tail_gate=False):
+    """H(0) CNOT(0,1), post-select q0 = ps_val into c0, measure q1 into c1.
+    With tail_gate the post-selection is non-destructive and X(0) follows
+    it: the canonical form cannot move the post-selection past a gate on its
+    own qubit, so the circuit is re-run per rep instead of sampled."""
+    qc = qa.QCircuit(2, 2)
+    qc.gate(H, 0).CTRL(X, 0, 1).post_select(0, ps_val, 0, destructive=not tail_gate)
+    if tail_gate:
+        qc.gate(X, 0)
+    return qc.measure(1, 1)
+
+
+# ---------------------------------------------------------------------------
+# qpp::reset in the oracle (closed forms)
+# ---------------------------------------------------------------------------
+def test_oracle_reset_single_qubit(orc):
+    """a|0> + b|1>: outcome 0 when u < |a|^2 (keeps a/|a| |0>), else 1,
+    whose correction X maps b/|b| |1> to b/|b| |0>."""
+    a, b = 0.6, 0.8j
+    psi = np.array([a, b])
+    for u, exp in ((0.2, np.array([1, 0])), (0.5, np.array([1j, 0]))):
+        assert np.max(np.abs(orc.reset(psi, [0], [2], u) - exp)) < 1e-15
+
+
+def test_oracle_reset_bell_and_qutrits(orc):
+    bell = np.array([1, 0, 0, 1]) / np.sqrt(2)
+    # measuring q0 of a Bell pair gives r with u < 1/2 <=> r = 0; state |r r>,
+    # then X^r on q0: |0 r>
+    assert np.max(np.abs(orc.reset(bell, [0], [2, 2], 0.3) - [1, 0, 0, 0])) < 1e-15
+    assert np.max(np.abs(orc.reset(bell, [0], [2, 2], 0.7) - [0, 1, 0, 0])) < 1e-15
+    # qutrits |1, 2>: reset q1 measures 2, Xd^(3-2) maps |2> -> |0>
+    k = np.zeros(9)
+    k[1 * 3 + 2] = 1
+    exp = np.zeros(9)
+    exp[1 * 3 + 0] = 1
+    assert np.max(np.abs(orc.reset(k, [1], [3, 3], 0.5) - exp)) < 1e-15
+    # both qutrits: Xd^(3-1) on q0 and Xd^(3-2) on q1 -> |0, 0>
+    exp0 = np.zeros(9)
+    exp0[0] = 1
+    assert np.max(np.abs(orc.reset(k, [0, 1], [3, 3], 0.5) - exp0)) < 1e-15
+
+
+def test_oracle_reset_density_matrix(orc):
+    """rho -> X^r P_r rho P_r X^r / tr(P_r rho) on the target, r from the
+    sample at u (itself pinned against libstdc++ draws)."""
+    rng = np.random.default_rng(3)
+    rho = randrho(8, rng)
+    for t, u in ((0, 0.1), (1, 0.6), (2, 0.95)):
+        _, dig, _ = orc.sample(rho, [t], [2, 2, 2], u)
+        r = dig[0]
+        P = np.diag([1.0 - r, float(r)])
+        ops = [np.eye(2)] * 3
+        ops[t] = P
+        Pf = np.kron(np.kron(ops[0], ops[1]), ops[2])
+        ops[t] = X if r else np.eye(2)
+        Xf = np.kron(np.kron(ops[0], ops[1]), ops[2])
+        m = Pf @ rho @ Pf
+        exp = Xf @ (m / np.trace(m)) @ Xf
+        got = orc.reset(rho, [t], [2, 2, 2], u)
+        assert np.max(np.abs(got - exp)) < 1e-14
+        # the target is in |0>: its reduced state is |0><0|
+        red = orc.ptrace(got, [q for q in range(3) if q != t], [2, 2, 2])
+        assert np.max(np.abs(red - np.diag([1, 0]))) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# oracle engine: reset / discard / post-selection
+# ---------------------------------------------------------------------------
+def test_oracle_engine_reset_and_discard(orc):
+    qc = qa.QCircuit(2, 2)
+    qc.gate(X, 0).CTRL(X, 0, 1).reset(0).measure([0, 1], 0)
+    stats, dits, _, _ = orc.engine_run(qc, reps=20, seed=5)
+    assert stats == {(0, 1): 20} and dits == [0, 1]
+    # discard q0 of a Bell pair, then measure q1: the same value as the
+    # discarded one, either way about half of the time
+    qc = qa.QCircuit(2, 1)
+    qc.gate(H, 0).CTRL(X, 0, 1).discard(0).measure(1, 0)
+    stats, dits, _, st = orc.engine_run(qc, reps=400, seed=6)
+    assert set(stats) <= {(0,), (1,)} and 150 < stats.get((0,), 0) < 250
+    assert st.shape == (1,) and abs(abs(st[0]) - 1) < 1e-12
+
+
+def test_oracle_canonical_form_moves_measurements_past_reset(orc):
+    """A reset step has no targets for can_swap (qcircuit.hpp:7371-7449), so
+    for reps > 1 the canonical form moves measure(0) past reset(0): X, then
+    the measurement sees the reset qubit.  reps = 1 keeps program order."""
+    qc = qa.QCircuit(1, 1)
+    qc.gate(X, 0).measure(0, 0, destructive=False).reset(0)
+    stats, dits, _, _ = orc.engine_run(qc, reps=1, seed=1)
+    assert stats == {(1,): 1} and dits == [1]
+    stats, dits, _, _ = orc.engine_run(qc, reps=5, seed=1)
+    assert stats == {(0,): 5} and dits == [0]
+
+
+def _bell_sampling_expected(orc, seed, reps, ensure, max_ps):
+    """The sampling tail (qengine.hpp:414-473) on bell_ps_circuit(1), worked
+    out by hand: every attempt draws once and samples (q0, q1) from the Bell
+    state (index 0 or 3); the first success runs the steps once -- the
+    post-selection drawing until q0 = 1 (ensure on, no limit), then one draw
+    for measure q1 -- and adds (1, 1); later successes add (1, 1)."""
+    r = orc.Rng(seed)
+    p = [0.5, 0, 0, 0.5]
+    found, count, rep, att = False, 0, 0, 0
+    while rep < reps:
+        att += 1
+        ok = orc.discrete_draw(p, r.canonical()) == 3
+        if not ok:
+            if ensure and att < max_ps:
+                continue
+            rep, att = rep + 1, 0
+            continue
+        if not found:
+            found = True
+            while orc.discrete_draw(p, r.canonical()) != 3:
+                pass
+            r.canonical()
+        count += 1
+        rep, att = rep + 1, 0
+    return ({(1, 1): count} if count else {}), found
+
+
+@pytest.mark.parametrize("ensure,max_ps", [(False, INF), (True, 2), (True, INF)])
+def test_oracle_post_select_sampling_draw_order(orc, ensure, max_ps):
+    qc = bell_ps_circuit(1)
+    for seed in (1, 2, 3, 4):
+        stats, dits, probs, st, ok = orc.engine_run(
+            qc, reps=64, seed=seed, ensure_post_selection=ensure,
+            max_post_selection_reps=max_ps, with_ps_ok=True)
+        exp, found = _bell_sampling_expected(orc, seed, 64, ensure, max_ps)
+        assert stats == exp and ok == found
+        if found:
+            assert dits == [1, 1] and np.allclose(probs, [0.5, 1.0], rtol=0, atol=1e-12)
+
+
+def test_oracle_post_select_retries_draw_order(orc):
+    """Re-run per rep (qengine.hpp:316-345): one draw for the
+    post-selection, then -- on success only -- one for the measurement of
+    q1, which is 1 (the state is |11>, then X(0): |01>)."""
+    qc = bell_ps_circuit(1, tail_gate=True)
+    p = [0.5, 0, 0, 0.5]
+    for seed in (7, 8):
+        stats, dits, _, st, ok = orc.engine_run(qc, reps=40, seed=seed, with_ps_ok=True)
+        r = orc.Rng(seed)
+        n_ok = 0
+        for _ in range(40):
+            if orc.discrete_draw(p, r.canonical()) == 3:
+                n_ok += 1
+                r.canonical()
+        assert stats == ({(1, 1): n_ok} if n_ok else {}) and ok == (n_ok > 0)
+        assert dits == [1, 1] and st.shape == (2,)
+        assert np.max(np.abs(st - [1, 0])) < 1e-12
+
+
+def test_oracle_post_select_impossible(orc):
+    """q0 stays |0>; post-selecting 1 fails every time.  Without ensure the
+    failed result is kept (dit 0, state collapsed); with ensure and a limit
+    the step gives up and leaves state and dits untouched."""
+    qc = qa.QCircuit(2, 2)
+    qc.gate(H, 1).post_select(0, 1, 0).measure(1, 1)
+    stats, dits, probs, st, ok = orc.engine_run(qc, reps=1, seed=3, with_ps_ok=True)
+    assert stats == {} and not ok and dits == [0, 0] and abs(probs[0] - 1) < 1e-12
+    assert st.shape == (2,)
+    stats, dits, probs, st, ok = orc.engine_run(qc, reps=1, seed=3, ensure_post_selection=True,
+                                                max_post_selection_reps=5, with_ps_ok=True)
+    assert stats == {} and not ok and probs == [0.0, 0.0] and st.shape == (4,)
+    with pytest.raises(RuntimeError):
+        orc.engine_run(qc, reps=1, seed=3, ensure_post_selection=True)
+
+
+# ---------------------------------------------------------------------------
+# device: qs_reset and the engine steps against the oracle
+# ---------------------------------------------------------------------------
+gpu = pytest.mark.gpu
+
+
+@gpu
+def test_reset_vs_oracle(orc):
+    rng = np.random.default_rng(11)
+    cases = [([2] * 5, [3]), ([2] * 5, [0, 4, 2]), ([2] * 12, [5, 11]), ([3, 3, 3], [2, 0]),
+             ([2] * 13, list(range(13)))]
+    for dims, target in cases:
+        D = int(np.prod(dims))
+        for A in (randket(D, rng), randrho(D, rng) if D <= 256 else None):
+            if A is None:
+                continue
+            for u in (0.05, 0.5, 0.93):
+                got = qa.reset(A, target, dims, u)
+                exp = orc.reset(A, target, dims, u)
+                assert np.max(np.abs(got - exp)) <= 1e-12, (dims, target, u)
+
+
+@gpu
+def test_reset_errors():
+    psi = randket(6, np.random.default_rng(1))
+    with pytest.raises(qa.SubsysMismatchDims):
+        qa.reset(psi, [2], [2, 3])
+    # the correction is sized by dims[i] (here dims[0] = 2) and qpp's apply
+    # rejects it on a qutrit target
+    with pytest.raises(qa.MatrixMismatchSubsys):
+        qa.reset(psi, [1], [2, 3])
+    assert np.max(np.abs(qa.reset(psi, [0], [2, 3], 0.4) -
+                         qa.reset(psi, [0], [2, 3], 0.4))) == 0
+
+
+def random_step_circuit(n, nc, seed, d=2, n_ops=14, tail=True):
+    """Gates, controlled gates, measurements (destructive or not),
+    post-selections with random values, resets and discards on live
+    qudits, c_regs in range."""
+    rng = np.random.default_rng(seed)
+    qc = qa.QCircuit(n, nc, d)
+    live = list(range(n))
+    for _ in range(n_ops):
+        kind = rng.choice(["g", "g", "c", "m", "ps", "reset", "discard"])
+        if kind == "c" and len(live) < 2:
+            kind = "g"
+        if kind == "g":
+            qc.gate(haar(d, rng), int(rng.choice(live)))
+        elif kind == "c":
+            a, b = (int(x) for x in rng.choice(live, 2, replace=False))
+            qc.CTRL(Xd(d), a, b)
+        elif kind in ("m", "ps"):
+            k = int(rng.integers(1, min(2, len(live)) + 1))
+            t = [int(x) for x in rng.choice(live, k, replace=False)]
+            c = int(rng.integers(0, nc - k + 1))
+            dest = bool(rng.integers(0, 2)) and len(live) > k
+            if kind == "m":
+                qc.measure(t if k > 1 else t[0], c, dest)
+            else:
+                qc.post_select(t, [int(rng.integers(0, d)) for _ in t], c, dest)
+            if dest:
+                live = [q for q in live if q not in t]
+        elif kind == "reset":
+            k = int(rng.integers(1, min(2, len(live)) + 1))
+            qc.reset([int(x) for x in rng.choice(live, k, replace=False)])
+        elif len(live) > 1:
+            q = int(rng.choice(live))
+            qc.discard(q)
+            live.remove(q)
+    if tail:
+        qc.measure(live, 0, destructive=False)
+    return qc
+
+
+def _compare(orc, qc, reps, seed, kind=qa.KET, noise_p=-1.0, ensure=False, max_ps=INF):
+    e = qa.QEngine(qc, kind=kind, noise_p=noise_p, seed=seed)
+    e.set_ensure_post_selection(ensure).set_max_post_selection_reps(max_ps)
+    try:
+        stats, dits, probs, st, ok = orc.engine_run(
+            qc, reps=reps, seed=seed, noise_p=noise_p, is_rho=kind == qa.RHO,
+            ensure_post_selection=ensure, max_post_selection_reps=max_ps, with_ps_ok=True)
+    except RuntimeError as err:
+        # the oracle stopped with qpp's error (QuditAlreadyMeasured for a step
+        # the canonical form moved after a discard of its qudit, OutOfRange
+        # for a post-selection retried without limit that never succeeds)
+        with pytest.raises(qa._BY_CODE[err.code]):
+            e.execute(reps)
+        return
+    e.execute(reps)
+    assert e.get_stats() == stats
+    assert e.get_dits() == dits
+    assert e.post_select_ok() == ok
+    assert np.max(np.abs(np.array(e.get_probs()) - np.array(probs))) <= 1e-12
+    got = e.get_state().download()
+    assert got.shape == st.shape and np.max(np.abs(got - st)) <= 1e-12
+
+
+@gpu
+@pytest.mark.parametrize("ensure,max_ps", [(False, INF), (True, 2), (True, INF)])
+def test_engine_bell_post_selection_vs_oracle(orc, ensure, max_ps):
+    for tail in (False, True):
+        for seed in (1, 2, 3):
+            _compare(orc, bell_ps_circuit(1, tail), 64, seed, ensure=ensure, max_ps=max_ps)
+
+
+@gpu
+@pytest.mark.parametrize("seed", range(12))
+def test_engine_random_steps_vs_oracle(orc, seed):
+    n = 3 + seed % 4
+    qc = random_step_circuit(n, n, seed)
+    for reps in (1, 30):
+        for ensure, max_ps in ((False, INF), (True, 3)):
+            _compare(orc, qc, reps, 100 + seed, ensure=ensure, max_ps=max_ps)
+
+
+@gpu
+def test_engine_steps_large_and_rho_and_noise(orc):
+    # 13 qubits: gate runs between the steps go through fused tile programs
+    qc = random_step_circuit(13, 13, 77, n_ops=18)
+    _compare(orc, qc, 1, 5)
+    _compare(orc, qc, 6, 6, ensure=True, max_ps=2)
+    qc = random_step_circuit(4, 4, 78)
+    _compare(orc, qc, 10, 7, kind=qa.RHO)
+    _compare(orc, qc, 5, 8, noise_p=0.1)
+    _compare(orc, qc, 5, 9, noise_p=0.1, ensure=True, max_ps=2)
+    # qutrits: resets shift by Xd powers
+    qc = random_step_circuit(3, 3, 79, d=3)
+    _compare(orc, qc, 1, 10)
+    _compare(orc, qc, 12, 11)
+
+
+@gpu
+def test_engine_post_select_impossible(orc):
+    qc = qa.QCircuit(2, 2)
+    qc.gate(H, 1).post_select(0, 1, 0).measure(1, 1)
+    _compare(orc, qc, 1, 3)
+    _compare(orc, qc, 1, 3, ensure=True, max_ps=5)
+    e = qa.QEngine(qc, seed=3).set_ensure_post_selection(True)
+    with pytest.raises(qa.OutOfRange):
+        e.execute(1)
