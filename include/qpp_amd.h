@@ -67,6 +67,7 @@ typedef enum qs_status {
     QS_ERR_NOT_IMPLEMENTED = 17,         /* exception::NotImplemented */
     QS_ERR_OVERFLOW = 18,                /* std::overflow_error (normalize) */
     QS_ERR_IO = 19,                      /* std::runtime_error (load/save streams) */
+    QS_ERR_INVALID_CONDITIONAL = 20,     /* exception::InvalidConditional */
     /* engine-side conditions with no qpp counterpart */
     QS_ERR_INVALID_HANDLE = 100,
     QS_ERR_OUT_OF_MEMORY = 101,          /* device memory exhausted */
@@ -267,10 +268,18 @@ typedef enum qs_step_type {
                                  qengine.hpp:804-808 */
     QS_STEP_DISCARD = 8,      /* DISCARD / DISCARD_MANY: destructive
                                  measure_seq, results dropped, :810-821 */
-    QS_STEP_POST_SELECT = 9   /* POST_SELECT(_MANY) / _ND (flag): measure_seq
+    QS_STEP_POST_SELECT = 9,  /* POST_SELECT(_MANY) / _ND (flag): measure_seq
                                  of all targets, written from c_reg; the run
                                  goes on only if the results equal the
                                  post-selection values, :650-733 */
+    QS_STEP_COND_IF = 10,     /* runtime if (qcircuit.hpp cond_if,
+                                 qengine.hpp:837-860): true when dit
+                                 pool[ctrl_off + j] equals pool[ps_off + j]
+                                 for j < nctrl; false jumps past the matching
+                                 COND_ELSE, or else COND_END */
+    QS_STEP_COND_ELSE = 11,   /* reached from the true branch: jumps past
+                                 the matching COND_END */
+    QS_STEP_COND_END = 12     /* end of the conditional block */
 } qs_step_type;
 
 typedef struct qs_step {

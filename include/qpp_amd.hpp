@@ -95,6 +95,7 @@ QPP_AMD_EXC(TypeMismatch, QS_ERR_TYPE_MISMATCH)
 QPP_AMD_EXC(SizeMismatch, QS_ERR_SIZE_MISMATCH)
 QPP_AMD_EXC(QuditAlreadyMeasured, QS_ERR_QUDIT_ALREADY_MEASURED)
 QPP_AMD_EXC(NotImplemented, QS_ERR_NOT_IMPLEMENTED)
+QPP_AMD_EXC(InvalidConditional, QS_ERR_INVALID_CONDITIONAL)
 QPP_AMD_EXC(DeviceError, QS_ERR_DEVICE)
 #undef QPP_AMD_EXC
 }  // namespace exception
@@ -122,6 +123,7 @@ inline void check(int rc) {
         case QS_ERR_NOT_IMPLEMENTED: throw exception::NotImplemented(m);
         case QS_ERR_OVERFLOW: throw std::overflow_error("qpp::normalize(): Division by zero!");
         case QS_ERR_IO: throw std::runtime_error(m);
+        case QS_ERR_INVALID_CONDITIONAL: throw exception::InvalidConditional(m);
         default: throw exception::DeviceError(m);
     }
 }
@@ -427,6 +429,25 @@ class QCircuit {
         pool_.insert(pool_.end(), ps_vals.begin(), ps_vals.end());
         return *this;
     }
+    // runtime conditional block (qcircuit.hpp cond_if / cond_else /
+    // cond_end): the condition is "dit dits[j] equals vals[j] for every j",
+    // the form OpenQASM's if (c==n) takes
+    QCircuit& cond_if(const std::vector<idx>& dits, const std::vector<idx>& vals) {
+        if (dits.size() != vals.size())
+            throw std::invalid_argument("qpp_amd::QCircuit::cond_if(): one value per dit");
+        qs_step s{};
+        s.type = QS_STEP_COND_IF;
+        s.nctrl = static_cast<uint32_t>(dits.size());
+        s.ctrl_off = static_cast<uint32_t>(pool_.size());
+        pool_.insert(pool_.end(), dits.begin(), dits.end());
+        s.ps_off = static_cast<uint32_t>(pool_.size());
+        pool_.insert(pool_.end(), vals.begin(), vals.end());
+        s.shift_off = UINT32_MAX;
+        steps_.push_back(s);
+        return *this;
+    }
+    QCircuit& cond_else() { return marker(QS_STEP_COND_ELSE); }
+    QCircuit& cond_end() { return marker(QS_STEP_COND_END); }
     // reset / discard (qcircuit.hpp:3745-3940)
     QCircuit& reset(idx target) { return meas(QS_STEP_RESET, {target}, 0, false); }
     QCircuit& reset(const std::vector<idx>& target) { return meas(QS_STEP_RESET, target, 0, false); }
@@ -459,6 +480,13 @@ class QCircuit {
     }
 
   private:
+    QCircuit& marker(uint32_t type) {
+        qs_step s{};
+        s.type = type;
+        s.shift_off = UINT32_MAX;
+        steps_.push_back(s);
+        return *this;
+    }
     QCircuit& add(uint32_t type, const cmat& U, const std::vector<idx>& ctrl,
                   const std::vector<idx>& target,
                   std::optional<std::vector<idx>> shift = std::nullopt) {

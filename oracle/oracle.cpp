@@ -980,8 +980,14 @@ struct Step {
     std::optional<std::vector<idx>> shift;
     idx c_reg;
     bool destructive;
-    std::vector<idx> ps;  // POST_SELECT values, one per target
+    std::vector<idx> ps;  // POST_SELECT values; COND_IF dit values
+    idx jump = 0;         // COND_IF: its ELSE, else its END; COND_ELSE: its END
 };
+
+bool is_runtime(const Step& s) {
+    return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE ||
+           s.type == QS_STEP_COND_END;
+}
 
 // is_measurement, classes/qcircuit.hpp:7008-7025: the projective
 // measurements and post-selections (reset and discard are not measurements)
@@ -1028,6 +1034,8 @@ bool can_swap(const Step& a, const Step& b) {
 // exist in the program format): bubble every measurement after the first
 // one towards the end while it can swap with a non-measurement.
 std::vector<Step> canonical_form(const std::vector<Step>& steps) {
+    // a circuit with runtime steps is not reordered (:7563-7567)
+    if (std::any_of(steps.begin(), steps.end(), is_runtime)) return steps;
     auto first = std::find_if(steps.begin(), steps.end(), is_measurement);
     std::vector<Step> before(steps.begin(), first);
     std::vector<Step> rest(first, steps.end());
@@ -1255,8 +1263,19 @@ struct Engine {
             reset_step(s);
         else if (s.type == QS_STEP_DISCARD)
             discard_step(s);
-        else if (s.type != QS_STEP_NOP)
+        else if (s.type != QS_STEP_NOP && !is_runtime(s))
             gate_step(s);
+    }
+
+    // visit_runtime_step_, qengine.hpp:826-870 (IF / ELSE / ENDIF): the
+    // position the loop continues after
+    idx runtime_next(const Step& s, idx i) const {
+        if (s.type == QS_STEP_COND_IF) {
+            for (idx j = 0; j < s.ctrl.size(); ++j)
+                if (st.dits[s.ctrl[j]] != s.ps[j]) return s.jump;
+            return i;
+        }
+        return s.type == QS_STEP_COND_ELSE ? s.jump : i;
     }
 
     // execute_steps_once_, qengine.hpp:240-294 (no runtime steps): stops
@@ -1270,6 +1289,7 @@ struct Engine {
         for (idx i = pos; i < steps_.size(); ++i) {
             if (is_measurement(steps_[i])) measured = true;
             execute_step(steps_[i]);
+            i = runtime_next(steps_[i], i);
             if (!st.ps_ok) {
                 if (ensure) st.ensure_ps = saved_ensure;
                 return;
@@ -1394,7 +1414,8 @@ struct Engine {
         // (:122-143): the first measurement / reset / discard step, and
         // whether only projective measurements follow it
         const idx pos = std::find_if(s.begin(), s.end(), [](const Step& x) {
-                            return is_measurement(x) || is_reset_or_discard(x);
+                            return is_measurement(x) || is_reset_or_discard(x) ||
+                                   is_runtime(x);
                         }) - s.begin();
         bool can_sample = true;
         for (idx i = pos; i < s.size(); ++i)
@@ -1658,8 +1679,28 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
         t.destructive = s.destructive != 0;
         if (s.type == QS_STEP_POST_SELECT)
             t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.ntarget);
+        if (s.type == QS_STEP_COND_IF)
+            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.nctrl);
         e.steps.push_back(t);
     }
+    // match IF / ELSE / END (QCircuit::validate_conditionals)
+    std::vector<idx> open;
+    for (idx i = 0; i < e.steps.size(); ++i) {
+        const uint32_t ty = e.steps[i].type;
+        if (ty == QS_STEP_COND_IF) {
+            open.push_back(i);
+        } else if (ty == QS_STEP_COND_ELSE) {
+            if (open.empty() || e.steps[open.back()].type != QS_STEP_COND_IF)
+                return QS_ERR_INVALID_CONDITIONAL;
+            e.steps[open.back()].jump = i;
+            open.back() = i;
+        } else if (ty == QS_STEP_COND_END) {
+            if (open.empty()) return QS_ERR_INVALID_CONDITIONAL;
+            e.steps[open.back()].jump = i;
+            open.pop_back();
+        }
+    }
+    if (!open.empty()) return QS_ERR_INVALID_CONDITIONAL;
     e.reset_state(std::nullopt);
     e.st.ensure_ps = ensure_ps != 0;
     e.st.max_ps = max_ps;

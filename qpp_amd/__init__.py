@@ -76,6 +76,7 @@ QuditAlreadyMeasured = _mk("QuditAlreadyMeasured", 16)
 NotImplemented_ = _mk("NotImplemented", 17)
 OverflowError_ = _mk("OverflowError", 18)
 RuntimeError_ = _mk("RuntimeError", 19)  # std::runtime_error (load/save streams)
+InvalidConditional = _mk("InvalidConditional", 20)
 InvalidHandle = _mk("InvalidHandle", 100)
 OutOfMemory = _mk("OutOfMemory", 101)
 DeviceError = _mk("DeviceError", 102)
@@ -86,7 +87,7 @@ _BY_CODE = {c.code: c for c in [
     MatrixMismatchSubsys, DimsInvalid, DimsNotEqual, DimsMismatchMatrix,
     DimsMismatchCvector, SubsysMismatchDims, PermInvalid, PermMismatchDims,
     OutOfRange, TypeMismatch, SizeMismatch, QuditAlreadyMeasured,
-    NotImplemented_, OverflowError_, RuntimeError_, InvalidHandle, OutOfMemory,
+    NotImplemented_, OverflowError_, RuntimeError_, InvalidConditional, InvalidHandle, OutOfMemory,
     DeviceError, NoDevice]}
 
 # ---------------------------------------------------------------------------
@@ -587,7 +588,8 @@ class Rng:
 # circuits (qpp::QCircuit lowered to the qs_circuit program format)
 # ---------------------------------------------------------------------------
 (STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY,
- STEP_RESET, STEP_DISCARD, STEP_POST_SELECT) = range(10)
+ STEP_RESET, STEP_DISCARD, STEP_POST_SELECT, STEP_COND_IF, STEP_COND_ELSE,
+ STEP_COND_END) = range(13)
 
 
 class _Step(ctypes.Structure):
@@ -681,6 +683,27 @@ class QCircuit:
         destructively and drop the results."""
         t = [int(target)] if np.isscalar(target) else list(target)
         self.steps.append((STEP_DISCARD, 0, [], t, None, 0, 1))
+        return self
+
+    def cond_if(self, dits, values):
+        """QCircuit::cond_if (runtime conditional, qengine.hpp:837-860) with
+        the condition "dit dits[j] == values[j] for every j" -- the form
+        OpenQASM's if (c==n) takes.  Close with cond_end(), optionally after
+        cond_else()."""
+        d = [int(dits)] if np.isscalar(dits) else list(dits)
+        v = [int(values)] if np.isscalar(values) else list(values)
+        if len(v) != len(d):
+            raise ValueError("cond_if: one value per dit")
+        self.ps_vals[len(self.steps)] = v
+        self.steps.append((STEP_COND_IF, 0, d, [], None, 0, 0))
+        return self
+
+    def cond_else(self):
+        self.steps.append((STEP_COND_ELSE, 0, [], [], None, 0, 0))
+        return self
+
+    def cond_end(self):
+        self.steps.append((STEP_COND_END, 0, [], [], None, 0, 0))
         return self
 
     def nop(self):
@@ -1129,3 +1152,6 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+from . import qasm  # noqa: E402  (OpenQASM 2.0 reader; needs QCircuit)

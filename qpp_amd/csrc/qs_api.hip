@@ -57,6 +57,7 @@ const char* description(int code) {
         case QS_ERR_NOT_IMPLEMENTED: return "Not yet implemented";
         case QS_ERR_OVERFLOW: return "Division by zero";
         case QS_ERR_IO: return "I/O error";
+        case QS_ERR_INVALID_CONDITIONAL: return "Invalid conditional";
         case QS_ERR_INVALID_HANDLE: return "Invalid handle";
         case QS_ERR_OUT_OF_MEMORY: return "Out of device memory";
         case QS_ERR_DEVICE: return "HIP runtime error";

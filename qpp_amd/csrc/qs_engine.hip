@@ -59,8 +59,13 @@ struct EStep {
     bool has_shift = false;
     uint64_t c_reg = 0;
     bool destructive = true;
-    std::vector<uint64_t> ps;  // POST_SELECT values, one per target
+    std::vector<uint64_t> ps;  // POST_SELECT values, COND_IF dit values
+    size_t jump = 0;           // COND_IF: its COND_ELSE or else COND_END; COND_ELSE: its COND_END
 };
+
+bool is_runtime(const EStep& s) {
+    return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE || s.type == QS_STEP_COND_END;
+}
 
 // is_measurement, classes/qcircuit.hpp:7008-7025: measurements and
 // post-selections; reset and discard are not measurements
@@ -95,8 +100,10 @@ bool can_swap(const EStep& a, const EStep& b) {
     return !overlap(a.ctrl, b.ctrl) && !overlap(ta, tb) && !overlap(c_regs_of(a), c_regs_of(b));
 }
 
-// canonical_form, classes/qcircuit.hpp:7556-7612
+// canonical_form, classes/qcircuit.hpp:7556-7612 (a circuit with runtime
+// steps is left as it is, :7563-7567)
 std::vector<EStep> canonical_form(const std::vector<EStep>& steps) {
+    if (std::any_of(steps.begin(), steps.end(), is_runtime)) return steps;
     auto first = std::find_if(steps.begin(), steps.end(), is_meas);
     std::vector<EStep> before(steps.begin(), first), rest(first, steps.end());
     for (std::ptrdiff_t ri = static_cast<std::ptrdiff_t>(rest.size()) - 1; ri >= 0; --ri) {
@@ -153,6 +160,7 @@ struct qs_engine {
     // get_relative_pos_, qengine.hpp:514-526: a step on a destructively
     // measured qudit is QuditAlreadyMeasured
     bool on_measured(const EStep& x) const {
+        if (is_runtime(x)) return false;
         for (const auto* v : {&x.target, &x.ctrl})
             for (uint64_t q : *v)
                 if (was_measured(q)) return true;
@@ -441,7 +449,16 @@ struct qs_engine {
             if (noise_p >= 0) noise_apps(pending);
             const EStep& x = s[i];
             if (on_measured(x)) return already_measured(pending);
-            if (is_meas(x) || is_reset_or_discard(x)) {
+            if (x.type == QS_STEP_COND_IF) {
+                // visit_runtime_step_, qengine.hpp:837-860: on false, go on
+                // after the matching ELSE (or END)
+                bool t = true;
+                for (size_t j = 0; j < x.ctrl.size(); ++j) t = t && dits[x.ctrl[j]] == x.ps[j];
+                if (!t) i = x.jump;
+            } else if (x.type == QS_STEP_COND_ELSE) {
+                i = x.jump;  // the true branch ends here (:862-866)
+            } else if (x.type == QS_STEP_COND_END) {
+            } else if (is_meas(x) || is_reset_or_discard(x)) {
                 if (int rc = run_gates(pending, cacheable && !pending.empty())) return rc;
                 pending.clear();
                 int rc = QS_OK;
@@ -700,7 +717,7 @@ struct qs_engine {
         if (s.empty()) return QS_OK;
         // compute_optimize_up_to_pos_ and can_sample_from_, qengine.hpp:87-143
         const size_t pos = std::find_if(s.begin(), s.end(), [](const EStep& x) {
-                               return is_meas(x) || is_reset_or_discard(x);
+                               return is_meas(x) || is_reset_or_discard(x) || is_runtime(x);
                            }) - s.begin();
         bool can_sample = true;
         for (size_t i = pos; i < s.size(); ++i)
@@ -759,7 +776,7 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         t.c_reg = s.c_reg;
         t.destructive = s.destructive != 0;
-        if (s.type > QS_STEP_POST_SELECT) {
+        if (s.type > QS_STEP_COND_END) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, W, "step type");
         }
@@ -768,8 +785,11 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
             return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::measure()", "c_reg");
         }
         bool in_range = true;
-        for (uint64_t q : t.ctrl) in_range = in_range && q < qc->nq;
+        const uint64_t ctrl_lim = t.type == QS_STEP_COND_IF ? qc->nc : qc->nq;  // dits or qudits
+        for (uint64_t q : t.ctrl) in_range = in_range && q < ctrl_lim;
         for (uint64_t q : t.target) in_range = in_range && q < qc->nq;
+        if (t.type == QS_STEP_COND_IF)
+            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.nctrl);
         if (!in_range) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, W, "ctrl/target");
@@ -783,6 +803,35 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
                 }
         }
         e->steps.push_back(t);
+    }
+    // match COND_IF / COND_ELSE / COND_END (QCircuit::validate_conditionals)
+    std::vector<size_t> open;
+    for (size_t i = 0; i < e->steps.size(); ++i) {
+        EStep& t = e->steps[i];
+        if (t.type == QS_STEP_COND_IF) {
+            open.push_back(i);
+        } else if (t.type == QS_STEP_COND_ELSE) {
+            if (open.empty() || e->steps[open.back()].type != QS_STEP_COND_IF) {
+                delete e;
+                return fail(QS_ERR_INVALID_CONDITIONAL, "qpp::QCircuit::cond_else()",
+                            "No matching cond_if()");
+            }
+            e->steps[open.back()].jump = i;
+            open.back() = i;  // the ELSE now waits for the END
+        } else if (t.type == QS_STEP_COND_END) {
+            if (open.empty()) {
+                delete e;
+                return fail(QS_ERR_INVALID_CONDITIONAL, "qpp::QCircuit::cond_end()",
+                            "No matching cond_if()");
+            }
+            e->steps[open.back()].jump = i;
+            open.pop_back();
+        }
+    }
+    if (!open.empty()) {
+        delete e;
+        return fail(QS_ERR_INVALID_CONDITIONAL, "qpp::QEngineT::execute()",
+                    "Missing QCircuit::cond_end() delimiter");
     }
     std::vector<uint64_t> dims(qc->nq, qc->d);
     if (int rc = qs_state_create(dtype, kind, qc->nq, dims.data(), &e->st)) {

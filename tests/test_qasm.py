@@ -1,0 +1,247 @@
+"""OpenQASM 2.0 reader (qpp_amd.qasm) and runtime conditional steps.
+
+The fixtures under tests/golden/qasm/ are the reference's own test programs
+(unit_tests/tests/qasm/circuits/units/*.qasm, qasmtools/qasm/generic/*.qasm and
+the qasmtools/qasm/ibmqx2/*.qasm programs that qasm.cpp:41-56 reads).
+The known answers are those of unit_tests/tests/qasm/qasm.cpp:11-186, checked
+on the oracle engine (CPU) and on the device engine, which must also agree
+with the oracle on every fixture (statistics, dits, probabilities, state)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from helpers import H
+
+import qpp_amd as qa
+from qpp_amd import qasm
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "qasm")
+ALL = sorted(glob.glob(os.path.join(FIX, "*.qasm")))
+
+
+def fixture(name):
+    return qasm.read_from_file(os.path.join(FIX, name))
+
+
+def rz(t):
+    return np.diag([np.exp(-0.5j * t), np.exp(0.5j * t)])
+
+
+def ry(t):
+    return np.array([[math.cos(t / 2), -math.sin(t / 2)], [math.sin(t / 2), math.cos(t / 2)]])
+
+
+def rx(t):
+    return np.array([[math.cos(t / 2), -1j * math.sin(t / 2)], [-1j * math.sin(t / 2), math.cos(t / 2)]])
+
+
+def mket(bits):
+    v = np.zeros(2 ** len(bits), dtype=complex)
+    v[int("".join(str(b) for b in bits), 2) if bits else 0] = 1
+    return v
+
+
+# ---------------------------------------------------------------------------
+# reader (CPU)
+# ---------------------------------------------------------------------------
+def test_all_reference_programs_parse():
+    """qasm.cpp:11-57 (StdCompliance) and the unit circuits."""
+    assert len(ALL) == 26
+    for f in ALL:
+        qc = qasm.read_from_file(f)
+        assert qc.nq >= 1 and len(qc.steps) > 0
+
+
+def test_read_from_string_basic():
+    """qasm.cpp:181-184."""
+    qc = qasm.read_from_string('OPENQASM 2.0;\ninclude "qelib1.inc";\n qreg q[10];')
+    assert qc.nq == 10 and qc.nc == 0 and qc.steps == []
+
+
+def test_u_convention_closed_form():
+    """U(theta, phi, lambda) = e^{i(phi+lambda)/2} RZ(phi) RY(theta) RZ(lambda)
+    (Qiskit's convention, the default build)."""
+    for th, ph, la in ((0.1, 0.2, 0.3), (math.pi, 0, math.pi), (-1.3, 2.1, 0.4)):
+        qc = qasm.read_from_string(f"OPENQASM 2.0; qreg q[1]; U({th},{ph},{la}) q[0];")
+        U = qc.mats[qc.steps[0][1]]
+        exp = np.exp(0.5j * (ph + la)) * rz(ph) @ ry(th) @ rz(la)
+        assert np.max(np.abs(U - exp)) < 1e-15
+
+
+def test_lowering_of_builtin_gates():
+    """builtingates.qasm: registers a[1], b[2] take qubits 0 and 1..2; U is
+    one gate step per qubit, CX one two-qubit step, measure one
+    non-destructive step per qubit."""
+    qc = fixture("builtingates.qasm")
+    assert (qc.nq, qc.nc) == (3, 2)
+    kinds = [(s[0], s[3], s[5], s[6]) for s in qc.steps]
+    assert kinds == [(qa.STEP_GATE, [0], 0, 0), (qa.STEP_GATE, [1], 0, 0),
+                     (qa.STEP_GATE, [1, 2], 0, 0), (qa.STEP_MEASURE, [1], 0, 0),
+                     (qa.STEP_MEASURE, [2], 1, 0)]
+    cnot = qc.mats[qc.steps[2][1]]
+    assert np.array_equal(cnot, np.eye(4)[[0, 1, 3, 2]])
+
+
+def test_register_mapping_and_stdlib_expansion():
+    # h q maps over the register; cu1 expands to qelib1's 5 steps; u2 is one U
+    qc = qasm.read_from_string(
+        'OPENQASM 2.0; include "qelib1.inc"; qreg q[3]; h q; cu1(pi/2) q[0],q[2]; u2(0,pi) q[1];')
+    kinds = [(s[0], s[3]) for s in qc.steps]
+    assert kinds == [(1, [0]), (1, [1]), (1, [2]), (1, [0]), (1, [0, 2]), (1, [2]), (1, [0, 2]),
+                     (1, [2]), (1, [1])]
+    # u2(0, pi) = U(pi/2, 0, pi) = H in this convention
+    assert np.max(np.abs(qc.mats[qc.steps[-1][1]] - H)) < 1e-15
+    # cu1(lambda) = diag(1, 1, 1, e^{i lambda}) after the expansion
+    M = np.eye(8, dtype=complex)
+    for s in qc.steps[3:8]:
+        g = qc.mats[s[1]]
+        M = _embed(g, s[3], 3) @ M
+    assert np.max(np.abs(M - np.diag([1, 1, 1, 1, 1, 1j, 1, 1j]))) < 1e-14
+
+
+def _embed(g, targets, n):
+    """the 2^n matrix of gate g on `targets` (target[0] most significant)."""
+    k = len(targets)
+    full = np.zeros((2 ** n, 2 ** n), dtype=complex)
+    for col in range(2 ** n):
+        bits = [(col >> (n - 1 - q)) & 1 for q in range(n)]
+        sub = int("".join(str(bits[t]) for t in targets), 2)
+        for r in range(2 ** k):
+            nb = list(bits)
+            for j, t in enumerate(targets):
+                nb[t] = (r >> (k - 1 - j)) & 1
+            full[int("".join(map(str, nb)), 2), col] += g[r, sub]
+    return full
+
+
+def test_if_statement_lowering():
+    qc = fixture("teleportation.qasm")
+    ifs = [i for i, s in enumerate(qc.steps) if s[0] == qa.STEP_COND_IF]
+    assert len(ifs) == 2
+    for i in ifs:
+        assert qc.steps[i + 2][0] == qa.STEP_COND_END
+    # if (d==1): creg d is dit 1; if (c==1): dit 0
+    assert qc.steps[ifs[0]][2] == [1] and qc.ps_vals[ifs[0]] == [1]
+    assert qc.steps[ifs[1]][2] == [0] and qc.ps_vals[ifs[1]] == [1]
+    # a 2-bit creg compares bit i of n with dit i
+    qc = qasm.read_from_string('OPENQASM 2.0; include "qelib1.inc"; qreg q[1]; creg c[2];'
+                               ' if (c==2) x q[0];')
+    assert qc.steps[0][2] == [0, 1] and qc.ps_vals[0] == [0, 1]
+
+
+def test_reader_errors():
+    with pytest.raises(qasm.QasmError):
+        qasm.read_from_string("OPENQASM 2.0; qreg q[1]; h q[0];")  # no qelib1
+    with pytest.raises(qasm.QasmError):
+        qasm.read_from_string('OPENQASM 2.0; include "qelib1.inc"; qreg q[1]; h q[1];')
+    with pytest.raises(qasm.QasmError):
+        qasm.read_from_string('OPENQASM 2.0; include "qelib1.inc"; qreg q[2]; foo q[0];')
+    with pytest.raises(qasm.QasmError):
+        qasm.read_from_string("OPENQASM 3.0; qreg q[1];")
+
+
+# ---------------------------------------------------------------------------
+# qasm.cpp known answers on the oracle engine (CPU)
+# ---------------------------------------------------------------------------
+def check_builtin(state, dits):
+    """qasm.cpp:59-83: a[0] holds U(0.1,0.2,0.3)|0>, b = |c0 c1> = |11>."""
+    psi2 = rz(0.2) @ ry(0.1) @ rz(0.3) @ np.array([1, 0])
+    psi3 = rz(0.2 + 3 * math.pi) @ rx(math.pi / 2) @ rz(0.1 + math.pi) @ rx(math.pi / 2) @ \
+        rz(0.3) @ np.array([1, 0])
+    m = mket(dits)
+    assert abs(abs(np.vdot(state, np.kron(psi2, m))) - 1) < 1e-12
+    assert abs(abs(np.vdot(state, np.kron(psi3, m))) - 1) < 1e-12
+    assert dits == [1, 1]
+
+
+def check_teleport(state):
+    """qasm.cpp:85-98: the last qubit ends in |0>."""
+    psi = state.reshape(2, 2, 2)
+    rho = np.einsum("abi,abj->ij", psi, psi.conj())
+    assert np.max(np.abs(rho - np.diag([1, 0]))) < 1e-12
+
+
+def known_answer_checks(run):
+    st, dits = run("builtingates.qasm")
+    check_builtin(st, dits)
+    for seed in range(8):
+        st, _ = run("teleportation.qasm", seed)
+        check_teleport(st)
+    st, _ = run("mappedgates.qasm")
+    assert np.max(np.abs(st - np.kron(H @ [1, 0], H @ [1, 0]))) < 1e-12
+    st, dits = run("nondestrmeas.qasm")
+    assert np.max(np.abs(st - mket([dits[0]]))) < 1e-12 and dits == [1]
+    st, _ = run("reset.qasm")
+    assert np.max(np.abs(st - [1, 0])) < 1e-12
+    st, _ = run("scinot.qasm")
+    phi = H @ rz(1.0e-3) @ H @ np.array([1, 0])
+    assert abs(abs(np.vdot(st, phi)) - 1) < 1e-12
+
+
+def test_known_answers_oracle(orc):
+    def run(name, seed=5489):
+        _, dits, _, st = orc.engine_run(fixture(name), reps=1, seed=seed)
+        return st, dits
+    known_answer_checks(run)
+
+
+def test_oracle_conditional_blocks(orc):
+    """IF / ELSE / END: the branch taken follows the dits; an IF without
+    its END is InvalidConditional."""
+    X = np.array([[0, 1], [1, 0]])
+    for m in (0, 1):
+        qc = qa.QCircuit(2, 1)
+        if m:
+            qc.gate(X, 0)
+        qc.measure(0, 0, destructive=False)
+        qc.cond_if([0], [1]).gate(X, 1).cond_else().gate(H, 1).cond_end()
+        _, dits, _, st = orc.engine_run(qc, reps=1, seed=3)
+        exp = np.kron(mket([m]), [0, 1] if m else H @ [1, 0])
+        assert dits == [m] and np.max(np.abs(st - exp)) < 1e-12
+    bad = qa.QCircuit(1, 1).cond_if([0], [1]).gate(X, 0)
+    with pytest.raises(RuntimeError):
+        orc.engine_run(bad, reps=1)
+
+
+# ---------------------------------------------------------------------------
+# device engine
+# ---------------------------------------------------------------------------
+gpu = pytest.mark.gpu
+
+
+@gpu
+def test_known_answers_device():
+    def run(name, seed=5489):
+        e = qa.QEngine(fixture(name), seed=seed).execute(1)
+        return e.get_state().download(), e.get_dits()
+    known_answer_checks(run)
+
+
+@gpu
+@pytest.mark.parametrize("path", ALL, ids=[os.path.basename(p) for p in ALL])
+def test_device_engine_vs_oracle_on_fixtures(orc, path):
+    qc = qasm.read_from_file(path)
+    runs = [(1, 11, -1.0), (64, 12, -1.0)]
+    if qc.nq <= 10:
+        runs.append((8, 13, 0.05))
+    for reps, seed, p in runs:
+        e = qa.QEngine(qc, seed=seed, noise_p=p).execute(reps)
+        stats, dits, probs, st = orc.engine_run(qc, reps=reps, seed=seed, noise_p=p)
+        assert e.get_stats() == stats
+        assert e.get_dits() == dits
+        assert np.max(np.abs(np.array(e.get_probs()) - np.array(probs)), initial=0) <= 1e-12
+        got = e.get_state().download()
+        assert got.shape == st.shape and np.max(np.abs(got - st)) <= 1e-12
+
+
+@gpu
+def test_device_conditional_errors():
+    X = np.array([[0, 1], [1, 0]])
+    with pytest.raises(qa.InvalidConditional):
+        qa.QEngine(qa.QCircuit(1, 1).cond_if([0], [1]).gate(X, 0))
+    with pytest.raises(qa.InvalidConditional):
+        qa.QEngine(qa.QCircuit(1, 1).cond_end())
