@@ -279,7 +279,12 @@ typedef enum qs_step_type {
                                  COND_ELSE, or else COND_END */
     QS_STEP_COND_ELSE = 11,   /* reached from the true branch: jumps past
                                  the matching COND_END */
-    QS_STEP_COND_END = 12     /* end of the conditional block */
+    QS_STEP_COND_END = 12,    /* end of the conditional block */
+    QS_STEP_CCTRL = 13,       /* cCTRL (qengine.hpp:581-630): the nctrl dits
+                                 at ctrl_off, each plus its shift mod d, must
+                                 all equal one value k; then U^k on the
+                                 targets */
+    QS_STEP_CCTRL_FAN = 14    /* cCTRL_FAN: the same, U^k on each target */
 } qs_step_type;
 
 typedef struct qs_step {

@@ -429,6 +429,20 @@ class QCircuit {
         pool_.insert(pool_.end(), ps_vals.begin(), ps_vals.end());
         return *this;
     }
+    // classically controlled gates (qcircuit.hpp:2604-2960): U^k on the
+    // targets when every control dit, plus its shift mod d, equals k
+    QCircuit& cCTRL(const cmat& U, const std::vector<idx>& ctrl_dits, const std::vector<idx>& target,
+                    std::optional<std::vector<idx>> shift = std::nullopt) {
+        return add(QS_STEP_CCTRL, U, ctrl_dits, target, shift);
+    }
+    QCircuit& cCTRL(const cmat& U, idx ctrl_dit, idx target) {
+        return add(QS_STEP_CCTRL, U, {ctrl_dit}, {target});
+    }
+    QCircuit& cCTRL_fan(const cmat& U, const std::vector<idx>& ctrl_dits,
+                        const std::vector<idx>& target,
+                        std::optional<std::vector<idx>> shift = std::nullopt) {
+        return add(QS_STEP_CCTRL_FAN, U, ctrl_dits, target, shift);
+    }
     // runtime conditional block (qcircuit.hpp cond_if / cond_else /
     // cond_end): the condition is "dit dits[j] equals vals[j] for every j",
     // the form OpenQASM's if (c==n) takes

@@ -984,6 +984,10 @@ struct Step {
     idx jump = 0;         // COND_IF: its ELSE, else its END; COND_ELSE: its END
 };
 
+bool is_cctrl(const Step& s) {
+    return s.type == QS_STEP_CCTRL || s.type == QS_STEP_CCTRL_FAN;
+}
+
 bool is_runtime(const Step& s) {
     return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE ||
            s.type == QS_STEP_COND_END;
@@ -1009,6 +1013,8 @@ std::vector<idx> c_regs_of(const Step& s) {
     if (is_measurement(s)) {
         c.resize(s.target.size());
         std::iota(c.begin(), c.end(), s.c_reg);
+    } else if (is_cctrl(s)) {
+        c = s.ctrl;  // a cCTRL's control dits (qcircuit.hpp:7420-7427)
     }
     return c;
 }
@@ -1024,7 +1030,8 @@ bool can_swap(const Step& a, const Step& b) {
         return std::find_first_of(x.begin(), x.end(), y.begin(), y.end()) !=
                x.end();
     };
-    if (overlap(a.ctrl, b.ctrl)) return false;
+    const std::vector<idx> none;
+    if (overlap(is_cctrl(a) ? none : a.ctrl, is_cctrl(b) ? none : b.ctrl)) return false;
     if (overlap(targets_of(a), targets_of(b))) return false;
     if (overlap(c_regs_of(a), c_regs_of(b))) return false;
     return true;
@@ -1039,6 +1046,17 @@ std::vector<Step> canonical_form(const std::vector<Step>& steps) {
     auto first = std::find_if(steps.begin(), steps.end(), is_measurement);
     std::vector<Step> before(steps.begin(), first);
     std::vector<Step> rest(first, steps.end());
+    // cCTRL steps to the end first (:7580-7595), then the measurements
+    for (std::ptrdiff_t ri = static_cast<std::ptrdiff_t>(rest.size()) - 1;
+         ri >= 0; --ri) {
+        if (!is_cctrl(rest[ri])) continue;
+        for (std::size_t after = ri + 1; after < rest.size(); ++after) {
+            if (can_swap(rest[after - 1], rest[after]) && !is_cctrl(rest[after]))
+                std::swap(rest[after - 1], rest[after]);
+            else
+                break;
+        }
+    }
     for (std::ptrdiff_t ri = static_cast<std::ptrdiff_t>(rest.size()) - 1;
          ri >= 0; --ri) {
         if (!is_measurement(rest[ri])) continue;
@@ -1263,8 +1281,28 @@ struct Engine {
             reset_step(s);
         else if (s.type == QS_STEP_DISCARD)
             discard_step(s);
+        else if (is_cctrl(s))
+            cctrl_step(s);
         else if (s.type != QS_STEP_NOP && !is_runtime(s))
             gate_step(s);
+    }
+
+    // classically controlled gate, qengine.hpp:581-630: the control dits
+    // (each plus its shift mod d) must agree on k; then powm(U, k) on the
+    // targets (cCTRL_FAN: on each target)
+    void cctrl_step(const Step& s) {
+        const auto trel = rel(s.target);
+        auto val = [&](idx m) {
+            return s.shift ? (st.dits[s.ctrl[m]] + (*s.shift)[m]) % d : st.dits[s.ctrl[m]];
+        };
+        const idx k = val(0);
+        for (idx m = 1; m < s.ctrl.size(); ++m)
+            if (val(m) != k) return;
+        const Mat U = powm(mats[s.gate], k);
+        if (s.type == QS_STEP_CCTRL_FAN)
+            for (idx t : trel) st.qstate = apply_any(st.qstate, is_rho, U, {t}, st.dims);
+        else
+            st.qstate = apply_any(st.qstate, is_rho, U, trel, st.dims);
     }
 
     // visit_runtime_step_, qengine.hpp:826-870 (IF / ELSE / ENDIF): the

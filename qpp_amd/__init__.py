@@ -589,7 +589,7 @@ class Rng:
 # ---------------------------------------------------------------------------
 (STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY,
  STEP_RESET, STEP_DISCARD, STEP_POST_SELECT, STEP_COND_IF, STEP_COND_ELSE,
- STEP_COND_END) = range(13)
+ STEP_COND_END, STEP_CCTRL, STEP_CCTRL_FAN) = range(15)
 
 
 class _Step(ctypes.Structure):
@@ -657,6 +657,23 @@ class QCircuit:
             kind = STEP_MEASURE_MANY
         c = t[0] if c_reg is None else int(c_reg)
         self.steps.append((kind, 0, [], t, None, c, int(bool(destructive))))
+        return self
+
+    def cCTRL(self, U, ctrl_dits, target, shift=None):
+        """QCircuit::cCTRL (qcircuit.hpp:2604-2960; qengine.hpp:581-630):
+        U^k on the target(s) when every control dit (plus its shift mod d)
+        equals k."""
+        c = [int(ctrl_dits)] if np.isscalar(ctrl_dits) else list(ctrl_dits)
+        t = [int(target)] if np.isscalar(target) else list(target)
+        self.steps.append((STEP_CCTRL, self._mat(U), c, t, None if shift is None else list(shift),
+                           0, 0))
+        return self
+
+    def cCTRL_fan(self, U, ctrl_dits, targets, shift=None):
+        """QCircuit::cCTRL_fan: U^k on each target."""
+        c = [int(ctrl_dits)] if np.isscalar(ctrl_dits) else list(ctrl_dits)
+        self.steps.append((STEP_CCTRL_FAN, self._mat(U), c, list(targets),
+                           None if shift is None else list(shift), 0, 0))
         return self
 
     def post_select(self, target, ps_vals, c_reg=None, destructive=True):

@@ -63,6 +63,8 @@ struct EStep {
     size_t jump = 0;           // COND_IF: its COND_ELSE or else COND_END; COND_ELSE: its COND_END
 };
 
+bool is_cctrl(const EStep& s) { return s.type == QS_STEP_CCTRL || s.type == QS_STEP_CCTRL_FAN; }
+
 bool is_runtime(const EStep& s) {
     return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE || s.type == QS_STEP_COND_END;
 }
@@ -83,6 +85,8 @@ std::vector<uint64_t> c_regs_of(const EStep& s) {
     if (is_meas(s)) {
         c.resize(s.target.size());
         std::iota(c.begin(), c.end(), s.c_reg);
+    } else if (is_cctrl(s)) {
+        c = s.ctrl;  // its control dits (qcircuit.hpp:7420-7427)
     }
     return c;
 }
@@ -97,7 +101,9 @@ bool can_swap(const EStep& a, const EStep& b) {
     const std::vector<uint64_t> none;
     const auto& ta = is_reset_or_discard(a) ? none : a.target;
     const auto& tb = is_reset_or_discard(b) ? none : b.target;
-    return !overlap(a.ctrl, b.ctrl) && !overlap(ta, tb) && !overlap(c_regs_of(a), c_regs_of(b));
+    const auto& ca = is_cctrl(a) ? none : a.ctrl;  // a cCTRL's controls are dits
+    const auto& cb = is_cctrl(b) ? none : b.ctrl;
+    return !overlap(ca, cb) && !overlap(ta, tb) && !overlap(c_regs_of(a), c_regs_of(b));
 }
 
 // canonical_form, classes/qcircuit.hpp:7556-7612 (a circuit with runtime
@@ -106,6 +112,16 @@ std::vector<EStep> canonical_form(const std::vector<EStep>& steps) {
     if (std::any_of(steps.begin(), steps.end(), is_runtime)) return steps;
     auto first = std::find_if(steps.begin(), steps.end(), is_meas);
     std::vector<EStep> before(steps.begin(), first), rest(first, steps.end());
+    // cCTRL steps first (:7580-7595), then measurements
+    for (std::ptrdiff_t ri = static_cast<std::ptrdiff_t>(rest.size()) - 1; ri >= 0; --ri) {
+        if (!is_cctrl(rest[ri])) continue;
+        for (size_t after = ri + 1; after < rest.size(); ++after) {
+            if (can_swap(rest[after - 1], rest[after]) && !is_cctrl(rest[after]))
+                std::swap(rest[after - 1], rest[after]);
+            else
+                break;
+        }
+    }
     for (std::ptrdiff_t ri = static_cast<std::ptrdiff_t>(rest.size()) - 1; ri >= 0; --ri) {
         if (!is_meas(rest[ri])) continue;
         for (size_t after = ri + 1; after < rest.size(); ++after) {
@@ -126,6 +142,7 @@ struct GateApp {
     int gate;
     std::vector<uint64_t> ctrl, target, shift;
     bool has_shift;
+    std::vector<double> own;  // the matrix, when it is a power U^k (k > 1) of gate
 };
 
 const double kPauli[3][8] = {
@@ -161,7 +178,8 @@ struct qs_engine {
     // measured qudit is QuditAlreadyMeasured
     bool on_measured(const EStep& x) const {
         if (is_runtime(x)) return false;
-        for (const auto* v : {&x.target, &x.ctrl})
+        const std::vector<uint64_t> none;
+        for (const auto* v : {&x.target, is_cctrl(x) ? &none : &x.ctrl})
             for (uint64_t q : *v)
                 if (was_measured(q)) return true;
         return false;
@@ -213,6 +231,7 @@ struct qs_engine {
         for (const GateApp& a : gl) {
             uint64_t dim;
             const double* m = mat(a.gate, dim);
+            if (!a.own.empty()) m = a.own.data();
             const uint64_t* sh = a.has_shift ? a.shift.data() : nullptr;
             int rc = QS_OK;
             switch (a.type) {
@@ -248,6 +267,7 @@ struct qs_engine {
         for (const GateApp& a : gl) {
             const bool fan = a.type == QS_STEP_FAN || a.type == QS_STEP_CTRL_FAN;
             if (!fan && a.target.size() > 2) fusible = false;
+            if (!a.own.empty()) fusible = false;
         }
         if (!fusible) return run_gate_by_gate(gl);
         std::string key;
@@ -318,6 +338,44 @@ struct qs_engine {
         else
             qs_program_destroy(prog);
         return rc;
+    }
+
+    // cCTRL / cCTRL_FAN, qengine.hpp:581-630: every control dit, plus its
+    // shift mod d, must equal the first one's value k; then U^k on the
+    // targets (each target for the fan form).  Nothing is applied when the
+    // dits disagree or k = 0 (U^0 = I leaves the state as it is).
+    void cctrl_apps(const EStep& s, std::vector<GateApp>& out) const {
+        auto val = [&](size_t m) {
+            return s.has_shift ? (dits[s.ctrl[m]] + s.shift[m]) % d : dits[s.ctrl[m]];
+        };
+        const uint64_t k = val(0);
+        for (size_t m = 1; m < s.ctrl.size(); ++m)
+            if (val(m) != k) return;
+        if (k == 0) return;
+        GateApp a;
+        a.type = s.type == QS_STEP_CCTRL_FAN ? QS_STEP_FAN : QS_STEP_GATE;
+        a.gate = int(s.gate);
+        a.target = rel(s.target);
+        a.has_shift = false;
+        if (k > 1) {
+            // powm(U, k) by repeated products, column-major complex
+            const uint64_t D = mat_dim[s.gate];
+            const std::complex<double>* U =
+                reinterpret_cast<const std::complex<double>*>(mats.data() + 2 * mat_off[s.gate]);
+            std::vector<std::complex<double>> P(U, U + D * D), T(D * D);
+            for (uint64_t r = 1; r < k; ++r) {
+                for (uint64_t j = 0; j < D; ++j)
+                    for (uint64_t i = 0; i < D; ++i) {
+                        std::complex<double> acc = 0;
+                        for (uint64_t l = 0; l < D; ++l) acc += P[i + l * D] * U[l + j * D];
+                        T[i + j * D] = acc;
+                    }
+                P.swap(T);
+            }
+            const double* p = reinterpret_cast<const double*>(P.data());
+            a.own.assign(p, p + 2 * D * D);
+        }
+        out.push_back(std::move(a));
     }
 
     GateApp app_of(const EStep& s) const {
@@ -474,6 +532,8 @@ struct qs_engine {
                 }
                 if (rc) return rc;
                 if (is_meas(x)) measured = true;
+            } else if (is_cctrl(x)) {
+                cctrl_apps(x, pending);
             } else if (x.type != QS_STEP_NOP) {
                 pending.push_back(app_of(x));
             }
@@ -726,7 +786,10 @@ struct qs_engine {
         for (size_t i = 0; i < pos; ++i)
             if (s[i].type != QS_STEP_NOP) {
                 if (on_measured(s[i])) return already_measured(pre);
-                pre.push_back(app_of(s[i]));
+                if (is_cctrl(s[i]))
+                    cctrl_apps(s[i], pre);
+                else
+                    pre.push_back(app_of(s[i]));
             }
         if (int rc = run_gates(pre, true)) return rc;
         if (reps > 1 && can_sample) return execute_sampling(s, pos, reps);
@@ -776,7 +839,7 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         t.c_reg = s.c_reg;
         t.destructive = s.destructive != 0;
-        if (s.type > QS_STEP_COND_END) {
+        if (s.type > QS_STEP_CCTRL_FAN) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, W, "step type");
         }
@@ -785,7 +848,12 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
             return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::measure()", "c_reg");
         }
         bool in_range = true;
-        const uint64_t ctrl_lim = t.type == QS_STEP_COND_IF ? qc->nc : qc->nq;  // dits or qudits
+        const bool dit_ctrl = t.type == QS_STEP_COND_IF || is_cctrl(t);
+        const uint64_t ctrl_lim = dit_ctrl ? qc->nc : qc->nq;  // dits or qudits
+        if (is_cctrl(t) && t.ctrl.empty()) {
+            delete e;
+            return fail(QS_ERR_ZERO_SIZE, "qpp::QCircuit::cCTRL()", "ctrl_dits");
+        }
         for (uint64_t q : t.ctrl) in_range = in_range && q < ctrl_lim;
         for (uint64_t q : t.target) in_range = in_range && q < qc->nq;
         if (t.type == QS_STEP_COND_IF)

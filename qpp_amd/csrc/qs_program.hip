@@ -48,7 +48,7 @@ int lower_steps(const qs_circuit* qc, uint32_t b, uint32_t e, bool rho,
             st.type == QS_STEP_DISCARD)
             return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::program", "measurement step in a program");
         if (st.type > QS_STEP_CTRL_FAN)
-            return fail(QS_ERR_OUT_OF_RANGE, "qpp_amd::program", "step type");
+            return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::program", "runtime or classically controlled step");
         const uint64_t* tg = qc->pool + st.target_off;
         const uint64_t* cg = qc->pool + st.ctrl_off;
         const uint64_t* sh = st.shift_off == UINT32_MAX ? nullptr : qc->pool + st.shift_off;

@@ -220,15 +220,26 @@ def test_reset_errors():
                          qa.reset(psi, [0], [2, 3], 0.4))) == 0
 
 
-def random_step_circuit(n, nc, seed, d=2, n_ops=14, tail=True):
+def random_step_circuit(n, nc, seed, d=2, n_ops=14, tail=True, cctrl=False):
     """Gates, controlled gates, measurements (destructive or not),
     post-selections with random values, resets and discards on live
-    qudits, c_regs in range."""
+    qudits, c_regs in range; with cctrl also classically controlled gates
+    (plain or fan, with or without shifts)."""
     rng = np.random.default_rng(seed)
     qc = qa.QCircuit(n, nc, d)
     live = list(range(n))
     for _ in range(n_ops):
-        kind = rng.choice(["g", "g", "c", "m", "ps", "reset", "discard"])
+        kind = rng.choice(["g", "g", "c", "m", "ps", "reset", "discard"] + (["cc", "cc"] if cctrl else []))
+        if kind == "cc":
+            k = int(rng.integers(1, min(2, nc) + 1))
+            dits = [int(x) for x in rng.choice(nc, k, replace=False)]
+            shift = [int(rng.integers(0, d)) for _ in dits] if rng.integers(0, 2) else None
+            if rng.integers(0, 2) and len(live) > 1:
+                t = [int(x) for x in rng.choice(live, 2, replace=False)]
+                qc.cCTRL_fan(haar(d, rng), dits, t, shift)
+            else:
+                qc.cCTRL(haar(d, rng), dits, int(rng.choice(live)), shift)
+            continue
         if kind == "c" and len(live) < 2:
             kind = "g"
         if kind == "g":
@@ -325,3 +336,65 @@ def test_engine_post_select_impossible(orc):
     e = qa.QEngine(qc, seed=3).set_ensure_post_selection(True)
     with pytest.raises(qa.OutOfRange):
         e.execute(1)
+
+
+# ---------------------------------------------------------------------------
+# classically controlled gates (qengine.hpp:581-630)
+# ---------------------------------------------------------------------------
+def teleport_cctrl(U):
+    """q0 = U|0> teleported to q2 with cCTRL corrections X^{m1} then Z^{m0}."""
+    Z = np.diag([1, -1])
+    qc = qa.QCircuit(3, 2)
+    qc.gate(U, 0).gate(H, 1).CTRL(X, 1, 2).CTRL(X, 0, 1).gate(H, 0)
+    qc.measure(0, 0).measure(1, 1)
+    return qc.cCTRL(X, 1, 2).cCTRL(Z, 0, 2)
+
+
+def test_oracle_cctrl_teleportation(orc):
+    rng = np.random.default_rng(9)
+    U = haar(2, rng)
+    psi = U @ [1, 0]
+    for seed in range(6):
+        for reps in (1, 5):
+            stats, dits, _, st = orc.engine_run(teleport_cctrl(U), reps=reps, seed=seed)
+            assert st.shape == (2,) and abs(abs(np.vdot(psi, st)) - 1) < 1e-12
+            assert sum(stats.values()) == reps
+
+
+def test_oracle_cctrl_powers_and_shifts(orc):
+    """Qutrits: q0 = |2> measured into c0, then cCTRL(Xd, c0) on q1 applies
+    Xd^k with k = (2 + shift) mod 3; two dits that disagree apply nothing."""
+    for shift, k in ((None, 2), ([0], 2), ([1], 0), ([2], 1)):
+        qc = qa.QCircuit(2, 2, 3)
+        qc.gate(Xd(3) @ Xd(3), 0).measure(0, 0, destructive=False)
+        qc.cCTRL(Xd(3), [0], 1, shift)
+        _, dits, _, st = orc.engine_run(qc, reps=1, seed=1)
+        exp = np.zeros(9)
+        exp[2 * 3 + k] = 1
+        assert dits == [2, 0] and np.max(np.abs(st - exp)) < 1e-12
+    qc = qa.QCircuit(2, 2, 3)
+    qc.gate(Xd(3), 0).measure(0, 0, destructive=False).cCTRL(Xd(3), [0, 1], 1)
+    _, dits, _, st = orc.engine_run(qc, reps=1, seed=1)
+    exp = np.zeros(9)
+    exp[1 * 3] = 1
+    assert dits == [1, 0] and np.max(np.abs(st - exp)) < 1e-12
+
+
+@gpu
+def test_engine_cctrl_vs_oracle(orc):
+    rng = np.random.default_rng(9)
+    U = haar(2, rng)
+    for seed in range(4):
+        _compare(orc, teleport_cctrl(U), 1, seed)
+        _compare(orc, teleport_cctrl(U), 20, seed)
+    for seed in range(10):
+        n = 3 + seed % 4
+        qc = random_step_circuit(n, n, 500 + seed, cctrl=True)
+        _compare(orc, qc, 1, seed)
+        _compare(orc, qc, 25, seed, ensure=True, max_ps=2)
+    qc = random_step_circuit(13, 13, 600, n_ops=20, cctrl=True)
+    _compare(orc, qc, 3, 1)
+    qc = random_step_circuit(3, 3, 601, d=3, cctrl=True)
+    _compare(orc, qc, 1, 2)
+    _compare(orc, qc, 10, 3)
+    _compare(orc, random_step_circuit(4, 4, 602, cctrl=True), 6, 4, noise_p=0.1)
