@@ -585,6 +585,20 @@ class QEngine {
         check(qs_engine_get_state(e_, &s));
         return State(s).download();
     }
+    // QEngineT::set_state (qengine.hpp:1195-1216): a state over the
+    // non-measured qudits
+    QEngine& set_state(const cmat& psi) {
+        const idx D = psi.rows();
+        State s(psi, std::vector<idx>{D});
+        check(qs_engine_set_state(e_, s.get()));
+        return *this;
+    }
+    idx get_dit(idx i) const {
+        const auto d = get_dits();
+        if (i >= d.size())
+            throw exception::OutOfRange("[WHERE: qpp::QEngineT::get_dit()] Argument out of range! [CONTEXT: i]");
+        return d[i];
+    }
     QEngine& reset() {
         check(qs_engine_reset(e_));
         ensure_ = false;

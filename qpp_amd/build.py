@@ -75,7 +75,30 @@ def build(verbose=False, jobs=None):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_pyqpp(verbose)
     return LIB
+
+
+def pyqpp_path():
+    import sysconfig
+    return os.path.join(HERE, "pyqpp" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_pyqpp(verbose=False):
+    """The pybind11 module qpp_amd.pyqpp (host C++ over libqpp_amd.so)."""
+    import sysconfig
+    import pybind11
+    out = pyqpp_path()
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-fvisibility=hidden",
+           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(CSRC, "pyqpp.cpp"), "-L", HERE, "-lqpp_amd",
+           "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":
