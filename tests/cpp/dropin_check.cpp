@@ -233,6 +233,48 @@ int main() {
         CHECK(stats == ref, "QEngine statistics");
         CHECK(e.get_dits() == std::vector<idx>(dits.begin(), dits.end()), "QEngine dits");
     }
+    {  // qpp::reset, and circuits with cCTRL, post-selection and if/else
+        std::vector<cplx> ref(D);
+        std::vector<uint64_t> t64{3, 6};
+        orc_reset(dp(psi), 0, n, dims64.data(), 2, t64.data(), 0.37,
+                  reinterpret_cast<double*>(ref.data()));
+        CHECK(maxdiff(reset(psi, {3, 6}, dims, 0.37), ref) <= 1e-12, "reset");
+        cmat X(2, 2), Z(2, 2), H(2, 2);
+        X(0, 1) = X(1, 0) = 1;
+        Z(0, 0) = 1;
+        Z(1, 1) = -1;
+        H(0, 0) = H(0, 1) = H(1, 0) = 1 / std::sqrt(2.0);
+        H(1, 1) = -1 / std::sqrt(2.0);
+        QCircuit qc(3, 3);
+        qc.gate(haar(2, g), 0).gate(H, 1).CTRL(X, 1, 2).CTRL(X, 0, 1).gate(H, 0);
+        qc.measure(0, 0).measure(1, 1).cCTRL(X, 1, 2).cCTRL(Z, 0, 2);
+        qc.gate(H, 2).post_select(2, 1, 2, false);
+        qc.cond_if({2}, {1}).gate(X, 2).cond_else().gate(Z, 2).cond_end();
+        const auto& prog = qc.program();
+        for (idx reps : {idx{1}, idx{40}}) {
+            QEngine e(qc, 99);
+            e.set_ensure_post_selection(true).set_max_post_selection_reps(3);
+            e.execute(reps);
+            uint64_t nout = 0;
+            std::vector<uint64_t> od(64 * 3), oc(64), dits(3);
+            std::vector<double> probs(3), st(2 * 2);
+            uint64_t st_len = 0;
+            int ok = 0;
+            orc_engine_run_ps(&prog, 0, -1.0, 99, reps, 1, 3, 64, &nout, od.data(), oc.data(),
+                              dits.data(), probs.data(), st.data(), &st_len, &ok);
+            std::map<std::vector<idx>, idx> ref_stats;
+            for (uint64_t i = 0; i < nout; ++i)
+                ref_stats[std::vector<idx>(od.begin() + i * 3, od.begin() + (i + 1) * 3)] = oc[i];
+            const cmat fin = e.get_state();
+            std::vector<cplx> fin_ref(reinterpret_cast<cplx*>(st.data()),
+                                      reinterpret_cast<cplx*>(st.data()) + st_len);
+            CHECK(e.get_stats() == ref_stats && e.post_select_ok() == (ok != 0) &&
+                      e.get_dits() == std::vector<idx>(dits.begin(), dits.end()) &&
+                      maxdiff(fin, fin_ref) <= 1e-12,
+                  reps == 1 ? "engine: cCTRL, post-selection, if/else (reps 1)"
+                            : "engine: cCTRL, post-selection, if/else (reps 40)");
+        }
+    }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;
 }
