@@ -284,7 +284,14 @@ typedef enum qs_step_type {
                                  at ctrl_off, each plus its shift mod d, must
                                  all equal one value k; then U^k on the
                                  targets */
-    QS_STEP_CCTRL_FAN = 14    /* cCTRL_FAN: the same, U^k on each target */
+    QS_STEP_CCTRL_FAN = 14,   /* cCTRL_FAN: the same, U^k on each target */
+    QS_STEP_COND_WHILE = 15   /* runtime while (cond_while, qengine.hpp:837-880):
+                                 the COND_IF condition; false jumps past the
+                                 matching COND_END, whose visit re-evaluates it
+                                 and, while true, goes back to the body.
+                                 Destructive measurements, post-selections and
+                                 discards inside a while are rejected
+                                 (InvalidConditional, as QCircuit does) */
 } qs_step_type;
 
 typedef struct qs_step {

@@ -460,6 +460,12 @@ class QCircuit {
         steps_.push_back(s);
         return *this;
     }
+    // runtime while loop (cond_while ... cond_end), same condition form
+    QCircuit& cond_while(const std::vector<idx>& dits, const std::vector<idx>& vals) {
+        cond_if(dits, vals);
+        steps_.back().type = QS_STEP_COND_WHILE;
+        return *this;
+    }
     QCircuit& cond_else() { return marker(QS_STEP_COND_ELSE); }
     QCircuit& cond_end() { return marker(QS_STEP_COND_END); }
     // reset / discard (qcircuit.hpp:3745-3940)

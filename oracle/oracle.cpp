@@ -981,7 +981,9 @@ struct Step {
     idx c_reg;
     bool destructive;
     std::vector<idx> ps;  // POST_SELECT values; COND_IF dit values
-    idx jump = 0;         // COND_IF: its ELSE, else its END; COND_ELSE: its END
+    idx jump = 0;         // COND_IF / WHILE: its ELSE, else its END; COND_ELSE:
+                          // its END; the END of a WHILE: the WHILE
+    bool loop_end = false;
 };
 
 bool is_cctrl(const Step& s) {
@@ -990,7 +992,7 @@ bool is_cctrl(const Step& s) {
 
 bool is_runtime(const Step& s) {
     return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE ||
-           s.type == QS_STEP_COND_END;
+           s.type == QS_STEP_COND_END || s.type == QS_STEP_COND_WHILE;
 }
 
 // is_measurement, classes/qcircuit.hpp:7008-7025: the projective
@@ -1305,15 +1307,22 @@ struct Engine {
             st.qstate = apply_any(st.qstate, is_rho, U, trel, st.dims);
     }
 
-    // visit_runtime_step_, qengine.hpp:826-870 (IF / ELSE / ENDIF): the
-    // position the loop continues after
-    idx runtime_next(const Step& s, idx i) const {
-        if (s.type == QS_STEP_COND_IF) {
-            for (idx j = 0; j < s.ctrl.size(); ++j)
-                if (st.dits[s.ctrl[j]] != s.ps[j]) return s.jump;
-            return i;
-        }
-        return s.type == QS_STEP_COND_ELSE ? s.jump : i;
+    bool cond_true(const Step& c) const {
+        for (idx j = 0; j < c.ctrl.size(); ++j)
+            if (st.dits[c.ctrl[j]] != c.ps[j]) return false;
+        return true;
+    }
+
+    // visit_runtime_step_, qengine.hpp:826-880 (IF / WHILE / ELSE / ENDIF /
+    // ENDWHILE): the position the loop continues after
+    idx runtime_next(const std::vector<Step>& steps_, idx i) const {
+        const Step& s = steps_[i];
+        if (s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_WHILE)
+            return cond_true(s) ? i : s.jump;
+        if (s.type == QS_STEP_COND_ELSE) return s.jump;
+        if (s.type == QS_STEP_COND_END && s.loop_end && cond_true(steps_[s.jump]))
+            return s.jump;  // back to the loop's body
+        return i;
     }
 
     // execute_steps_once_, qengine.hpp:240-294 (no runtime steps): stops
@@ -1327,7 +1336,7 @@ struct Engine {
         for (idx i = pos; i < steps_.size(); ++i) {
             if (is_measurement(steps_[i])) measured = true;
             execute_step(steps_[i]);
-            i = runtime_next(steps_[i], i);
+            i = runtime_next(steps_, i);
             if (!st.ps_ok) {
                 if (ensure) st.ensure_ps = saved_ensure;
                 return;
@@ -1717,16 +1726,24 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
         t.destructive = s.destructive != 0;
         if (s.type == QS_STEP_POST_SELECT)
             t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.ntarget);
-        if (s.type == QS_STEP_COND_IF)
+        if (s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_WHILE)
             t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.nctrl);
         e.steps.push_back(t);
     }
     // match IF / ELSE / END (QCircuit::validate_conditionals)
     std::vector<idx> open;
+    int in_while = 0;
     for (idx i = 0; i < e.steps.size(); ++i) {
         const uint32_t ty = e.steps[i].type;
+        // QCircuit rejects destructive measurements and discards in a while
+        if (in_while && ((is_measurement(e.steps[i]) && e.steps[i].destructive) ||
+                         ty == QS_STEP_DISCARD))
+            return QS_ERR_INVALID_CONDITIONAL;
         if (ty == QS_STEP_COND_IF) {
             open.push_back(i);
+        } else if (ty == QS_STEP_COND_WHILE) {
+            open.push_back(i);
+            ++in_while;
         } else if (ty == QS_STEP_COND_ELSE) {
             if (open.empty() || e.steps[open.back()].type != QS_STEP_COND_IF)
                 return QS_ERR_INVALID_CONDITIONAL;
@@ -1734,7 +1751,13 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
             open.back() = i;
         } else if (ty == QS_STEP_COND_END) {
             if (open.empty()) return QS_ERR_INVALID_CONDITIONAL;
-            e.steps[open.back()].jump = i;
+            Step& top = e.steps[open.back()];
+            top.jump = i;
+            if (top.type == QS_STEP_COND_WHILE) {
+                e.steps[i].jump = open.back();
+                e.steps[i].loop_end = true;
+                --in_while;
+            }
             open.pop_back();
         }
     }

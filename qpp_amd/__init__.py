@@ -589,7 +589,7 @@ class Rng:
 # ---------------------------------------------------------------------------
 (STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY,
  STEP_RESET, STEP_DISCARD, STEP_POST_SELECT, STEP_COND_IF, STEP_COND_ELSE,
- STEP_COND_END, STEP_CCTRL, STEP_CCTRL_FAN) = range(15)
+ STEP_COND_END, STEP_CCTRL, STEP_CCTRL_FAN, STEP_COND_WHILE) = range(16)
 
 
 class _Step(ctypes.Structure):
@@ -713,6 +713,18 @@ class QCircuit:
             raise ValueError("cond_if: one value per dit")
         self.ps_vals[len(self.steps)] = v
         self.steps.append((STEP_COND_IF, 0, d, [], None, 0, 0))
+        return self
+
+    def cond_while(self, dits, values):
+        """QCircuit::cond_while (qengine.hpp:837-880): repeat the steps up
+        to cond_end() while dit dits[j] == values[j] for every j.
+        Destructive measurements and discards are not allowed inside."""
+        d = [int(dits)] if np.isscalar(dits) else list(dits)
+        v = [int(values)] if np.isscalar(values) else list(values)
+        if len(v) != len(d):
+            raise ValueError("cond_while: one value per dit")
+        self.ps_vals[len(self.steps)] = v
+        self.steps.append((STEP_COND_WHILE, 0, d, [], None, 0, 0))
         return self
 
     def cond_else(self):

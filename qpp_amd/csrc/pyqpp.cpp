@@ -317,6 +317,13 @@ PYBIND11_MODULE(pyqpp, m) {
             },
             "Runtime conditional: true when dit dits[j] equals vals[j] for every j",
             py::arg("dits"), py::arg("vals"), py::return_value_policy::reference)
+        .def(
+            "cond_while",
+            [](QCircuit& c, const py::object& dits, const py::object& vals) -> QCircuit& {
+                return c.cond_while(idx_list(dits), idx_list(vals));
+            },
+            "Runtime while loop with the cond_if condition form", py::arg("dits"),
+            py::arg("vals"), py::return_value_policy::reference)
         .def("cond_else", &QCircuit::cond_else, py::return_value_policy::reference)
         .def("cond_end", &QCircuit::cond_end, py::return_value_policy::reference)
         .def("nop", &QCircuit::nop, py::return_value_policy::reference);
@@ -433,6 +440,9 @@ PYBIND11_MODULE(pyqpp, m) {
                     c.post_select(tgt, ps[py::int_(i)].cast<std::vector<idx>>(), c_reg, dest);
                     break;
                 case QS_STEP_COND_IF: c.cond_if(ctrl, ps[py::int_(i)].cast<std::vector<idx>>()); break;
+                case QS_STEP_COND_WHILE:
+                    c.cond_while(ctrl, ps[py::int_(i)].cast<std::vector<idx>>());
+                    break;
                 case QS_STEP_COND_ELSE: c.cond_else(); break;
                 case QS_STEP_COND_END: c.cond_end(); break;
                 default: c.nop(); break;

@@ -60,13 +60,16 @@ struct EStep {
     uint64_t c_reg = 0;
     bool destructive = true;
     std::vector<uint64_t> ps;  // POST_SELECT values, COND_IF dit values
-    size_t jump = 0;           // COND_IF: its COND_ELSE or else COND_END; COND_ELSE: its COND_END
+    size_t jump = 0;           // COND_IF / COND_WHILE: its COND_ELSE or else COND_END;
+                               // COND_ELSE: its COND_END; COND_END of a while: the COND_WHILE
+    bool loop_end = false;     // a COND_END that closes a COND_WHILE
 };
 
 bool is_cctrl(const EStep& s) { return s.type == QS_STEP_CCTRL || s.type == QS_STEP_CCTRL_FAN; }
 
 bool is_runtime(const EStep& s) {
-    return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE || s.type == QS_STEP_COND_END;
+    return s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_ELSE ||
+           s.type == QS_STEP_COND_END || s.type == QS_STEP_COND_WHILE;
 }
 
 // is_measurement, classes/qcircuit.hpp:7008-7025: measurements and
@@ -176,6 +179,12 @@ struct qs_engine {
 
     // get_relative_pos_, qengine.hpp:514-526: a step on a destructively
     // measured qudit is QuditAlreadyMeasured
+    bool cond_true(const EStep& c) const {
+        for (size_t j = 0; j < c.ctrl.size(); ++j)
+            if (dits[c.ctrl[j]] != c.ps[j]) return false;
+        return true;
+    }
+
     bool on_measured(const EStep& x) const {
         if (is_runtime(x)) return false;
         const std::vector<uint64_t> none;
@@ -507,15 +516,16 @@ struct qs_engine {
             if (noise_p >= 0) noise_apps(pending);
             const EStep& x = s[i];
             if (on_measured(x)) return already_measured(pending);
-            if (x.type == QS_STEP_COND_IF) {
+            if (x.type == QS_STEP_COND_IF || x.type == QS_STEP_COND_WHILE) {
                 // visit_runtime_step_, qengine.hpp:837-860: on false, go on
                 // after the matching ELSE (or END)
-                bool t = true;
-                for (size_t j = 0; j < x.ctrl.size(); ++j) t = t && dits[x.ctrl[j]] == x.ps[j];
-                if (!t) i = x.jump;
+                if (!cond_true(x)) i = x.jump;
             } else if (x.type == QS_STEP_COND_ELSE) {
                 i = x.jump;  // the true branch ends here (:862-866)
             } else if (x.type == QS_STEP_COND_END) {
+                // ENDWHILE (:869-880): while the loop's condition holds, go
+                // on at the first step of its body
+                if (x.loop_end && cond_true(s[x.jump])) i = x.jump;
             } else if (is_meas(x) || is_reset_or_discard(x)) {
                 if (int rc = run_gates(pending, cacheable && !pending.empty())) return rc;
                 pending.clear();
@@ -839,7 +849,7 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         t.c_reg = s.c_reg;
         t.destructive = s.destructive != 0;
-        if (s.type > QS_STEP_CCTRL_FAN) {
+        if (s.type > QS_STEP_COND_WHILE) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, W, "step type");
         }
@@ -848,7 +858,7 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
             return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::measure()", "c_reg");
         }
         bool in_range = true;
-        const bool dit_ctrl = t.type == QS_STEP_COND_IF || is_cctrl(t);
+        const bool dit_ctrl = t.type == QS_STEP_COND_IF || t.type == QS_STEP_COND_WHILE || is_cctrl(t);
         const uint64_t ctrl_lim = dit_ctrl ? qc->nc : qc->nq;  // dits or qudits
         if (is_cctrl(t) && t.ctrl.empty()) {
             delete e;
@@ -856,7 +866,7 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         for (uint64_t q : t.ctrl) in_range = in_range && q < ctrl_lim;
         for (uint64_t q : t.target) in_range = in_range && q < qc->nq;
-        if (t.type == QS_STEP_COND_IF)
+        if (t.type == QS_STEP_COND_IF || t.type == QS_STEP_COND_WHILE)
             t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.nctrl);
         if (!in_range) {
             delete e;
@@ -872,12 +882,23 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         e->steps.push_back(t);
     }
-    // match COND_IF / COND_ELSE / COND_END (QCircuit::validate_conditionals)
+    // match COND_IF / COND_WHILE / COND_ELSE / COND_END
+    // (QCircuit::cond_* and validate_conditionals, qcircuit.hpp:940-1130)
     std::vector<size_t> open;
+    int in_while = 0;
     for (size_t i = 0; i < e->steps.size(); ++i) {
         EStep& t = e->steps[i];
+        const bool destroys = ((is_meas(t) && t.destructive) || t.type == QS_STEP_DISCARD);
+        if (in_while && destroys) {
+            delete e;
+            return fail(QS_ERR_INVALID_CONDITIONAL, "qpp::QCircuit::measure()",
+                        "destructive step inside a while");
+        }
         if (t.type == QS_STEP_COND_IF) {
             open.push_back(i);
+        } else if (t.type == QS_STEP_COND_WHILE) {
+            open.push_back(i);
+            ++in_while;
         } else if (t.type == QS_STEP_COND_ELSE) {
             if (open.empty() || e->steps[open.back()].type != QS_STEP_COND_IF) {
                 delete e;
@@ -892,7 +913,13 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
                 return fail(QS_ERR_INVALID_CONDITIONAL, "qpp::QCircuit::cond_end()",
                             "No matching cond_if()");
             }
-            e->steps[open.back()].jump = i;
+            EStep& top = e->steps[open.back()];
+            top.jump = i;
+            if (top.type == QS_STEP_COND_WHILE) {
+                t.jump = open.back();
+                t.loop_end = true;
+                --in_while;
+            }
             open.pop_back();
         }
     }

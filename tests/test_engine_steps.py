@@ -398,3 +398,58 @@ def test_engine_cctrl_vs_oracle(orc):
     _compare(orc, qc, 1, 2)
     _compare(orc, qc, 10, 3)
     _compare(orc, random_step_circuit(4, 4, 602, cctrl=True), 6, 4, noise_p=0.1)
+
+
+# ---------------------------------------------------------------------------
+# runtime while loops (qengine.hpp:837-880)
+# ---------------------------------------------------------------------------
+THETA = 0.37
+
+
+def ry(t):
+    return np.array([[np.cos(t / 2), -np.sin(t / 2)], [np.sin(t / 2), np.cos(t / 2)]])
+
+
+def until_one(extra_if=False):
+    """while c0 == 0: H q0; measure q0 -> c0 (non-destructive); Ry(THETA) q1.
+    q1 counts the iterations; with extra_if, X q2 when c0 == 1 inside."""
+    qc = qa.QCircuit(3 if extra_if else 2, 1)
+    qc.cond_while([0], [0]).gate(H, 0).measure(0, 0, destructive=False).gate(ry(THETA), 1)
+    if extra_if:
+        qc.cond_if([0], [1]).gate(X, 2).cond_end()
+    return qc.cond_end()
+
+
+def test_oracle_while_loop_iterations(orc):
+    """Each iteration measures q0 of H|x> with one draw; q0 = 1 exactly
+    when u >= 1/2 (the first two basis indices carry q0 = 0 and probability
+    1/2 together).  The loop ends at the first such draw, after k
+    iterations: the state is |1> (x) Ry(k THETA)|0> up to a sign."""
+    for seed in range(10):
+        r = orc.Rng(seed)
+        k = 1
+        while r.canonical() < 0.5:
+            k += 1
+        _, dits, _, st = orc.engine_run(until_one(), reps=1, seed=seed)
+        exp = np.kron([0, 1], ry(k * THETA) @ [1, 0])
+        assert dits == [1] and abs(abs(np.vdot(exp, st)) - 1) < 1e-12
+
+
+def test_oracle_while_rejects_destructive_steps(orc):
+    bad = qa.QCircuit(2, 1).cond_while([0], [0]).gate(H, 0).measure(0, 0).cond_end()
+    with pytest.raises(RuntimeError):
+        orc.engine_run(bad, reps=1)
+    bad = qa.QCircuit(2, 1).cond_while([0], [0]).discard(0).cond_end()
+    with pytest.raises(RuntimeError):
+        orc.engine_run(bad, reps=1)
+
+
+@gpu
+def test_engine_while_vs_oracle(orc):
+    for seed in range(6):
+        _compare(orc, until_one(), 1, seed)
+        _compare(orc, until_one(extra_if=True), 12, seed)
+        _compare(orc, until_one(extra_if=True), 5, seed, noise_p=0.05)
+    bad = qa.QCircuit(2, 1).cond_while([0], [0]).gate(H, 0).measure(0, 0).cond_end()
+    with pytest.raises(qa.InvalidConditional):
+        qa.QEngine(bad)
