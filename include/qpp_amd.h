@@ -397,6 +397,12 @@ int qs_engine_reset(qs_engine* e);
 int qs_engine_set_post_selection(qs_engine* e, int ensure, uint64_t max_reps);
 /* QEngineT::post_select_ok(): 0 after a run whose post-selection failed. */
 int qs_engine_post_select_ok(const qs_engine* e, int* ok);
+/* QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:104-121, 174-176):
+ * the Kraus indices drawn before circuit step `step` (0 the identity, 1, 2, 3
+ * X, Y, Z of the depolarizing channel), one per live qubit, in ascending
+ * qubit order, for every visit of the step over all runs since creation or
+ * the last qs_engine_reset.  out may be NULL; *count gets the number. */
+int qs_engine_noise_results(const qs_engine* e, uint64_t step, uint64_t* out, uint64_t* count);
 
 /* ------------------------------------------------------------------------ */
 /* Batched noisy trajectories (one persistent launch)                        */

@@ -555,7 +555,7 @@ class QEngine {
     // QubitDepolarizingNoise(noise_p)
     explicit QEngine(const QCircuit& qc, std::uint64_t seed = 5489, double noise_p = -1,
                      qs_kind kind = QS_KET, qs_dtype dt = QS_C128)
-        : nc_(qc.get_nc()) {
+        : nc_(qc.get_nc()), nsteps_(qc.program().nsteps) {
         check(qs_engine_create(&qc.program(), kind, dt, noise_p, seed, &e_));
     }
     QEngine(const QEngine&) = delete;
@@ -624,6 +624,18 @@ class QEngine {
     }
     bool get_ensure_post_selection() const { return ensure_; }
     idx get_max_post_selection_reps() const { return max_ps_; }
+    // QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:174-176)
+    std::vector<std::vector<idx>> get_noise_results() const {
+        std::vector<std::vector<idx>> r(nsteps_);
+        for (idx i = 0; i < nsteps_; ++i) {
+            uint64_t n = 0;
+            check(qs_engine_noise_results(e_, i, nullptr, &n));
+            std::vector<uint64_t> v(n);
+            check(qs_engine_noise_results(e_, i, v.data(), &n));
+            r[i].assign(v.begin(), v.end());
+        }
+        return r;
+    }
     bool post_select_ok() const {
         int ok = 0;
         check(qs_engine_post_select_ok(e_, &ok));
@@ -634,6 +646,7 @@ class QEngine {
     bool ensure_ = false;
     idx max_ps_ = UINT64_MAX;
     idx nc_;
+    idx nsteps_;
     qs_engine* e_ = nullptr;
 };
 

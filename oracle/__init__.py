@@ -61,6 +61,7 @@ def lib():
                                         c_uint64, c_uint64, u64p, u64p, u64p, u64p, dp, dp, u64p,
                                         POINTER(c_int)]
         L.orc_reset.argtypes = [dp, c_int, c_uint32, u64p, c_uint32, u64p, c_double, dp]
+        L.orc_last_noise_results.argtypes = [c_uint64, u64p, u64p]
         _lib = L
     return _lib
 
@@ -285,6 +286,19 @@ def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1
         state = state.reshape((Dn, Dn), order="F")
     out = (stats, [int(x) for x in dits[:nc]], [float(x) for x in probs[:nc]], state)
     return out + (bool(ok.value),) if with_ps_ok else out
+
+
+def last_noise_results(nsteps):
+    """get_noise_results() of the engine the last engine_run call ran."""
+    out = []
+    for i in range(nsteps):
+        n = np.zeros(1, dtype=np.uint64)
+        lib().orc_last_noise_results(i, None, n.ctypes.data_as(POINTER(c_uint64)))
+        buf = np.zeros(max(1, int(n[0])), dtype=np.uint64)
+        lib().orc_last_noise_results(i, buf.ctypes.data_as(POINTER(c_uint64)),
+                                     n.ctypes.data_as(POINTER(c_uint64)))
+        out.append([int(x) for x in buf[: int(n[0])]])
+    return out
 
 
 class Rng:

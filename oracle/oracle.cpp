@@ -1101,6 +1101,10 @@ struct Engine {
     std::mt19937 gen;
     EngineState st;
     std::map<std::vector<idx>, idx> stats;
+    // noise_results_ (qnoisy_engine.hpp:62, 104-121): per step, the Kraus
+    // index drawn for each live qubit at each visit
+    std::vector<std::vector<idx>> noise_results;
+    idx cur_ip = 0;  // the step being executed
 
     double canonical() { return std::generate_canonical<double, 53>(gen); }
 
@@ -1249,6 +1253,7 @@ struct Engine {
         const std::vector<double> w{1 - p, p / 3, p / 3, p / 3};
         const auto cp = discrete_cp(w);
         const idx k = discrete_draw(cp, canonical());
+        noise_results[cur_ip].push_back(k);
         static const cplx I(0, 1);
         Mat K(2, 2);
         switch (k) {
@@ -1335,6 +1340,7 @@ struct Engine {
         bool measured = false;
         for (idx i = pos; i < steps_.size(); ++i) {
             if (is_measurement(steps_[i])) measured = true;
+            cur_ip = i;
             execute_step(steps_[i]);
             i = runtime_next(steps_, i);
             if (!st.ps_ok) {
@@ -1467,13 +1473,19 @@ struct Engine {
         bool can_sample = true;
         for (idx i = pos; i < s.size(); ++i)
             if (!is_measurement(s[i])) can_sample = false;
-        for (idx i = 0; i < pos; ++i) execute_step(s[i]);
+        for (idx i = 0; i < pos; ++i) {
+            cur_ip = i;
+            execute_step(s[i]);
+        }
         if (reps > 1 && can_sample)
             execute_sampling(s, pos, reps);
         else
             execute_with_retries(s, pos, reps);
     }
 };
+
+// the noise results of the last orc_engine_run(_ps) call
+std::vector<std::vector<idx>> g_last_noise;
 
 }  // namespace
 
@@ -1696,6 +1708,13 @@ int orc_engine_run(const qs_circuit* qc, int is_rho, double noise_p,
                              nullptr);
 }
 
+int orc_last_noise_results(uint64_t step, uint64_t* out, uint64_t* count) {
+    if (step >= g_last_noise.size()) return QS_ERR_OUT_OF_RANGE;
+    *count = g_last_noise[step].size();
+    if (out) std::copy(g_last_noise[step].begin(), g_last_noise[step].end(), out);
+    return 0;
+}
+
 int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
                       uint64_t seed, uint64_t reps, int ensure_ps,
                       uint64_t max_ps, uint64_t max_outcomes,
@@ -1762,6 +1781,7 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
         }
     }
     if (!open.empty()) return QS_ERR_INVALID_CONDITIONAL;
+    e.noise_results.assign(e.steps.size(), {});
     e.reset_state(std::nullopt);
     e.st.ensure_ps = ensure_ps != 0;
     e.st.max_ps = max_ps;
@@ -1770,6 +1790,7 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
     } catch (const EngineError& err) {
         return err.code;
     }
+    g_last_noise = e.noise_results;
     if (ps_ok) *ps_ok = e.st.ps_ok ? 1 : 0;
     idx k = 0;
     for (auto& kv : e.stats) {

@@ -140,6 +140,11 @@ int orc_engine_run_ps(const struct qs_circuit* qc, int is_rho, double noise_p,
                       uint64_t* stats_counts, uint64_t* dits, double* probs,
                       double* out_state, uint64_t* out_len, int* ps_ok);
 
+/* QNoisyEngineT::get_noise_results() of the engine the last
+ * orc_engine_run(_ps) call ran: step `step`'s Kraus indices (out may be
+ * NULL; *count gets the number). */
+int orc_last_noise_results(uint64_t step, uint64_t* out, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1013,6 +1013,21 @@ class QEngine:
                                               getattr(self, "_max_ps", (1 << 64) - 1)))
         return self
 
+    def get_noise_results(self):
+        """QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:174-176): per
+        circuit step, the Kraus index (0 I, 1 X, 2 Y, 3 Z) drawn for each
+        live qubit at each visit."""
+        L = lib()
+        L.qs_engine_noise_results.argtypes = [c_void_p, c_uint64, POINTER(c_uint64), POINTER(c_uint64)]
+        out = []
+        for i in range(len(self.circuit.steps)):
+            n = c_uint64()
+            _check(L.qs_engine_noise_results(self._h, i, None, byref(n)))
+            buf = (c_uint64 * max(1, n.value))()
+            _check(L.qs_engine_noise_results(self._h, i, buf, byref(n)))
+            out.append([int(buf[j]) for j in range(n.value)])
+        return out
+
     def post_select_ok(self):
         L = lib()
         L.qs_engine_post_select_ok.argtypes = [c_void_p, POINTER(c_int)]

@@ -352,6 +352,7 @@ PYBIND11_MODULE(pyqpp, m) {
             py::arg("state"), py::return_value_policy::reference)
         .def("get_stats", [](const PyEngine& e) { return stats_dict(e.e->get_stats()); })
         .def("post_select_ok", [](const PyEngine& e) { return e.e->post_select_ok(); })
+        .def("get_noise_results", [](const PyEngine& e) { return e.e->get_noise_results(); })
         .def(
             "set_ensure_post_selection",
             [](PyEngine& e, bool v) -> PyEngine& { e.e->set_ensure_post_selection(v); return e; },

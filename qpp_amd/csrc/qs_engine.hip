@@ -169,6 +169,9 @@ struct qs_engine {
     std::vector<double> probs;
     std::map<std::vector<uint64_t>, uint64_t> stats;
     std::map<std::string, qs_program*> progs;
+    // QNoisyEngineT::noise_results_ (qnoisy_engine.hpp:62, 104-121): per
+    // step, the Kraus index drawn for each live qubit at each visit
+    std::vector<std::vector<uint64_t>> noise_results;
     // internal/classes/qengine_state.hpp:66-72
     bool ps_ok = true, ensure_ps = false;
     uint64_t max_ps = std::numeric_limits<uint64_t>::max();
@@ -476,7 +479,7 @@ struct qs_engine {
 
     // noise before a step (qnoisy_engine.hpp:108-116): one draw per live
     // qubit in ascending order, k = lower_bound(cp{1-p, p/3, p/3, p/3}, u)
-    void noise_apps(std::vector<GateApp>& pending) {
+    void noise_apps(std::vector<GateApp>& pending, size_t ip) {
         const std::vector<double> w{1 - noise_p, noise_p / 3, noise_p / 3, noise_p / 3};
         const double sum = std::accumulate(w.begin(), w.end(), 0.0);
         std::vector<double> q(4), cp(4);
@@ -486,6 +489,7 @@ struct qs_engine {
         for (uint64_t qb : live()) {
             const double u = canonical();
             const int k = int(std::lower_bound(cp.begin(), cp.end(), u) - cp.begin());
+            noise_results[ip].push_back(uint64_t(k));
             if (k == 0) continue;
             GateApp a;
             a.type = QS_STEP_GATE;
@@ -513,7 +517,7 @@ struct qs_engine {
         std::vector<GateApp> pending;
         bool cacheable = noise_p < 0;
         for (size_t i = pos; i < s.size(); ++i) {
-            if (noise_p >= 0) noise_apps(pending);
+            if (noise_p >= 0) noise_apps(pending, i);
             const EStep& x = s[i];
             if (on_measured(x)) return already_measured(pending);
             if (x.type == QS_STEP_COND_IF || x.type == QS_STEP_COND_WHILE) {
@@ -928,6 +932,7 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         return fail(QS_ERR_INVALID_CONDITIONAL, "qpp::QEngineT::execute()",
                     "Missing QCircuit::cond_end() delimiter");
     }
+    e->noise_results.assign(e->steps.size(), {});
     std::vector<uint64_t> dims(qc->nq, qc->d);
     if (int rc = qs_state_create(dtype, kind, qc->nq, dims.data(), &e->st)) {
         delete e;
@@ -1018,6 +1023,19 @@ int qs_engine_reset(qs_engine* e) {
     e->ps_ok = true;
     e->ensure_ps = false;
     e->max_ps = std::numeric_limits<uint64_t>::max();
+    // qnoisy_engine.hpp:152-158 empties the list and the next execute then
+    // indexes it; here every step's list is emptied instead
+    for (auto& v : e->noise_results) v.clear();
+    return QS_OK;
+}
+
+int qs_engine_noise_results(const qs_engine* e, uint64_t step, uint64_t* out, uint64_t* count) {
+    constexpr const char* W = "qpp::QNoisyEngineT::get_noise_results()";
+    if (!e || !count) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (step >= e->noise_results.size()) return fail(QS_ERR_OUT_OF_RANGE, W, "step");
+    const auto& v = e->noise_results[step];
+    *count = v.size();
+    if (out) std::copy(v.begin(), v.end(), out);
     return QS_OK;
 }
 

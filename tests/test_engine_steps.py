@@ -284,10 +284,12 @@ def _compare(orc, qc, reps, seed, kind=qa.KET, noise_p=-1.0, ensure=False, max_p
         with pytest.raises(qa._BY_CODE[err.code]):
             e.execute(reps)
         return
+    noise = orc.last_noise_results(len(qc.steps))
     e.execute(reps)
     assert e.get_stats() == stats
     assert e.get_dits() == dits
     assert e.post_select_ok() == ok
+    assert e.get_noise_results() == noise
     assert np.max(np.abs(np.array(e.get_probs()) - np.array(probs))) <= 1e-12
     got = e.get_state().download()
     assert got.shape == st.shape and np.max(np.abs(got - st)) <= 1e-12
@@ -453,3 +455,36 @@ def test_engine_while_vs_oracle(orc):
     bad = qa.QCircuit(2, 1).cond_while([0], [0]).gate(H, 0).measure(0, 0).cond_end()
     with pytest.raises(qa.InvalidConditional):
         qa.QEngine(bad)
+
+
+# ---------------------------------------------------------------------------
+# noise results (qnoisy_engine.hpp:104-121, 174-176)
+# ---------------------------------------------------------------------------
+def test_oracle_noise_results_follow_the_stream(orc):
+    """H(0) then measure both qubits: before each step one draw per live
+    qubit (k = 0 I, 1 X, 2 Y, 3 Z from {1-p, p/3, p/3, p/3}), then one draw
+    for the measurement; the results accumulate over the reps."""
+    qc = qa.QCircuit(2, 2)
+    qc.gate(H, 0).measure([0, 1], 0)
+    p = 0.6
+    w = [1 - p, p / 3, p / 3, p / 3]
+    for seed in (1, 2, 3):
+        orc.engine_run(qc, reps=3, noise_p=p, seed=seed)
+        got = orc.last_noise_results(2)
+        r = orc.Rng(seed)
+        exp = [[], []]
+        for _ in range(3):
+            exp[0] += [orc.discrete_draw(w, r.canonical()) for _ in range(2)]
+            exp[1] += [orc.discrete_draw(w, r.canonical()) for _ in range(2)]
+            r.canonical()
+        assert got == exp
+
+
+@gpu
+def test_engine_noise_results_and_reset(orc):
+    qc = random_step_circuit(5, 5, 900, cctrl=True)
+    _compare(orc, qc, 7, 3, noise_p=0.2)
+    e = qa.QEngine(qc, noise_p=0.2, seed=3).execute(2)
+    assert sum(len(v) for v in e.get_noise_results()) > 0
+    e.reset()
+    assert all(v == [] for v in e.get_noise_results())

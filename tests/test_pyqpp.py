@@ -130,6 +130,7 @@ def test_engines_vs_oracle(orc):
         e = pyqpp.QNoisyEngine(build(pyqpp), pyqpp.QubitDepolarizingNoise(0.1), seed=seed).execute(reps)
         stats, dits, _, st = orc.engine_run(build(qa), reps=reps, seed=seed, noise_p=0.1)
         assert e.get_stats() == stats and np.max(np.abs(e.get_state() - col(st))) <= 1e-12
+        assert e.get_noise_results() == orc.last_noise_results(len(build(qa).steps))
     # post-selection controls (a rep fails after 4 missed draws) and set_state
     def bell_ps(mod):
         qc = mod.QCircuit(2, 2)
