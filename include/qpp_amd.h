@@ -395,6 +395,14 @@ int qs_engine_reset(qs_engine* e);
  * measured again (a new draw) up to max_reps attempts in all.  Defaults:
  * ensure 0, max_reps UINT64_MAX (internal/classes/qengine_state.hpp:66-72). */
 int qs_engine_set_post_selection(qs_engine* e, int ensure, uint64_t max_reps);
+/* QEngineT::set_dits (qengine.hpp:1155-1164): SizeMismatch unless n == nc.
+ * execute() starts from zero dits, as qpp's does. */
+int qs_engine_set_dits(qs_engine* e, const uint64_t* dits, uint64_t n);
+/* QEngineT::get_measured_d (qengine.hpp:1052-1062): measured[i] = 1 for a
+ * destructively measured qudit i (nq entries). */
+int qs_engine_get_measured(const qs_engine* e, uint8_t* measured);
+/* QEngineT::reset_stats: the statistics emptied, state and dits kept. */
+int qs_engine_reset_stats(qs_engine* e);
 /* QEngineT::post_select_ok(): 0 after a run whose post-selection failed. */
 int qs_engine_post_select_ok(const qs_engine* e, int* ok);
 /* QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:104-121, 174-176):

@@ -15,6 +15,7 @@
 #ifndef QPP_AMD_HPP_
 #define QPP_AMD_HPP_
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <map>
@@ -555,7 +556,7 @@ class QEngine {
     // QubitDepolarizingNoise(noise_p)
     explicit QEngine(const QCircuit& qc, std::uint64_t seed = 5489, double noise_p = -1,
                      qs_kind kind = QS_KET, qs_dtype dt = QS_C128)
-        : nc_(qc.get_nc()), nsteps_(qc.program().nsteps) {
+        : nc_(qc.get_nc()), nq_(qc.get_nq()), nsteps_(qc.program().nsteps) {
         check(qs_engine_create(&qc.program(), kind, dt, noise_p, seed, &e_));
     }
     QEngine(const QEngine&) = delete;
@@ -624,6 +625,29 @@ class QEngine {
     }
     bool get_ensure_post_selection() const { return ensure_; }
     idx get_max_post_selection_reps() const { return max_ps_; }
+    // qengine.hpp:1035-1164: measured qudits, dits, statistics
+    QEngine& set_dits(const std::vector<idx>& dits) {
+        std::vector<uint64_t> d(dits.begin(), dits.end());
+        check(qs_engine_set_dits(e_, d.data(), d.size()));
+        return *this;
+    }
+    QEngine& set_dit(idx i, idx value) {
+        auto d = get_dits();
+        if (i >= d.size())
+            throw exception::OutOfRange("[WHERE: qpp::QEngineT::set_dit()] Argument out of range! [CONTEXT: i]");
+        d[i] = value;
+        return set_dits(d);
+    }
+    std::vector<idx> get_measured_d() const { return measured_where(true); }
+    std::vector<idx> get_non_measured_d() const { return measured_where(false); }
+    bool was_measured_d(idx i) const {
+        const auto m = get_measured_d();
+        return std::find(m.begin(), m.end(), i) != m.end();
+    }
+    QEngine& reset_stats() {
+        check(qs_engine_reset_stats(e_));
+        return *this;
+    }
     // QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:174-176)
     std::vector<std::vector<idx>> get_noise_results() const {
         std::vector<std::vector<idx>> r(nsteps_);
@@ -643,9 +667,18 @@ class QEngine {
     }
 
   private:
+    std::vector<idx> measured_where(bool want) const {
+        std::vector<uint8_t> m(nq_ + 1);
+        check(qs_engine_get_measured(e_, m.data()));
+        std::vector<idx> r;
+        for (idx i = 0; i < nq_; ++i)
+            if ((m[i] != 0) == want) r.push_back(i);
+        return r;
+    }
     bool ensure_ = false;
     idx max_ps_ = UINT64_MAX;
     idx nc_;
+    idx nq_;
     idx nsteps_;
     qs_engine* e_ = nullptr;
 };

@@ -1013,6 +1013,50 @@ class QEngine:
                                               getattr(self, "_max_ps", (1 << 64) - 1)))
         return self
 
+    def set_dits(self, dits):
+        """QEngineT::set_dits (qengine.hpp:1155-1164)."""
+        L = lib()
+        L.qs_engine_set_dits.argtypes = [c_void_p, POINTER(c_uint64), c_uint64]
+        d, dp = _u64(dits)
+        _check(L.qs_engine_set_dits(self._h, dp, len(d)))
+        return self
+
+    def set_dit(self, i, value):
+        d = self.get_dits()
+        if i >= len(d):
+            raise OutOfRange("[WHERE: qpp::QEngineT::set_dit()] Argument out of range! [CONTEXT: i]")
+        d[i] = int(value)
+        return self.set_dits(d)
+
+    def get_dit(self, i):
+        d = self.get_dits()
+        if i >= len(d):
+            raise OutOfRange("[WHERE: qpp::QEngineT::get_dit()] Argument out of range! [CONTEXT: i]")
+        return d[i]
+
+    def _measured(self):
+        L = lib()
+        L.qs_engine_get_measured.argtypes = [c_void_p, POINTER(ctypes.c_uint8)]
+        m = (ctypes.c_uint8 * max(1, self.circuit.nq))()
+        _check(L.qs_engine_get_measured(self._h, m))
+        return [bool(m[i]) for i in range(self.circuit.nq)]
+
+    def get_measured_d(self):
+        """QEngineT::get_measured_d (qengine.hpp:1052-1062)."""
+        return [i for i, x in enumerate(self._measured()) if x]
+
+    def get_non_measured_d(self):
+        return [i for i, x in enumerate(self._measured()) if not x]
+
+    def was_measured_d(self, i):
+        return self._measured()[i]
+
+    def reset_stats(self):
+        L = lib()
+        L.qs_engine_reset_stats.argtypes = [c_void_p]
+        _check(L.qs_engine_reset_stats(self._h))
+        return self
+
     def get_noise_results(self):
         """QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:174-176): per
         circuit step, the Kraus index (0 I, 1 X, 2 Y, 3 Z) drawn for each

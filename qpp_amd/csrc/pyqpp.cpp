@@ -353,6 +353,21 @@ PYBIND11_MODULE(pyqpp, m) {
         .def("get_stats", [](const PyEngine& e) { return stats_dict(e.e->get_stats()); })
         .def("post_select_ok", [](const PyEngine& e) { return e.e->post_select_ok(); })
         .def("get_noise_results", [](const PyEngine& e) { return e.e->get_noise_results(); })
+        .def("get_measured_d", [](const PyEngine& e) { return e.e->get_measured_d(); })
+        .def("get_non_measured_d", [](const PyEngine& e) { return e.e->get_non_measured_d(); })
+        .def("was_measured_d", [](const PyEngine& e, idx i) { return e.e->was_measured_d(i); },
+             py::arg("i"))
+        .def(
+            "set_dit",
+            [](PyEngine& e, idx i, idx v) -> PyEngine& { e.e->set_dit(i, v); return e; },
+            py::arg("i"), py::arg("value"), py::return_value_policy::reference)
+        .def(
+            "set_dits",
+            [](PyEngine& e, const std::vector<idx>& d) -> PyEngine& { e.e->set_dits(d); return e; },
+            py::arg("dits"), py::return_value_policy::reference)
+        .def(
+            "reset_stats", [](PyEngine& e) -> PyEngine& { e.e->reset_stats(); return e; },
+            py::return_value_policy::reference)
         .def(
             "set_ensure_post_selection",
             [](PyEngine& e, bool v) -> PyEngine& { e.e->set_ensure_post_selection(v); return e; },

@@ -1029,6 +1029,26 @@ int qs_engine_reset(qs_engine* e) {
     return QS_OK;
 }
 
+int qs_engine_set_dits(qs_engine* e, const uint64_t* dits, uint64_t n) {
+    constexpr const char* W = "qpp::QEngineT::set_dits()";
+    if (!e || (n && !dits)) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (n != e->dits.size()) return fail(QS_ERR_SIZE_MISMATCH, W, "dits");
+    std::copy(dits, dits + n, e->dits.begin());
+    return QS_OK;
+}
+
+int qs_engine_get_measured(const qs_engine* e, uint8_t* measured) {
+    if (!e || !measured) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_measured_d()");
+    for (uint64_t i = 0; i < e->nq; ++i) measured[i] = e->was_measured(i) ? 1 : 0;
+    return QS_OK;
+}
+
+int qs_engine_reset_stats(qs_engine* e) {
+    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::reset_stats()");
+    e->stats.clear();
+    return QS_OK;
+}
+
 int qs_engine_noise_results(const qs_engine* e, uint64_t step, uint64_t* out, uint64_t* count) {
     constexpr const char* W = "qpp::QNoisyEngineT::get_noise_results()";
     if (!e || !count) return fail(QS_ERR_INVALID_HANDLE, W);

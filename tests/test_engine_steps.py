@@ -488,3 +488,27 @@ def test_engine_noise_results_and_reset(orc):
     assert sum(len(v) for v in e.get_noise_results()) > 0
     e.reset()
     assert all(v == [] for v in e.get_noise_results())
+
+
+@gpu
+def test_engine_bookkeeping():
+    """get_measured_d / get_non_measured_d / was_measured_d, set_dit(s),
+    get_dit and reset_stats (qengine.hpp:1035-1164), through the ctypes
+    engine and the pyqpp module."""
+    from qpp_amd import pyqpp
+    for mod in (qa, pyqpp):
+        qc = mod.QCircuit(3, 2)
+        qc.gate(X, 1).measure(1, 0).measure(2, 1, False)
+        e = mod.QEngine(qc, seed=4).execute(5)
+        assert e.get_measured_d() == [1] and e.get_non_measured_d() == [0, 2]
+        assert e.was_measured_d(1) and not e.was_measured_d(0)
+        assert e.get_dits() == [1, 0] and e.get_dit(0) == 1
+        e.set_dits([0, 1])
+        assert e.get_dits() == [0, 1]
+        e.set_dit(0, 1)
+        assert e.get_dits() == [1, 1]
+        with pytest.raises((qa.SizeMismatch, RuntimeError)):
+            e.set_dits([1])
+        assert e.get_stats() == {(1, 0): 5}
+        e.reset_stats()
+        assert e.get_stats() == {}
