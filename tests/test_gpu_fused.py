@@ -152,3 +152,32 @@ def test_fused_c64(qa, orc):
     # (norm drift included); the complex64 path's bar is 1e-5
     infid = 1 - abs(np.vdot(ref, out)) ** 2
     assert maxabs(out, ref) <= 5e-6 and infid <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c64", [False, True])
+def test_full_size_circuit_and_inverse(qa, c64):
+    """The benchmark workload at its real size: a 30-qubit complex128 state
+    (16 GiB) through the depth-40 random circuit (1200 Haar gates, 600 CNOTs)
+    as tile-fused passes, then through the inverse circuit (adjoints in
+    reverse order).  The state must come back to |0...0>, with every fused
+    pass keeping the norm: size-independent properties that a wrong index,
+    lane or phase anywhere in the 2^30 amplitudes would break.  The
+    complex64 path keeps its norm and overlap in FP64; its bar is the
+    precision path's (infidelity 1e-5)."""
+    from qpp_amd import workloads
+    n, depth = 30, 40
+    gates = workloads.random_layers(n, depth, seed=30)
+    inv = [(k, t, c, np.asarray(U).conj().T) for k, t, c, U in reversed(gates)]
+    dt = qa.C64 if c64 else qa.C128
+    tol_norm, tol_inf, tol_abs = (1e-5, 1e-5, 1e-3) if c64 else (1e-12, 1e-12, 1e-10)
+    st = qa.State([2] * n, dtype=dt)
+    st.set_basis(0)
+    qa.Program(n, gates, dtype=dt, reg_bits=4, low_bits=3, max_ops=48).run(st)
+    assert abs(st.norm2() - 1) < tol_norm
+    qa.Program(n, inv, dtype=dt, reg_bits=4, low_bits=3, max_ops=48).run(st)
+    ref = qa.State([2] * n)
+    ref.set_basis(0)
+    r = st.compare(ref)
+    assert abs(r["norm2_a"] - 1) < tol_norm
+    assert r["infidelity"] < tol_inf and r["max_abs"] < tol_abs, r
