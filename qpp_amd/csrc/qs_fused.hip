@@ -519,11 +519,46 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
         // load phase (top RB tile bits, no ops) comes first and the kernel
         // returns to it before storing.
         bool first = true;
+        std::vector<uint64_t> pneed(pit.size()), ptouch(pit.size());
+        for (size_t i = 0; i < pit.size(); ++i) {
+            for (uint32_t b : pit[i].need) pneed[i] |= uint64_t{1} << b;
+            for (uint32_t b : pit[i].touch) ptouch[i] |= uint64_t{1} << b;
+        }
         while (!ptodo.empty()) {
             std::set<uint32_t> R;
             std::vector<size_t> ptake = greedy(pit, ptodo, R, RB, SIZE_MAX);
             // complete R with unused tile bits, preferring high ones
             for (int b = int(m) - 1; R.size() < RB && b >= 0; --b) R.insert(uint32_t(b));
+            if (opt.improve_sets) {
+                // hill climbing on the register bits, as for the passes' tile
+                // bits: swap one for another tile bit while the phase then
+                // takes more gates (tile bits below lc stay out, so a
+                // coalesced phase stays coalesced)
+                uint64_t Rm = 0;
+                for (uint32_t b : R) Rm |= uint64_t{1} << b;
+                size_t best = take_count(pneed, ptouch, ptodo, Rm, SIZE_MAX);
+                for (bool better = true; better;) {
+                    better = false;
+                    for (uint32_t a0 = 0; a0 < m && !better; ++a0) {
+                        if (!((Rm >> a0) & 1)) continue;
+                        for (uint32_t b0 = lc; b0 < m && !better; ++b0) {
+                            if ((Rm >> b0) & 1) continue;
+                            const uint64_t R2 = (Rm & ~(uint64_t{1} << a0)) | (uint64_t{1} << b0);
+                            const size_t c = take_count(pneed, ptouch, ptodo, R2, SIZE_MAX);
+                            if (c > best) {
+                                best = c;
+                                Rm = R2;
+                                better = true;
+                            }
+                        }
+                    }
+                }
+                R.clear();
+                for (uint32_t b = 0; b < m; ++b)
+                    if ((Rm >> b) & 1) R.insert(b);
+                std::set<uint32_t> Rg = R;
+                ptake = greedy(pit, ptodo, Rg, RB, SIZE_MAX);
+            }
             FPhase ph = make_phase(R, m, RB, Sv, lc);
             if (first && !ph.coalesced) {
                 std::set<uint32_t> R0;
