@@ -536,15 +536,34 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
                 // coalesced phase stays coalesced)
                 uint64_t Rm = 0;
                 for (uint32_t b : R) Rm |= uint64_t{1} << b;
-                size_t best = take_count(pneed, ptouch, ptodo, Rm, SIZE_MAX);
+                // gates taken, then (on ties) gates on tile bits below lc
+                // taken: a phase that does them early leaves a coalesced
+                // layout for the pass's store more often
+                const uint64_t lowm = (uint64_t{1} << lc) - 1;
+                auto score = [&](uint64_t Rx) {
+                    uint64_t blocked = 0;
+                    size_t n = 0, low = 0;
+                    for (size_t id : ptodo) {
+                        if (!(ptouch[id] & blocked) && !(pneed[id] & ~Rx)) {
+                            ++n;
+                            if (pneed[id] & lowm) ++low;
+                            continue;
+                        }
+                        blocked |= ptouch[id];
+                    }
+                    return n * 1024 + low;
+                };
+                size_t best = score(Rm);
                 for (bool better = true; better;) {
                     better = false;
                     for (uint32_t a0 = 0; a0 < m && !better; ++a0) {
                         if (!((Rm >> a0) & 1)) continue;
-                        for (uint32_t b0 = lc; b0 < m && !better; ++b0) {
+                        // the first phase keeps tile bits below lc out (it
+                        // loads HBM directly); later phases may take them in
+                        for (uint32_t b0 = first ? lc : 0; b0 < m && !better; ++b0) {
                             if ((Rm >> b0) & 1) continue;
                             const uint64_t R2 = (Rm & ~(uint64_t{1} << a0)) | (uint64_t{1} << b0);
-                            const size_t c = take_count(pneed, ptouch, ptodo, R2, SIZE_MAX);
+                            const size_t c = score(R2);
                             if (c > best) {
                                 best = c;
                                 Rm = R2;
