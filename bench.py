@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 METRIC = "30q gate-apply GB/s (% HBM peak); served circuits/s + p99 at FP64-oracle tol"
 
 
@@ -695,6 +696,16 @@ def main():
         "fused_pass_GBps": round(pass_bytes / sec / 1e9, 1),
         "gate_apps_per_s": round(len(gates) / sec, 2),
         "circuits_per_s": round(1.0 / sec, 4),
+        # the arithmetic side of the same circuit: a dense complex 2x2 gate
+        # costs 14 FP64 flops per amplitude (two complex products and a sum
+        # per output), a CNOT none; rate over the circuit's wall time against
+        # MI355X's dense FP64 peak
+        "fp64_compute": {
+            "flops_per_circuit": n1q * 14 * 2 ** n,
+            "achieved_TFLOPs": round(n1q * 14 * 2 ** n / sec / 1e12, 2),
+            "peak_TFLOPs": FP64_PEAK_TFLOPS,
+            "frac": round(n1q * 14 * 2 ** n / sec / 1e12 / FP64_PEAK_TFLOPS, 4),
+        },
         "precision": precision,
         "roofline": {
             "bound": "hbm",
