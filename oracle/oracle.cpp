@@ -1426,6 +1426,11 @@ struct Engine {
                     }
                 }
                 if (failed) st.ps_ok = false;
+                // unlimited retries of a sample that never succeeds: give up
+                // (qpp loops for ever) after 2^16 attempts
+                if (failed && st.ensure_ps && st.max_ps == std::numeric_limits<idx>::max() &&
+                    attempts >= (idx{1} << 16))
+                    throw EngineError{QS_ERR_OUT_OF_RANGE};
             } while (failed && st.ensure_ps && attempts < st.max_ps);
             if (failed) continue;
             st.ps_ok = true;

@@ -730,6 +730,9 @@ struct qs_engine {
                 if (!(ensure_ps && attempts < max_ps)) {
                     ++rep;
                     attempts = 0;
+                } else if (max_ps == kUnlimited && attempts >= kPsGiveUp) {
+                    rc = fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
+                              "post-selection failed 65536 times with unlimited retries");
                 }
                 continue;
             }
@@ -756,6 +759,9 @@ struct qs_engine {
                     if (!(ensure_ps && attempts < max_ps)) {
                         ++rep;
                         attempts = 0;
+                    } else if (max_ps == kUnlimited && attempts >= kPsGiveUp) {
+                        rc = fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
+                                  "post-selection failed 65536 times with unlimited retries");
                     }
                     continue;
                 }

@@ -338,6 +338,15 @@ def test_engine_post_select_impossible(orc):
     e = qa.QEngine(qc, seed=3).set_ensure_post_selection(True)
     with pytest.raises(qa.OutOfRange):
         e.execute(1)
+    # the sampling tail (reps > 1, only measurements after the gates) gives
+    # up the same way instead of drawing for ever
+    qs = qa.QCircuit(2, 2)
+    qs.gate(H, 1).post_select(0, 1, 0).measure(1, 1)
+    with pytest.raises(RuntimeError):
+        orc.engine_run(qs, reps=4, seed=3, ensure_post_selection=True)
+    e = qa.QEngine(qs, seed=3).set_ensure_post_selection(True)
+    with pytest.raises(qa.OutOfRange):
+        e.execute(4)
 
 
 # ---------------------------------------------------------------------------
