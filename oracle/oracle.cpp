@@ -1105,6 +1105,7 @@ struct Engine {
     // index drawn for each live qubit at each visit
     std::vector<std::vector<idx>> noise_results;
     idx cur_ip = 0;  // the step being executed
+    idx loops = 0;   // while-loop repetitions in the current run
 
     double canonical() { return std::generate_canonical<double, 53>(gen); }
 
@@ -1320,13 +1321,17 @@ struct Engine {
 
     // visit_runtime_step_, qengine.hpp:826-880 (IF / WHILE / ELSE / ENDIF /
     // ENDWHILE): the position the loop continues after
-    idx runtime_next(const std::vector<Step>& steps_, idx i) const {
+    idx runtime_next(const std::vector<Step>& steps_, idx i) {
         const Step& s = steps_[i];
         if (s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_WHILE)
             return cond_true(s) ? i : s.jump;
         if (s.type == QS_STEP_COND_ELSE) return s.jump;
-        if (s.type == QS_STEP_COND_END && s.loop_end && cond_true(steps_[s.jump]))
+        if (s.type == QS_STEP_COND_END && s.loop_end && cond_true(steps_[s.jump])) {
+            // a loop that never ends (qpp runs it for ever) stops after 2^24
+            // repetitions in one run
+            if (++loops >= (idx{1} << 24)) throw EngineError{QS_ERR_OUT_OF_RANGE};
             return s.jump;  // back to the loop's body
+        }
         return i;
     }
 
@@ -1338,6 +1343,7 @@ struct Engine {
         const bool saved_ensure = st.ensure_ps;
         if (ensure) st.ensure_ps = true;
         bool measured = false;
+        loops = 0;
         for (idx i = pos; i < steps_.size(); ++i) {
             if (is_measurement(steps_[i])) measured = true;
             cur_ip = i;

@@ -52,6 +52,8 @@ constexpr uint64_t kMeasured = std::numeric_limits<uint64_t>::max();
 // probability (1-p)^65536, below 1e-28 for p >= 1e-3).
 constexpr uint64_t kUnlimited = std::numeric_limits<uint64_t>::max();
 constexpr uint64_t kPsGiveUp = uint64_t{1} << 16;
+// the same for a runtime while loop whose condition never turns false
+constexpr uint64_t kLoopGiveUp = uint64_t{1} << 24;
 
 struct EStep {
     uint32_t type = 0, gate = 0;
@@ -514,6 +516,7 @@ struct qs_engine {
 
     int steps_once(const std::vector<EStep>& s, size_t pos) {
         bool measured = false;
+        uint64_t loops = 0;
         std::vector<GateApp> pending;
         bool cacheable = noise_p < 0;
         for (size_t i = pos; i < s.size(); ++i) {
@@ -528,8 +531,17 @@ struct qs_engine {
                 i = x.jump;  // the true branch ends here (:862-866)
             } else if (x.type == QS_STEP_COND_END) {
                 // ENDWHILE (:869-880): while the loop's condition holds, go
-                // on at the first step of its body
-                if (x.loop_end && cond_true(s[x.jump])) i = x.jump;
+                // on at the first step of its body.  A loop whose condition
+                // never turns false would run for ever (as in qpp); it stops
+                // here after kLoopGiveUp repetitions in one run.
+                if (x.loop_end && cond_true(s[x.jump])) {
+                    if (++loops >= kLoopGiveUp) {
+                        run_gates(pending, false);
+                        return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
+                                    "while loop repeated 2^24 times");
+                    }
+                    i = x.jump;
+                }
             } else if (is_meas(x) || is_reset_or_discard(x)) {
                 if (int rc = run_gates(pending, cacheable && !pending.empty())) return rc;
                 pending.clear();

@@ -464,6 +464,11 @@ def test_engine_while_vs_oracle(orc):
     bad = qa.QCircuit(2, 1).cond_while([0], [0]).gate(H, 0).measure(0, 0).cond_end()
     with pytest.raises(qa.InvalidConditional):
         qa.QEngine(bad)
+    # a loop that can never end (no measurement in its body) stops with an
+    # error after 2^24 repetitions instead of running for ever
+    forever = qa.QCircuit(1, 1).cond_while([0], [0]).nop().cond_end()
+    with pytest.raises(qa.OutOfRange):
+        qa.QEngine(forever).execute(1)
 
 
 # ---------------------------------------------------------------------------
