@@ -44,10 +44,13 @@ class QasmError(ValueError):
 _S2 = 1 / math.sqrt(2)
 
 
+# numpy's IEEE functions, as qpp's std::cos / std::exp: a non-finite
+# parameter gives a NaN matrix, not an exception
 def _u(theta, phi, lam):
-    c, s = math.cos(theta / 2), math.sin(theta / 2)
-    return np.array([[c, -np.exp(1j * lam) * s],
-                     [np.exp(1j * phi) * s, np.exp(1j * (phi + lam)) * c]])
+    with np.errstate(all="ignore"):
+        c, s = np.cos(theta / 2), np.sin(theta / 2)
+        return np.array([[c, -np.exp(1j * lam) * s],
+                         [np.exp(1j * phi) * s, np.exp(1j * (phi + lam)) * c]])
 
 
 def _ctrl(U):
@@ -65,17 +68,20 @@ _T = np.diag([1, np.exp(1j * math.pi / 4)])
 
 
 def _rx(t):
-    return np.array([[math.cos(t / 2), -1j * math.sin(t / 2)],
-                     [-1j * math.sin(t / 2), math.cos(t / 2)]])
+    with np.errstate(all="ignore"):
+        c, s = np.cos(t / 2), np.sin(t / 2)
+        return np.array([[c, -1j * s], [-1j * s, c]])
 
 
 def _ry(t):
-    return np.array([[math.cos(t / 2), -math.sin(t / 2)],
-                     [math.sin(t / 2), math.cos(t / 2)]], dtype=complex)
+    with np.errstate(all="ignore"):
+        c, s = np.cos(t / 2), np.sin(t / 2)
+        return np.array([[c, -s], [s, c]], dtype=complex)
 
 
 def _rz(t):
-    return np.diag([np.exp(-0.5j * t), np.exp(0.5j * t)])
+    with np.errstate(all="ignore"):
+        return np.diag([np.exp(-0.5j * t), np.exp(0.5j * t)])
 
 
 _CNOT = np.eye(4, dtype=complex)[[0, 1, 3, 2]]

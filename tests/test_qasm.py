@@ -245,3 +245,22 @@ def test_device_conditional_errors():
         qa.QEngine(qa.QCircuit(1, 1).cond_if([0], [1]).gate(X, 0))
     with pytest.raises(qa.InvalidConditional):
         qa.QEngine(qa.QCircuit(1, 1).cond_end())
+
+
+def test_include_opaque_barrier_and_expressions(tmp_path):
+    """A user include file declares a gate; barriers (top level and in gate
+    bodies) add no steps; an opaque gate cannot be applied; expressions
+    follow C++ double arithmetic (1/0 = inf, right-associative ^)."""
+    (tmp_path / "mine.inc").write_text("gate twice(t) a { U(t,0,0) a; barrier a; U(t,0,0) a; }\n")
+    prog = tmp_path / "p.qasm"
+    prog.write_text('OPENQASM 2.0;\ninclude "mine.inc";\nqreg q[2];\nbarrier q;\n'
+                    "twice(2^3^0/4) q[1];\n")
+    qc = qasm.read_from_file(str(prog))
+    assert [(s[0], s[3]) for s in qc.steps] == [(qa.STEP_GATE, [1]), (qa.STEP_GATE, [1])]
+    U = qc.mats[qc.steps[0][1]]   # 2^(3^0)/4 = 0.5
+    c, s_ = math.cos(0.25), math.sin(0.25)
+    assert np.max(np.abs(U - np.array([[c, -s_], [s_, c]]))) < 1e-15
+    with pytest.raises(qasm.QasmError, match="opaque"):
+        qasm.read_from_string("OPENQASM 2.0; opaque magic a; qreg q[1]; magic q[0];")
+    qc = qasm.read_from_string("OPENQASM 2.0; qreg q[1]; U(1/0, 0, 0) q[0];")
+    assert np.all(np.isnan(qc.mats[0]) | np.isfinite(qc.mats[0]))
