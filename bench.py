@@ -588,7 +588,7 @@ def main():
     if mode == "fused" and args.autotune:
         from qpp_amd import autotune
         cands = [dict(reg_bits=4, low_bits=lb, max_ops=mo, grid_cap=gc)
-                 for lb in (3, 4, 5) for mo in (48, 64, 96) for gc in (8192, 16384)]
+                 for lb in (3, 4, 5) for mo in (48, 64, 96) for gc in (16384, 32768, 65536)]
         tt = time.perf_counter()
         tuned = autotune.tune(n, gates, candidates=cands, time_budget_s=90,
                               prefix_gates=12 * (n + n // 2))

@@ -27,7 +27,7 @@ CANDIDATES = [
     for rb in (3, 4)
     for lb in (3, 4, 5, 6)
     for mo in (16, 32, 48, 64, 96)
-    for gc in (2048, 4096, 8192, 16384)
+    for gc in (2048, 4096, 8192, 16384, 32768, 65536)
 ]
 
 
