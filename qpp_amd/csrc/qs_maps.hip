@@ -56,6 +56,10 @@ __device__ __forceinline__ uint64_t gather_bits(uint64_t y, const uint8_t* src,
     return x;
 }
 
+// Largest tile: 2^12 elements, i.e. 16 per thread of a 256-thread workgroup.
+constexpr uint32_t kBpMaxTileBits = 12;
+constexpr int kBpPer = 1 << (kBpMaxTileBits - 8);
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_bitperm(const T* __restrict__ in,
                                                  T* __restrict__ out,
@@ -63,31 +67,50 @@ __global__ __launch_bounds__(256) void k_bitperm(const T* __restrict__ in,
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
     const uint32_t tsz = 1u << bp.tbits;
+    const uint32_t t = threadIdx.x;
+    // G and the LDS packing act bit by bit, so every index splits into a
+    // part from the tile number (bits outside T) and a part from the
+    // element's position e = t + 256 i inside the tile (bits in T).  The
+    // second part is the same for every tile: computed once here, for the
+    // tiles this (persistent) workgroup visits.
+    uint64_t in_off[kBpPer], out_off[kBpPer];
+    uint32_t ld_pk[kBpPer], st_pk[kBpPer];
+#pragma unroll
+    for (int i = 0; i < kBpPer; ++i) {
+        const uint32_t e = t + 256u * i;
+        in_off[i] = out_off[i] = 0;
+        ld_pk[i] = st_pk[i] = 0;
+        if (e >= tsz) continue;
+        uint64_t yl = 0, ys = 0;
+        for (uint32_t k = 0; k < bp.tbits; ++k) {
+            yl |= uint64_t((e >> k) & 1) << bp.load_out[k];
+            ys |= uint64_t((e >> k) & 1) << bp.store_out[k];
+        }
+        uint32_t pl = 0, pst = 0;
+        for (uint32_t k = 0; k < bp.tbits; ++k) {
+            pl |= uint32_t((yl >> bp.tpos[k]) & 1) << k;
+            pst |= uint32_t((ys >> bp.tpos[k]) & 1) << k;
+        }
+        in_off[i] = gather_bits(yl, bp.src, bp.nb);
+        out_off[i] = ys;
+        ld_pk[i] = pl;
+        st_pk[i] = pst;
+    }
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        // output bits outside T from the tile number
-        uint64_t ybase = 0;
-        for (uint32_t k = 0; k < bp.nrest; ++k) ybase |= ((tile >> k) & 1) << bp.rest[k];
-        for (uint32_t e = threadIdx.x; e < tsz; e += blockDim.x) {
-            uint64_t y = ybase;
-            uint32_t packed = 0;
-            for (uint32_t k = 0; k < bp.tbits; ++k) {
-                const uint32_t bit = (e >> k) & 1;
-                y |= uint64_t(bit) << bp.load_out[k];
-            }
-            for (uint32_t k = 0; k < bp.tbits; ++k)
-                packed |= uint32_t((y >> bp.tpos[k]) & 1) << k;
-            s[packed] = in[gather_bits(y, bp.src, bp.nb)];
+        // output bits outside T from the tile number, and their input bits
+        uint64_t ybase = 0, xbase = 0;
+        for (uint32_t k = 0; k < bp.nrest; ++k) {
+            const uint64_t bit = (tile >> k) & 1;
+            ybase |= bit << bp.rest[k];
+            xbase |= bit << bp.src[bp.rest[k]];
         }
+#pragma unroll
+        for (int i = 0; i < kBpPer; ++i)
+            if (t + 256u * i < tsz) s[ld_pk[i]] = in[xbase | in_off[i]];
         __syncthreads();
-        for (uint32_t e = threadIdx.x; e < tsz; e += blockDim.x) {
-            uint64_t y = ybase;
-            for (uint32_t k = 0; k < bp.tbits; ++k)
-                y |= uint64_t((e >> k) & 1) << bp.store_out[k];
-            uint32_t packed = 0;
-            for (uint32_t k = 0; k < bp.tbits; ++k)
-                packed |= uint32_t((y >> bp.tpos[k]) & 1) << k;
-            out[y] = s[packed];
-        }
+#pragma unroll
+        for (int i = 0; i < kBpPer; ++i)
+            if (t + 256u * i < tsz) out[ybase | out_off[i]] = s[st_pk[i]];
         __syncthreads();
     }
 }
@@ -259,8 +282,20 @@ int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t
     BitPerm bp{};
     bp.nb = nb;
     for (uint32_t b = 0; b < nb; ++b) bp.src[b] = static_cast<uint8_t>(src[b]);
-    const uint32_t L = std::min<uint32_t>(nb, sizeof(T) == 16 ? 5 : 6);
-    // T = low L output bits + output bits whose source is a low input bit
+    // T = low L output bits + output bits whose source is a low input bit.
+    // L starts at one 512-byte run (32 double2 / 64 float2) and grows while
+    // the tile stays within 16 KiB of LDS: when the permutation leaves the
+    // low bits in place, a 2^L tile of 32 elements would keep only 32 of
+    // the 256 threads busy.
+    const uint32_t lmin = std::min<uint32_t>(nb, sizeof(T) == 16 ? 5 : 6);
+    const uint32_t tmax = sizeof(T) == 16 ? 10 : 11;
+    auto tsize = [&](uint32_t l) {
+        uint32_t c = l;
+        for (uint32_t b = l; b < nb; ++b) c += src[b] < l;
+        return c;
+    };
+    uint32_t L = lmin;
+    while (L < nb && tsize(L + 1) <= tmax) ++L;
     std::vector<uint32_t> low_out, low_in_fed(L);
     for (uint32_t b = 0; b < L; ++b) low_out.push_back(b);
     std::vector<bool> inT(nb, false);
@@ -292,9 +327,13 @@ int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t
         if (!inT[b]) rest.push_back(b);
     bp.nrest = static_cast<uint32_t>(rest.size());
     for (size_t k = 0; k < rest.size(); ++k) bp.rest[k] = static_cast<uint8_t>(rest[k]);
+    // the kernel keeps at most 2^kBpMaxTileBits / 256 elements per thread
+    if (bp.tbits > kBpMaxTileBits) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::bitperm", "tile");
     const uint64_t ntiles = uint64_t{1} << bp.nrest;
     const size_t shmem = sizeof(T) << bp.tbits;
-    hipLaunchKernelGGL(k_bitperm<T>, dim3(grid_for(ntiles, 1)), dim3(256), shmem,
+    // persistent: at most 8 workgroups per CU, each over several tiles
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ntiles, 2048));
+    hipLaunchKernelGGL(k_bitperm<T>, dim3(grid), dim3(256), shmem,
                        in->stream, static_cast<const T*>(in->d),
                        static_cast<T*>(out->d), ntiles, bp);
     QS_HIP(hipGetLastError(), "qpp_amd::bitperm");
