@@ -24,8 +24,8 @@
 // slot (tiles streamed through registers/LDS as in k_tile; the first pass
 // synthesises |0..0> instead of loading), then samples the measurement:
 // FP64 |a|^2, a block prefix with a rigorous error bound, and -- only when
-// the bound cannot decide -- libstdc++'s exact sequential algorithm
-// (random.tcc:2656-2712).
+// the bound cannot decide -- libstdc++'s sequential sums (random.tcc:
+// 2656-2712) reproduced bit for bit by parallel binade scans (qs_exact.hpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,7 +41,7 @@
 
 #include "qs_fused.hpp"
 #include "qs_internal.hpp"
-#include "qs_serial.hpp"
+#include "qs_exact.hpp"
 #include "qs_tile_ops.hpp"
 
 namespace {
@@ -109,6 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
     __shared__ uint32_t s_k;
     __shared__ uint64_t s_idx;
     __shared__ int s_certain;
+    __shared__ qs_exact::ChunkLds<kThreads> exl;
     const uint32_t t = threadIdx.x;
     cxt<R>* __restrict__ a = ws + uint64_t(blockIdx.x) * slot;
     // work queue: trajectories in `order` (largest first), claimed one at a
@@ -256,24 +257,12 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         if (!s_certain) {
             // libstdc++ discrete_distribution: sequential sum, normalise,
             // sequential partial sums, last := 1, lower_bound
-            // (random.tcc:2656-2712), replayed by the whole workgroup with
-            // the tile area of LDS as staging
-            double* buf = reinterpret_cast<double*>(smem_raw);
-            // the tile area holds 2 x (192 * PER) doubles
-            constexpr uint32_t PER = uint32_t(sizeof(cxt<R>) * kThreads * NR / 16) / qs_serial::kLoaders;
-            const double S = qs_serial::sum<PER>(N, buf, [&](uint64_t i) { return pnorm(a[i ^ xm]); });
-            if (t == 0) pre[0] = S;
-            __syncthreads();
-            const double Sv = pre[0];
-            bool pending = true;
-            uint64_t found = N - 1;
-            qs_serial::scan<PER>(
-                N, buf, [&](uint64_t i) { return __ddiv_rn(pnorm(a[i ^ xm]), Sv); },
-                [&]() { return pending ? u_traj : 2.0; },
-                [&](uint64_t k) {
-                    found = k;
-                    pending = false;
-                });
+            // (random.tcc:2656-2712), computed exactly by the workgroup's
+            // parallel binade scans (qs_exact.hpp)
+            const double S = qs_exact::wg_sum<kThreads, 8>(
+                N, [&](uint64_t i) { return pnorm(a[i ^ xm]); }, exl);
+            const uint64_t found = qs_exact::wg_lower_bound<kThreads, 8>(
+                N, [&](uint64_t i) { return __ddiv_rn(pnorm(a[i ^ xm]), S); }, u_traj, exl);
             if (t == 0) s_idx = found;
         }
         if (t == 0) {

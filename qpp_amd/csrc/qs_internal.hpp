@@ -61,8 +61,9 @@ hipStream_t default_stream();
 // (a serving lane), else the library stream
 hipStream_t current_stream();
 void set_thread_stream(hipStream_t s);
-// test hook: QPP_AMD_FORCE_EXACT=1 sends every sampled shot through the
-// exact sequential replay instead of the error-bounded parallel search
+// test hook: QPP_AMD_FORCE_EXACT=1 searches every sampled shot in the exact
+// table (the reference's sequential sums) instead of accepting the
+// error-bounded parallel search
 bool force_exact_sampling();
 // qpp::measure_seq with the sampled basis index given (qs_measure.hip)
 int measure_seq_given(qs_state* st, uint32_t ntarget, const uint64_t* target,

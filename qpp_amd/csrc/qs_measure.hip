@@ -6,16 +6,15 @@
 //   s    = p_0 + p_1 + ... (sequential),   q_k = p_k / s,
 //   cp_k = cp_{k-1} + q_k (sequential),    cp_{N-1} := 1,
 //   index = first k with cp_k >= u.
-// A sequential sum has no parallel form, so the GPU proceeds in two steps:
+// The GPU proceeds in two steps:
 //  1. k_chunk_sums / k_scan_chunks / k_search: a parallel prefix E_k of p in
-//     FP64 locates the candidate k with E_{k-1}/E < u <= E_k/E.
-//  2. The candidate is accepted when u is farther than B from both edges,
-//     where B bounds |cp_k - E_k/E| over all k (sum of the rounding errors of
-//     the sequential sum, the division and the parallel prefix; computed in
-//     bound_for()).  Shots within B of an edge are resolved by
-//     k_serial_exact, one workgroup replaying the reference's arithmetic in
-//     its order (the division is IEEE correctly rounded on gfx950, so q_k is
-//     computed in parallel; only the two running sums are serial).
+//     FP64 locates the candidate k with E_{k-1}/E < u <= E_k/E.  The
+//     candidate is accepted when u is farther than B from both edges, where
+//     B bounds |cp_k - E_k/E| over all k (bound_for()).
+//  2. When any shot is within B of an edge, the reference's table itself is
+//     built -- both sequential sums reproduced bit for bit by parallel scans
+//     of binade maps (k_ex_*, qs_exact_core.hpp) -- and every shot is
+//     searched in it.
 // Probabilities are re*re + im*im rounded as std::norm does (no FMA).
 #include <hip/hip_runtime.h>
 
@@ -26,7 +25,7 @@
 #include <vector>
 
 #include "qs_internal.hpp"
-#include "qs_serial.hpp"
+#include "qs_exact.hpp"
 
 namespace {
 
@@ -191,17 +190,6 @@ __global__ __launch_bounds__(256) void k_search(const double* __restrict__ p,
     }
 }
 
-// q_k = p_k / s with s the sequential sum; one block
-__global__ __launch_bounds__(256) void k_serial_sum(const double* __restrict__ p,
-                                                    uint64_t D, double* __restrict__ s_out) {
-    __shared__ double buf[2 * kChunk];
-    const double s = qs_serial::sum<kChunk / qs_serial::kLoaders>(D, buf, [&](uint64_t i) { return p[i]; });
-    if (threadIdx.x == 0) *s_out = s;
-}
-
-
-// The reference's cumulative table, replayed in order, for the shots sorted
-// by u (order[] gives the shot of each sorted position).  One block.
 // p[idx[s]] for every shot
 __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ p,
                                                 const uint64_t* __restrict__ idx, uint64_t n,
@@ -211,23 +199,168 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ p,
         out[s] = p[idx[s]];
 }
 
-// cp_k = cp_{k-1} + p_k / S in order, cp_{D-1} := 1; every shot in `order`
-// (sorted by u) gets the first k with cp_k >= u.  The quotients are formed
-// by the loader threads; the next threshold stays in a register.
-__global__ __launch_bounds__(256) void k_serial_exact(const double* __restrict__ p,
-                                                      uint64_t D,
-                                                      const double* __restrict__ S,
-                                                      const double* __restrict__ u,
-                                                      const uint32_t* __restrict__ order,
-                                                      uint32_t nsorted,
-                                                      uint64_t* __restrict__ idx) {
-    __shared__ double buf[2 * kChunk];
-    const double sv = *S;
-    uint32_t next = 0;
-    qs_serial::scan<kChunk / qs_serial::kLoaders>(
-        D, buf, [&](uint64_t i) { return __ddiv_rn(p[i], sv); },
-        [&]() { return next < nsorted ? u[order[next]] : 2.0; },
-        [&](uint64_t k) { idx[order[next++]] = k; });
+// ---------------------------------------------------------------------------
+// The reference's table, exactly: S = sequential sum of p, then
+// cp_k = cp_{k-1} + p_k / S (sequential), cp_{D-1} := 1.  The sums run as
+// parallel scans of binade maps (qs_exact.hpp) over blocks of kChunk
+// elements, thread t of a block holding the kExV consecutive elements
+// t*kExV ...:
+//   k_ex_block_tr  one map per block, in the binade its approximate prefix
+//                  (k_scan_chunks) falls in;
+//   k_ex_chain     one wave walks the blocks in order carrying the exact sum:
+//                  64 blocks per step when their maps apply, otherwise the
+//                  block is summed element by element (wave_run);
+//   k_ex_expand    every block rebuilds its elements' sums from its exact
+//                  entry value (wg_chunk) and writes cp;
+//   k_ex_search    lower_bound(cp, u) per shot.
+// The approximate prefix only picks the binade a block's map is built in;
+// a wrong pick sends the block to the element path and never changes a
+// result.  Element k of the summed sequence is p_k (S == nullptr) or
+// p_k / *S, the division correctly rounded as in __normalize.
+// ---------------------------------------------------------------------------
+constexpr int kExV = kChunk / 256;  // 16 elements per thread
+
+__device__ __forceinline__ double ex_elem(const double* __restrict__ p, uint64_t i,
+                                          const double* __restrict__ S, double sv) {
+    return S ? __ddiv_rn(p[i], sv) : p[i];
+}
+
+// the block's kExV elements of thread t (zeros past D)
+__device__ __forceinline__ void ex_load(const double* __restrict__ p, uint64_t D,
+                                        const double* __restrict__ S, double sv,
+                                        uint64_t b, double (&x)[kExV]) {
+    const uint64_t i0 = b * kChunk + uint64_t(threadIdx.x) * kExV;
+    if (i0 + kExV <= D) {
+        const double2* q = reinterpret_cast<const double2*>(p + i0);
+#pragma unroll
+        for (int j = 0; j < kExV / 2; ++j) {
+            const double2 v = q[j];
+            x[2 * j] = v.x;
+            x[2 * j + 1] = v.y;
+        }
+        if (S) {
+#pragma unroll
+            for (int j = 0; j < kExV; ++j) x[j] = __ddiv_rn(x[j], sv);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kExV; ++j) x[j] = i0 + j < D ? ex_elem(p, i0 + j, S, sv) : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ex_block_tr(const double* __restrict__ p, uint64_t D,
+                                                     const double* __restrict__ S,
+                                                     const double* __restrict__ prefix,
+                                                     uint64_t nb, qs_exact::Tr* __restrict__ T,
+                                                     int* __restrict__ Eb) {
+    using namespace qs_exact;
+    __shared__ Tr sh[4];
+    const double sv = S ? *S : 1.0;
+    for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const double approx = S ? prefix[b] / sv : prefix[b];
+        const int E = binade(approx);
+        double x[kExV];
+        ex_load(p, D, S, sv, b, x);
+        Tr mine = identity();
+#pragma unroll
+        for (int j = 0; j < kExV; ++j) mine = compose(mine, elem(x[j], E));
+        Tr ex, tot;
+        wg_scan<256>(mine, sh, ex, tot);
+        if (threadIdx.x == 0) {
+            T[b] = tot;
+            Eb[b] = E;
+        }
+    }
+}
+
+// One wave.  s_in[b] = exact sum before block b; s_in[nb] = the total.
+__global__ __launch_bounds__(64) void k_ex_chain(const double* __restrict__ p, uint64_t D,
+                                                 const double* __restrict__ S, uint64_t nb,
+                                                 const qs_exact::Tr* __restrict__ T,
+                                                 const int* __restrict__ Eb,
+                                                 double* __restrict__ s_in) {
+    using namespace qs_exact;
+    __shared__ double xs[kChunk];
+    const int lane = threadIdx.x;
+    const double sv = S ? *S : 1.0;
+    double s = 0.0;
+    uint64_t b = 0;
+    while (b < nb) {
+        const int E = binade(s);
+        const uint64_t m0 = mant(s);
+        const int cnt = nb - b < 64 ? int(nb - b) : 64;
+        const uint64_t j = b + lane;
+        const bool valid = lane < cnt && Eb[j] == E;
+        const Tr inc = wave_incl(valid ? T[j] : Tr{kTop, kTop}, lane);
+        const Tr ex = wave_excl(inc, lane);
+        const uint64_t mi = apply(m0, inc);
+        const unsigned long long bad = __ballot(lane < cnt && mi >= kTop);
+        const int f = bad ? __ffsll(bad) - 1 : cnt;  // blocks b .. b+f-1 apply
+        if (lane < f) s_in[j] = from_mant(apply(m0, ex), E);
+        if (f > 0) s = from_mant(apply(m0, shfl_tr(inc, f - 1)), E);
+        b += f;
+        if (f < cnt) {
+            // block b, element by element; its elements are staged in LDS
+            // first, all loads in flight together
+            if (lane == 0) s_in[b] = s;
+            const uint64_t e0 = b * kChunk;
+#pragma unroll
+            for (int g = 0; g < kChunk / 64; ++g) {
+                const uint64_t i = e0 + uint64_t(g) * 64 + lane;
+                xs[g * 64 + lane] = i < D ? ex_elem(p, i, S, sv) : 0.0;
+            }
+            __syncthreads();
+            double val = 0.0;
+            const uint64_t n = D - e0 < uint64_t(kChunk) ? D - e0 : uint64_t(kChunk);
+            for (uint64_t i0 = 0; i0 < n; i0 += 64) {
+                const int c = n - i0 < 64 ? int(n - i0) : 64;
+                s = wave_run(s, xs[i0 + lane], lane, c, val);
+            }
+            __syncthreads();  // xs is refilled by the next slow block
+            b += 1;
+        }
+    }
+    if (lane == 0) s_in[nb] = s;
+}
+
+__global__ __launch_bounds__(256) void k_ex_expand(const double* __restrict__ p, uint64_t D,
+                                                   const double* __restrict__ S,
+                                                   const double* __restrict__ s_in,
+                                                   uint64_t nb, double* __restrict__ cp) {
+    using namespace qs_exact;
+    __shared__ ChunkLds<256> L;
+    const double sv = S ? *S : 1.0;
+    for (uint64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        double x[kExV], out[kExV];
+        ex_load(p, D, S, sv, b, x);
+        (void)wg_chunk<256, kExV>(s_in[b], x, out, L);
+        const uint64_t i0 = b * kChunk + uint64_t(threadIdx.x) * kExV;
+#pragma unroll
+        for (int j = 0; j < kExV; ++j) {
+            const uint64_t i = i0 + j;
+            if (i < D) cp[i] = i == D - 1 ? 1.0 : out[j];
+        }
+        __syncthreads();  // L is reused by the next block
+    }
+}
+
+// first k with cp[k] >= u[s] (cp[D-1] == 1 > u)
+__global__ __launch_bounds__(256) void k_ex_search(const double* __restrict__ cp, uint64_t D,
+                                                   const double* __restrict__ u,
+                                                   uint64_t nshots,
+                                                   uint64_t* __restrict__ idx) {
+    const uint64_t s = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (s >= nshots) return;
+    const double us = u[s];
+    uint64_t lo = 0, hi = D;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (cp[mid] < us)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    idx[s] = lo < D ? lo : D - 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -365,28 +498,17 @@ bool check_subsys(const std::vector<uint64_t>& subsys,
     return true;
 }
 
-// Device buffers reused across calls (grown on demand).  Calls are
-// serialised on the library stream, and every user below synchronises
-// before returning, so reuse between calls is safe.
-struct Buf {
-    void* p = nullptr;
-    size_t n = 0;
-    int need(size_t bytes) {
-        if (bytes <= n) return QS_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        QS_HIP(hipMalloc(&p, bytes), "qpp_amd::scratch");
-        n = bytes;
-        return QS_OK;
-    }
-};
-thread_local Buf g_p, g_aux;
+// Scratch memory comes from the stream-ordered pool per call (hipMallocAsync
+// on the state's stream, released the same way), so nothing outlives a call
+// and no free synchronises the device.
+constexpr int kNormBlocks = 1024;
 
 int norm_complex(const qs_state* st, double* re, double* im) {
-    const int nb = 1024;
-    if (int rc = g_aux.need(sizeof(double2) * (nb + 1))) return rc;
-    auto* part = static_cast<double2*>(g_aux.p);
+    const int nb = kNormBlocks;
+    double2* part = nullptr;
+    QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&part), sizeof(double2) * (nb + 1),
+                          st->stream),
+           "qpp::norm()");
     const bool rho = st->kind == QS_RHO;
     if (st->dtype == QS_C128)
         hipLaunchKernelGGL(k_norm_partial<double2>, dim3(nb), dim3(256), 0, st->stream,
@@ -398,11 +520,14 @@ int norm_complex(const qs_state* st, double* re, double* im) {
                            int(rho), part);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, st->stream, part, nb,
                        part + nb);
-    double2 h;
-    QS_HIP(hipMemcpyAsync(&h, part + nb, sizeof(double2), hipMemcpyDeviceToHost,
-                          st->stream),
-           "qpp::norm()");
-    QS_HIP(hipStreamSynchronize(st->stream), "qpp::norm()");
+    double2 h{0, 0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&h, part + nb, sizeof(double2), hipMemcpyDeviceToHost, st->stream);
+    (void)hipFreeAsync(part, st->stream);
+    const hipError_t e2 = hipStreamSynchronize(st->stream);
+    if (e != hipSuccess) return qs::fail_hip(e, "qpp::norm()");
+    if (e2 != hipSuccess) return qs::fail_hip(e2, "qpp::norm()");
     *re = h.x;
     *im = h.y;
     return QS_OK;
@@ -449,94 +574,145 @@ double bound_for(uint64_t N) {
     return 2.0 * (2.0 * double(N) + 2.0 * lg + 2.0 * kChunk + 8.0) * eps;
 }
 
-// Indices of nshots draws.  u_dev: device uniforms.  Results in idx_dev.
-int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
-                  std::vector<uint64_t>& out_idx, double* out_prob_host = nullptr) {
-    const uint64_t D = st->D;
-    out_idx.assign(nshots, 0);
-    if (D < 2) {  // a one-element table returns 0 (random.tcc:2700)
-        if (out_prob_host) {
-            if (int rc = g_p.need(8)) return rc;
-            if (int rc = probs_device(st, static_cast<double*>(g_p.p))) return rc;
-            QS_HIP(hipMemcpyAsync(out_prob_host, g_p.p, 8, hipMemcpyDeviceToHost, st->stream),
-                   "qpp::sample()");
-            QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
-            for (uint64_t s = 1; s < nshots; ++s) out_prob_host[s] = out_prob_host[0];
-        }
-        return QS_OK;
-    }
+// Device workspace of one sample_device call, carved from one allocation
+// (256-byte aligned segments).
+struct SampleWs {
+    double* p;       // D probabilities
+    double* cp;      // D: the exact cumulative table
+    double* sums;    // nchunks block sums
+    double* prefix;  // nchunks + 1: their exclusive prefix and total
+    qs_exact::Tr* T; // nchunks block maps
+    int* Eb;         // nchunks binades of the maps
+    double* sp_in;   // nchunks + 1: exact sums of p before each block; [nchunks] = S
+    double* sq_in;   // nchunks + 1: the same for p / S
+    double* u;       // nshots uniforms (then the gathered probabilities)
+    uint64_t* idx;   // nshots indices
+    int* cert;       // nshots: decided by the approximate search
+};
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+size_t sample_ws_layout(uint64_t D, uint64_t nshots, unsigned char* base, SampleWs* w) {
     const uint64_t nchunks = (D + kChunk - 1) / kChunk;
-    // layout in g_aux: sums[nchunks] | prefix[nchunks+1] | u[nshots] |
-    // idx[nshots] | certain[nshots] | order[nshots] | s
-    const size_t off_prefix = nchunks * 8;
-    const size_t off_u = off_prefix + (nchunks + 1) * 8;
-    const size_t off_idx = off_u + nshots * 8;
-    const size_t off_cert = off_idx + nshots * 8;
-    const size_t off_order = off_cert + nshots * 4;
-    const size_t off_s = ((off_order + nshots * 4 + 15) / 16) * 16;
-    if (int rc = g_aux.need(off_s + 16)) return rc;
-    if (int rc = g_p.need(D * 8)) return rc;
-    auto* base = static_cast<unsigned char*>(g_aux.p);
-    auto* sums = reinterpret_cast<double*>(base);
-    auto* prefix = reinterpret_cast<double*>(base + off_prefix);
-    auto* ud = reinterpret_cast<double*>(base + off_u);
-    auto* idx = reinterpret_cast<uint64_t*>(base + off_idx);
-    auto* cert = reinterpret_cast<int*>(base + off_cert);
-    auto* order = reinterpret_cast<uint32_t*>(base + off_order);
-    auto* sdev = reinterpret_cast<double*>(base + off_s);
-    auto* p = static_cast<double*>(g_p.p);
-    if (int rc = probs_device(st, p)) return rc;
-    QS_HIP(hipMemcpyAsync(ud, u_host, nshots * 8, hipMemcpyHostToDevice, st->stream),
-           "qpp::sample()");
-    hipLaunchKernelGGL(k_chunk_sums, dim3(grid_for(nchunks, 1)), dim3(256), 0,
-                       st->stream, p, D, sums, nchunks);
-    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, st->stream, sums,
-                       nchunks, prefix);
-    const double B = bound_for(D);
-    hipLaunchKernelGGL(k_search, dim3(grid_for(nshots * 64, 256)), dim3(256), 0,
-                       st->stream, p, D, prefix, nchunks, ud, nshots, B, idx, cert);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* at = base ? base + off : nullptr;
+        off += align256(bytes);
+        return at;
+    };
+    SampleWs v;
+    v.p = reinterpret_cast<double*>(take(8 * D));
+    v.cp = reinterpret_cast<double*>(take(8 * D));
+    v.sums = reinterpret_cast<double*>(take(8 * nchunks));
+    v.prefix = reinterpret_cast<double*>(take(8 * (nchunks + 1)));
+    v.T = reinterpret_cast<qs_exact::Tr*>(take(sizeof(qs_exact::Tr) * nchunks));
+    v.Eb = reinterpret_cast<int*>(take(4 * nchunks));
+    v.sp_in = reinterpret_cast<double*>(take(8 * (nchunks + 1)));
+    v.sq_in = reinterpret_cast<double*>(take(8 * (nchunks + 1)));
+    v.u = reinterpret_cast<double*>(take(8 * nshots));
+    v.idx = reinterpret_cast<uint64_t*>(take(8 * nshots));
+    v.cert = reinterpret_cast<int*>(take(4 * nshots));
+    if (w) *w = v;
+    return off;
+}
+
+// Builds the exact table in w.cp (k_ex_* above) and searches every shot in
+// it.  Needs w.p, w.prefix (k_chunk_sums + k_scan_chunks) and w.u.
+int exact_table_search(const qs_state* st, const SampleWs& w, uint64_t nshots) {
+    const uint64_t D = st->D;
+    const uint64_t nb = (D + kChunk - 1) / kChunk;
+    const uint32_t g = grid_for(nb, 1);
+    hipStream_t sm = st->stream;
+    hipLaunchKernelGGL(k_ex_block_tr, dim3(g), dim3(256), 0, sm, w.p, D, nullptr, w.prefix, nb,
+                       w.T, w.Eb);
+    hipLaunchKernelGGL(k_ex_chain, dim3(1), dim3(64), 0, sm, w.p, D, nullptr, nb, w.T, w.Eb,
+                       w.sp_in);
+    const double* S = w.sp_in + nb;
+    hipLaunchKernelGGL(k_ex_block_tr, dim3(g), dim3(256), 0, sm, w.p, D, S, w.prefix, nb, w.T,
+                       w.Eb);
+    hipLaunchKernelGGL(k_ex_chain, dim3(1), dim3(64), 0, sm, w.p, D, S, nb, w.T, w.Eb, w.sq_in);
+    hipLaunchKernelGGL(k_ex_expand, dim3(g), dim3(256), 0, sm, w.p, D, S, w.sq_in, nb, w.cp);
+    hipLaunchKernelGGL(k_ex_search, dim3(grid_for(nshots, 256)), dim3(256), 0, sm, w.cp, D, w.u,
+                       nshots, w.idx);
     QS_HIP(hipGetLastError(), "qpp::sample()");
-    std::vector<int> hc(nshots);
-    QS_HIP(hipMemcpyAsync(out_idx.data(), idx, nshots * 8, hipMemcpyDeviceToHost,
-                          st->stream),
-           "qpp::sample()");
-    QS_HIP(hipMemcpyAsync(hc.data(), cert, nshots * 4, hipMemcpyDeviceToHost, st->stream),
-           "qpp::sample()");
-    QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
-    // p[index] of every shot (instruments.hpp:808), gathered on the device;
-    // the u slots are free once the search has run
-    auto probs_out = [&]() -> int {
-        if (!out_prob_host) return QS_OK;
-        hipLaunchKernelGGL(k_gather, dim3(grid_for(nshots, 256)), dim3(256), 0, st->stream, p,
-                           idx, nshots, ud);
+    return QS_OK;
+}
+
+// Indices of nshots draws from host uniforms u_host, with p[index] of each
+// in out_prob_host when given.  ws: a device workspace of at least
+// sample_ws_layout(D, nshots) bytes, or nullptr to take one from the
+// stream-ordered pool for the call.
+int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
+                  uint64_t* out_idx, double* out_prob_host, unsigned char* ws) {
+    const uint64_t D = st->D;
+    unsigned char* own = nullptr;
+    if (!ws) {
+        const size_t bytes = sample_ws_layout(D, nshots, nullptr, nullptr);
+        QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&own), bytes, st->stream), "qpp::sample()");
+        ws = own;
+    }
+    const int rc = [&]() -> int {
+        SampleWs w;
+        sample_ws_layout(D, nshots, ws, &w);
+        if (int rc = probs_device(st, w.p)) return rc;
+        if (D < 2) {  // a one-element table returns 0 (random.tcc:2700)
+            for (uint64_t s = 0; s < nshots; ++s) out_idx[s] = 0;
+            if (out_prob_host) {
+                QS_HIP(hipMemcpyAsync(out_prob_host, w.p, 8, hipMemcpyDeviceToHost, st->stream),
+                       "qpp::sample()");
+                QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+                for (uint64_t s = 1; s < nshots; ++s) out_prob_host[s] = out_prob_host[0];
+            }
+            return QS_OK;
+        }
+        const uint64_t nchunks = (D + kChunk - 1) / kChunk;
+        QS_HIP(hipMemcpyAsync(w.u, u_host, nshots * 8, hipMemcpyHostToDevice, st->stream),
+               "qpp::sample()");
+        hipLaunchKernelGGL(k_chunk_sums, dim3(grid_for(nchunks, 1)), dim3(256), 0, st->stream,
+                           w.p, D, w.sums, nchunks);
+        hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, st->stream, w.sums, nchunks,
+                           w.prefix);
+        const double B = bound_for(D);
+        hipLaunchKernelGGL(k_search, dim3(grid_for(nshots * 64, 256)), dim3(256), 0, st->stream,
+                           w.p, D, w.prefix, nchunks, w.u, nshots, B, w.idx, w.cert);
         QS_HIP(hipGetLastError(), "qpp::sample()");
-        QS_HIP(hipMemcpyAsync(out_prob_host, ud, nshots * 8, hipMemcpyDeviceToHost,
-                              st->stream),
+        std::vector<int> hc(nshots);
+        QS_HIP(hipMemcpyAsync(hc.data(), w.cert, nshots * 4, hipMemcpyDeviceToHost, st->stream),
                "qpp::sample()");
         QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
+        const bool force = qs::force_exact_sampling();
+        bool all = !force;
+        for (uint64_t s = 0; s < nshots && all; ++s) all = hc[s] != 0;
+        // shots the error bound cannot decide: every shot is searched in
+        // the exact table (a decided shot gets the same index there)
+        if (!all)
+            if (int rc = exact_table_search(st, w, nshots)) return rc;
+        QS_HIP(hipMemcpyAsync(out_idx, w.idx, nshots * 8, hipMemcpyDeviceToHost, st->stream),
+               "qpp::sample()");
+        // p[index] of every shot (instruments.hpp:808), gathered on the
+        // device; the u slots are free once the searches have run
+        if (out_prob_host) {
+            hipLaunchKernelGGL(k_gather, dim3(grid_for(nshots, 256)), dim3(256), 0, st->stream,
+                               w.p, w.idx, nshots, w.u);
+            QS_HIP(hipGetLastError(), "qpp::sample()");
+            QS_HIP(hipMemcpyAsync(out_prob_host, w.u, nshots * 8, hipMemcpyDeviceToHost,
+                                  st->stream),
+                   "qpp::sample()");
+        }
+        QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
         return QS_OK;
-    };
-    std::vector<uint32_t> amb;
-    const bool force = qs::force_exact_sampling();
-    for (uint64_t s = 0; s < nshots; ++s)
-        if (force || !hc[s]) amb.push_back(static_cast<uint32_t>(s));
-    if (amb.empty()) return probs_out();
-    // exact replay for the ambiguous shots, sorted by u
-    std::sort(amb.begin(), amb.end(), [&](uint32_t a, uint32_t b) {
-        return u_host[a] < u_host[b] || (u_host[a] == u_host[b] && a < b);
-    });
-    QS_HIP(hipMemcpyAsync(order, amb.data(), amb.size() * 4, hipMemcpyHostToDevice,
-                          st->stream),
-           "qpp::sample()");
-    hipLaunchKernelGGL(k_serial_sum, dim3(1), dim3(256), 0, st->stream, p, D, sdev);
-    hipLaunchKernelGGL(k_serial_exact, dim3(1), dim3(256), 0, st->stream, p, D, sdev, ud,
-                       order, static_cast<uint32_t>(amb.size()), idx);
-    QS_HIP(hipGetLastError(), "qpp::sample()");
-    QS_HIP(hipMemcpyAsync(out_idx.data(), idx, nshots * 8, hipMemcpyDeviceToHost,
-                          st->stream),
-           "qpp::sample()");
-    QS_HIP(hipStreamSynchronize(st->stream), "qpp::sample()");
-    return probs_out();
+    }();
+    if (own) {
+        (void)hipFreeAsync(own, st->stream);
+        if (rc != QS_OK) (void)hipStreamSynchronize(st->stream);
+    }
+    return rc;
+}
+
+int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
+                  std::vector<uint64_t>& out_idx, double* out_prob_host = nullptr) {
+    out_idx.assign(nshots, 0);
+    return sample_device(st, u_host, nshots, out_idx.data(), out_prob_host, nullptr);
 }
 
 int project(qs_state* st, uint64_t stride, uint64_t dim, uint64_t m) {
@@ -621,12 +797,18 @@ int qs_normalize(qs_state* st) {
 int qs_probs(const qs_state* st, double* host_out, uint64_t n_out) {
     if (!valid(st) || !host_out) return fail(QS_ERR_INVALID_HANDLE, "qpp::abssq()");
     if (n_out != st->D) return fail(QS_ERR_SIZE_MISMATCH, "qpp::abssq()", "n_out");
-    if (int rc = g_p.need(st->D * 8)) return rc;
-    if (int rc = probs_device(st, static_cast<double*>(g_p.p))) return rc;
-    QS_HIP(hipMemcpyAsync(host_out, g_p.p, st->D * 8, hipMemcpyDeviceToHost, st->stream),
-           "qpp::abssq()");
-    QS_HIP(hipStreamSynchronize(st->stream), "qpp::abssq()");
-    return QS_OK;
+    void* p = nullptr;
+    QS_HIP(hipMallocAsync(&p, st->D * 8, st->stream), "qpp::abssq()");
+    int rc = probs_device(st, static_cast<double*>(p));
+    if (rc == QS_OK) {
+        const hipError_t e =
+            hipMemcpyAsync(host_out, p, st->D * 8, hipMemcpyDeviceToHost, st->stream);
+        if (e != hipSuccess) rc = qs::fail_hip(e, "qpp::abssq()");
+    }
+    (void)hipFreeAsync(p, st->stream);
+    const hipError_t e = hipStreamSynchronize(st->stream);
+    if (rc == QS_OK && e != hipSuccess) rc = qs::fail_hip(e, "qpp::abssq()");
+    return rc;
 }
 
 // qpp::sample, instruments.hpp:746-929
