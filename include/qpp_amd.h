@@ -234,6 +234,19 @@ int qs_sample(const qs_state* st, uint32_t ntarget, const uint64_t* target,
               uint64_t nshots, const double* u, uint64_t* out_index,
               uint64_t* out_digits, double* out_prob);
 
+/* No counterpart in the reference.  qs_sample for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, validation, results and
+ * errors as qs_sample, but the call allocates nothing.  Its workspace is a
+ * fixed array in device memory, one per GPU (about 16.4 MiB, held until the
+ * process ends), so a call takes at most 2^20 basis states and 16384 shots
+ * (larger ones fail with QS_ERR_OUT_OF_MEMORY before anything runs).  Every
+ * shot is searched in the exact cumulative table.  Calls that overlap on one
+ * GPU share that array and race: serialise them. */
+int qs_sample_noalloc(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+                      uint64_t nshots, const double* u, uint64_t* out_index,
+                      uint64_t* out_digits, double* out_prob);
+
 /* qpp::measure_seq(A, target, dims, destructive)  instruments.hpp:985-1099,
  * in place, with one caller uniform for the sample.  results/probs: ntarget
  * entries in target order.  A destructive measurement removes the measured

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -590,29 +591,46 @@ struct SampleWs {
     int* cert;       // nshots: decided by the approximate search
 };
 
-size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+constexpr size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// byte sizes of the SampleWs segments, in member order
+constexpr int kWsSegs = 11;
+struct WsSizes {
+    size_t seg[kWsSegs];
+};
+constexpr WsSizes ws_sizes(uint64_t D, uint64_t nshots) {
+    const uint64_t nc = (D + kChunk - 1) / kChunk;
+    return WsSizes{{8 * D, 8 * D, 8 * nc, 8 * (nc + 1), sizeof(qs_exact::Tr) * nc, 4 * nc,
+                    8 * (nc + 1), 8 * (nc + 1), 8 * nshots, 8 * nshots, 4 * nshots}};
+}
+constexpr size_t ws_bytes(uint64_t D, uint64_t nshots) {
+    const WsSizes z = ws_sizes(D, nshots);
+    size_t off = 0;
+    for (int i = 0; i < kWsSegs; ++i) off += align256(z.seg[i]);
+    return off;
+}
 
 size_t sample_ws_layout(uint64_t D, uint64_t nshots, unsigned char* base, SampleWs* w) {
-    const uint64_t nchunks = (D + kChunk - 1) / kChunk;
+    const WsSizes z = ws_sizes(D, nshots);
+    unsigned char* at[kWsSegs];
     size_t off = 0;
-    auto take = [&](size_t bytes) {
-        unsigned char* at = base ? base + off : nullptr;
-        off += align256(bytes);
-        return at;
-    };
-    SampleWs v;
-    v.p = reinterpret_cast<double*>(take(8 * D));
-    v.cp = reinterpret_cast<double*>(take(8 * D));
-    v.sums = reinterpret_cast<double*>(take(8 * nchunks));
-    v.prefix = reinterpret_cast<double*>(take(8 * (nchunks + 1)));
-    v.T = reinterpret_cast<qs_exact::Tr*>(take(sizeof(qs_exact::Tr) * nchunks));
-    v.Eb = reinterpret_cast<int*>(take(4 * nchunks));
-    v.sp_in = reinterpret_cast<double*>(take(8 * (nchunks + 1)));
-    v.sq_in = reinterpret_cast<double*>(take(8 * (nchunks + 1)));
-    v.u = reinterpret_cast<double*>(take(8 * nshots));
-    v.idx = reinterpret_cast<uint64_t*>(take(8 * nshots));
-    v.cert = reinterpret_cast<int*>(take(4 * nshots));
-    if (w) *w = v;
+    for (int i = 0; i < kWsSegs; ++i) {
+        at[i] = base ? base + off : nullptr;
+        off += align256(z.seg[i]);
+    }
+    if (w) {
+        w->p = reinterpret_cast<double*>(at[0]);
+        w->cp = reinterpret_cast<double*>(at[1]);
+        w->sums = reinterpret_cast<double*>(at[2]);
+        w->prefix = reinterpret_cast<double*>(at[3]);
+        w->T = reinterpret_cast<qs_exact::Tr*>(at[4]);
+        w->Eb = reinterpret_cast<int*>(at[5]);
+        w->sp_in = reinterpret_cast<double*>(at[6]);
+        w->sq_in = reinterpret_cast<double*>(at[7]);
+        w->u = reinterpret_cast<double*>(at[8]);
+        w->idx = reinterpret_cast<uint64_t*>(at[9]);
+        w->cert = reinterpret_cast<int*>(at[10]);
+    }
     return off;
 }
 
@@ -713,6 +731,86 @@ int sample_device(const qs_state* st, const double* u_host, uint64_t nshots,
                   std::vector<uint64_t>& out_idx, double* out_prob_host = nullptr) {
     out_idx.assign(nshots, 0);
     return sample_device(st, u_host, nshots, out_idx.data(), out_prob_host, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free sampling (qs_sample_noalloc): the workspace of one call on
+// a state of at most kFixedMaxD basis states and kFixedMaxShots shots lives
+// in a file-scope device array.  Every GPU the library runs on holds its own
+// copy (about 16.4 MiB) until the process ends.  Its device address is
+// looked up once per device with hipGetSymbolAddress (the array's name is a
+// host-side stand-in, not a device address) and only that address is handed
+// to kernels and copies.  Nothing in it needs clearing: every segment is
+// written before it is read.  All calls on one GPU share the array, so calls
+// that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFixedMaxD = uint64_t{1} << 20;
+constexpr uint64_t kFixedMaxShots = uint64_t{1} << 14;
+constexpr size_t kFixedBytes = ws_bytes(kFixedMaxD, kFixedMaxShots);
+static __device__ __align__(256) unsigned char scratch[kFixedBytes];
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+unsigned char* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// the device address of `scratch` on device dev (set up on first use, with
+// dev current)
+int fixed_scratch(int dev, unsigned char** out) {
+    constexpr const char* W = "qs_sample_noalloc";
+    if (dev < 0 || dev >= kMaxDevices) return fail(QS_ERR_DEVICE, W, "device");
+    std::call_once(g_scratch_once[dev], [dev] {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+        void* ptr = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        g_scratch_err[dev] = e;
+        g_scratch_ptr[dev] = e == hipSuccess ? static_cast<unsigned char*>(ptr) : nullptr;
+    });
+    if (g_scratch_err[dev] != hipSuccess) return qs::fail_hip(g_scratch_err[dev], W);
+    *out = g_scratch_ptr[dev];
+    return QS_OK;
+}
+
+// Every shot searched in the exact table (no host decision in between, so
+// the launch sequence is fixed), in a caller-given workspace.  idx_host /
+// prob_host may be null.
+int sample_exact_only(const qs_state* st, const double* u_host, uint64_t nshots,
+                      uint64_t* idx_host, double* prob_host, unsigned char* ws) {
+    constexpr const char* W = "qpp::sample()";
+    const uint64_t D = st->D;
+    SampleWs w;
+    sample_ws_layout(D, nshots, ws, &w);
+    if (int rc = probs_device(st, w.p)) return rc;
+    if (D < 2) {  // a one-element table returns 0 (random.tcc:2700)
+        if (idx_host)
+            for (uint64_t s = 0; s < nshots; ++s) idx_host[s] = 0;
+        if (prob_host) {
+            QS_HIP(hipMemcpyAsync(prob_host, w.p, 8, hipMemcpyDeviceToHost, st->stream), W);
+            QS_HIP(hipStreamSynchronize(st->stream), W);
+            for (uint64_t s = 1; s < nshots; ++s) prob_host[s] = prob_host[0];
+        }
+        return QS_OK;
+    }
+    const uint64_t nchunks = (D + kChunk - 1) / kChunk;
+    QS_HIP(hipMemcpyAsync(w.u, u_host, nshots * 8, hipMemcpyHostToDevice, st->stream), W);
+    hipLaunchKernelGGL(k_chunk_sums, dim3(grid_for(nchunks, 1)), dim3(256), 0, st->stream, w.p,
+                       D, w.sums, nchunks);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, st->stream, w.sums, nchunks,
+                       w.prefix);
+    if (int rc = exact_table_search(st, w, nshots)) return rc;
+    if (idx_host)
+        QS_HIP(hipMemcpyAsync(idx_host, w.idx, nshots * 8, hipMemcpyDeviceToHost, st->stream), W);
+    if (prob_host) {
+        hipLaunchKernelGGL(k_gather, dim3(grid_for(nshots, 256)), dim3(256), 0, st->stream, w.p,
+                           w.idx, nshots, w.u);
+        QS_HIP(hipGetLastError(), W);
+        QS_HIP(hipMemcpyAsync(prob_host, w.u, nshots * 8, hipMemcpyDeviceToHost, st->stream), W);
+    }
+    QS_HIP(hipStreamSynchronize(st->stream), W);
+    return QS_OK;
 }
 
 int project(qs_state* st, uint64_t stride, uint64_t dim, uint64_t m) {
@@ -836,6 +934,50 @@ int qs_sample(const qs_state* st, uint32_t ntarget, const uint64_t* target,
             }
             for (uint32_t t = 0; t < ntarget; ++t) out_digits[s * ntarget + t] = mid[tv[t]];
         }
+    return QS_OK;
+}
+
+// qs_sample without any device allocation: the workspace is the fixed
+// per-device array `scratch` (see its comment); every shot is searched in
+// the exact table.  Host work uses no heap either: the target check runs in
+// place, and when out_index is null the indices are staged in out_digits.
+int qs_sample_noalloc(const qs_state* st, uint32_t ntarget, const uint64_t* target,
+                      uint64_t nshots, const double* u, uint64_t* out_index,
+                      uint64_t* out_digits, double* out_prob) {
+    constexpr const char* W = "qpp::sample()";
+    if (!valid(st)) return fail(QS_ERR_INVALID_HANDLE, W, "A");
+    qs::clear_error();
+    const uint64_t n = st->dims.size();
+    bool ok = ntarget <= n && (ntarget == 0 || target != nullptr);
+    for (uint32_t i = 0; ok && i < ntarget; ++i) {
+        ok = target[i] < n;
+        for (uint32_t j = 0; ok && j < i; ++j) ok = target[j] != target[i];
+    }
+    if (!ok) return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
+    if (nshots == 0) return fail(QS_ERR_OUT_OF_RANGE, W, "num_samples");
+    if (!u) return fail(QS_ERR_ZERO_SIZE, W, "u");
+    if (st->D > kFixedMaxD || nshots > kFixedMaxShots || n > 64)
+        return qs::fail_runtime(QS_ERR_OUT_OF_MEMORY,
+                                "qs_sample_noalloc: the fixed scratch holds at most 2^20 basis "
+                                "states and 16384 shots per call");
+    unsigned char* ws = nullptr;
+    if (int rc = fixed_scratch(st->device, &ws)) return rc;
+    const bool need_digits = out_digits != nullptr && ntarget > 0;
+    uint64_t* idx = out_index ? out_index : (need_digits ? out_digits : nullptr);
+    if (int rc = sample_exact_only(st, u, nshots, idx, out_prob, ws)) return rc;
+    if (need_digits) {
+        // rows from the last shot down: row s starts at s * ntarget >= s,
+        // so a staged index not yet read is never overwritten
+        for (uint64_t s = nshots; s-- > 0;) {
+            uint64_t rem = idx[s];
+            uint64_t mid[64];  // n <= 64 (qpp's maxn), checked above
+            for (uint64_t k = n; k-- > 0;) {
+                mid[k] = rem % st->dims[k];
+                rem /= st->dims[k];
+            }
+            for (uint32_t t = 0; t < ntarget; ++t) out_digits[s * ntarget + t] = mid[target[t]];
+        }
+    }
     return QS_OK;
 }
 
