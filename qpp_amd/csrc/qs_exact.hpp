@@ -3,7 +3,7 @@
 //
 // A running sum is carried as the double s itself.  Within a binade the maps
 // of qs_exact_core.hpp are scanned in parallel; the element whose add leaves
-// the binade is added in floating point by the lane that holds it, and the
+// the binade is added in floating point by the thread that holds it, and the
 // scan restarts after it.  The results equal the sequential left-to-right
 // sum bit for bit (round to nearest even, as on the host).
 #ifndef QPP_AMD_QS_EXACT_HPP_
@@ -41,33 +41,6 @@ __device__ __forceinline__ Tr wave_incl(Tr t, int lane) {
 __device__ __forceinline__ Tr wave_excl(Tr inc, int lane) {
     const Tr o = shfl_up_tr(inc, 1);
     return lane == 0 ? identity() : o;
-}
-
-// Exact running sums over up to 64 elements, lane j holding x_j (j < cnt).
-// s: the sum before them (wave-uniform).  val: the sum after the lane's own
-// element (lanes < cnt).  Returns the sum after element cnt - 1.
-__device__ inline double wave_run(double s, double x, int lane, int cnt, double& val) {
-    int start = 0;
-    while (start < cnt) {
-        const int E = binade(s);
-        const uint64_t m0 = mant(s);
-        const bool mine = lane >= start && lane < cnt;
-        const Tr inc = wave_incl(mine ? elem(x, E) : identity(), lane);
-        const Tr ex = wave_excl(inc, lane);  // every lane takes part in the shuffles
-        const Tr all = shfl_tr(inc, 63);
-        const uint64_t mi = apply(m0, inc);
-        const unsigned long long over = __ballot(mine && mi >= kTop);
-        if (!over) {
-            if (mine) val = from_mant(mi, E);
-            return from_mant(apply(m0, all), E);
-        }
-        const int f = __ffsll(over) - 1;
-        if (mine && lane < f) val = from_mant(mi, E);
-        if (lane == f) val = __dadd_rn(from_mant(apply(m0, ex), E), x);
-        s = __shfl(val, f, 64);
-        start = f + 1;
-    }
-    return s;
 }
 
 // Workgroup of NT threads (NT / 64 waves): exclusive scan of one map per

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ p,
 //                  (k_scan_chunks) falls in;
 //   k_ex_chain     one wave walks the blocks in order carrying the exact sum:
 //                  64 blocks per step when their maps apply, otherwise the
-//                  block is summed element by element (wave_run);
+//                  block's own elements are scanned by the wave (wg_chunk);
 //   k_ex_expand    every block rebuilds its elements' sums from its exact
 //                  entry value (wg_chunk) and writes cp;
 //   k_ex_search    lower_bound(cp, u) per shot.
@@ -275,24 +275,50 @@ __global__ __launch_bounds__(256) void k_ex_block_tr(const double* __restrict__ 
 }
 
 // One wave.  s_in[b] = exact sum before block b; s_in[nb] = the total.
+// The maps of the next 64 blocks are loaded while the current ones are
+// scanned.  A block whose map does not apply is summed by the wave itself
+// in two halves of 32 consecutive elements per lane, one scan each
+// (wg_chunk with one wave).
 __global__ __launch_bounds__(64) void k_ex_chain(const double* __restrict__ p, uint64_t D,
                                                  const double* __restrict__ S, uint64_t nb,
                                                  const qs_exact::Tr* __restrict__ T,
                                                  const int* __restrict__ Eb,
                                                  double* __restrict__ s_in) {
     using namespace qs_exact;
-    __shared__ double xs[kChunk];
+    __shared__ ChunkLds<64> L;
     const int lane = threadIdx.x;
     const double sv = S ? *S : 1.0;
     double s = 0.0;
     uint64_t b = 0;
+    // maps of blocks pb + lane, loaded ahead
+    uint64_t pb = 0;
+    Tr tn = Tr{kTop, kTop};
+    int en = 0;
+    if (uint64_t(lane) < nb) {
+        tn = T[lane];
+        en = Eb[lane];
+    }
     while (b < nb) {
+        if (b != pb) {  // the group does not start where the prefetch assumed
+            pb = b;
+            if (pb + lane < nb) {
+                tn = T[pb + lane];
+                en = Eb[pb + lane];
+            }
+        }
+        const Tr tc = tn;
+        const int ec = en;
+        pb = b + 64;
+        if (pb + lane < nb) {
+            tn = T[pb + lane];
+            en = Eb[pb + lane];
+        }
         const int E = binade(s);
         const uint64_t m0 = mant(s);
         const int cnt = nb - b < 64 ? int(nb - b) : 64;
         const uint64_t j = b + lane;
-        const bool valid = lane < cnt && Eb[j] == E;
-        const Tr inc = wave_incl(valid ? T[j] : Tr{kTop, kTop}, lane);
+        const bool valid = lane < cnt && ec == E;
+        const Tr inc = wave_incl(valid ? tc : Tr{kTop, kTop}, lane);
         const Tr ex = wave_excl(inc, lane);
         const uint64_t mi = apply(m0, inc);
         const unsigned long long bad = __ballot(lane < cnt && mi >= kTop);
@@ -301,23 +327,16 @@ __global__ __launch_bounds__(64) void k_ex_chain(const double* __restrict__ p, u
         if (f > 0) s = from_mant(apply(m0, shfl_tr(inc, f - 1)), E);
         b += f;
         if (f < cnt) {
-            // block b, element by element; its elements are staged in LDS
-            // first, all loads in flight together
             if (lane == 0) s_in[b] = s;
-            const uint64_t e0 = b * kChunk;
+            constexpr int V = kChunk / 128;  // two halves of 64 x 32 elements
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t i0 = b * kChunk + uint64_t(h) * (kChunk / 2) + uint64_t(lane) * V;
+                double x[V], out[V];
 #pragma unroll
-            for (int g = 0; g < kChunk / 64; ++g) {
-                const uint64_t i = e0 + uint64_t(g) * 64 + lane;
-                xs[g * 64 + lane] = i < D ? ex_elem(p, i, S, sv) : 0.0;
+                for (int k = 0; k < V; ++k) x[k] = i0 + k < D ? ex_elem(p, i0 + k, S, sv) : 0.0;
+                s = wg_chunk<64, V>(s, x, out, L);
             }
-            __syncthreads();
-            double val = 0.0;
-            const uint64_t n = D - e0 < uint64_t(kChunk) ? D - e0 : uint64_t(kChunk);
-            for (uint64_t i0 = 0; i0 < n; i0 += 64) {
-                const int c = n - i0 < 64 ? int(n - i0) : 64;
-                s = wave_run(s, xs[i0 + lane], lane, c, val);
-            }
-            __syncthreads();  // xs is refilled by the next slow block
             b += 1;
         }
     }

@@ -32,50 +32,18 @@ constexpr int kNT = 256;
 long g_restarts = 0;  // binade crossings resolved inside a run
 long g_slow = 0;      // blocks the chain summed element by element
 
-// wave_run: up to 64 elements, one per lane (qs_exact.hpp)
-double wave_run(double s, const double* x, int cnt, double* val) {
-    int start = 0;
-    while (start < cnt) {
-        const int E = binade(s);
-        const uint64_t m0 = mant(s);
-        Tr inc[64];
-        Tr acc = identity();
-        for (int l = 0; l < 64; ++l) {
-            acc = compose(acc, (l >= start && l < cnt) ? elem(x[l], E) : identity());
-            inc[l] = acc;
-        }
-        int f = -1;
-        for (int l = start; l < cnt; ++l)
-            if (apply(m0, inc[l]) >= kTop) {
-                f = l;
-                break;
-            }
-        if (f < 0) {
-            for (int l = start; l < cnt; ++l) val[l] = from_mant(apply(m0, inc[l]), E);
-            return from_mant(apply(m0, inc[63]), E);
-        }
-        ++g_restarts;
-        for (int l = start; l < f; ++l) val[l] = from_mant(apply(m0, inc[l]), E);
-        const Tr ex = f == 0 ? identity() : inc[f - 1];
-        val[f] = from_mant(apply(m0, ex), E) + x[f];
-        s = val[f];
-        start = f + 1;
-    }
-    return s;
-}
-
 // wg_chunk: NT threads x V consecutive elements each (qs_exact.hpp)
-template <int V>
+template <int NT, int V>
 double wg_chunk(double s, const double* x, double* out) {
     int start = 0;
     for (;;) {
         const int E = binade(s);
         const uint64_t m0 = mant(s);
-        std::vector<Tr> mine(kNT, identity()), ex(kNT);
-        for (int t = start; t < kNT; ++t)
+        std::vector<Tr> mine(NT, identity()), ex(NT);
+        for (int t = start; t < NT; ++t)
             for (int j = 0; j < V; ++j) mine[t] = compose(mine[t], elem(x[t * V + j], E));
         Tr acc = identity();
-        for (int t = 0; t < kNT; ++t) {
+        for (int t = 0; t < NT; ++t) {
             ex[t] = acc;
             acc = compose(acc, mine[t]);
         }
@@ -88,12 +56,12 @@ double wg_chunk(double s, const double* x, double* out) {
         };
         const uint64_t mend = apply(m0, acc);
         if (mend < kTop) {
-            for (int t = start; t < kNT; ++t) walk(t);
+            for (int t = start; t < NT; ++t) walk(t);
             return from_mant(mend, E);
         }
         ++g_restarts;
-        int f = kNT;
-        for (int t = start; t < kNT; ++t)
+        int f = NT;
+        for (int t = start; t < NT; ++t)
             if (apply(m0, compose(ex[t], mine[t])) >= kTop) {
                 f = t;
                 break;
@@ -120,7 +88,7 @@ double wg_chunk(double s, const double* x, double* out) {
         }
         s = v;
         start = f + 1;
-        if (start >= kNT) return s;
+        if (start >= NT) return s;
     }
 }
 
@@ -169,13 +137,13 @@ std::vector<double> chain(const Seq& x, uint64_t N, const std::vector<double>& a
         if (f < cnt) {
             ++g_slow;
             s_in[b] = s;
-            const uint64_t e0 = b * kChunk;
-            const uint64_t n = N - e0 < uint64_t(kChunk) ? N - e0 : uint64_t(kChunk);
-            double xs[64], val[64];
-            for (uint64_t i0 = 0; i0 < n; i0 += 64) {
-                const int c = n - i0 < 64 ? int(n - i0) : 64;
-                for (int l = 0; l < 64; ++l) xs[l] = l < c ? x(e0 + i0 + l) : 0.0;
-                s = wave_run(s, xs, c, val);
+            // one wave, two halves of 64 lanes x 32 consecutive elements
+            double xs[kChunk / 2], val[kChunk / 2];
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t e0 = b * kChunk + uint64_t(h) * (kChunk / 2);
+                for (uint64_t k = 0; k < uint64_t(kChunk / 2); ++k)
+                    xs[k] = e0 + k < N ? x(e0 + k) : 0.0;
+                s = wg_chunk<64, kChunk / 128>(s, xs, val);
             }
             b += 1;
         }
@@ -292,7 +260,7 @@ int main(int argc, char** argv) {
             const uint64_t i = b * kChunk + j;
             x[j] = i < N ? xq(i) : 0.0;
         }
-        wg_chunk<16>(sq[b], x.data(), out.data());
+        wg_chunk<kNT, 16>(sq[b], x.data(), out.data());
         for (uint64_t j = 0; j < uint64_t(kChunk); ++j) {
             const uint64_t i = b * kChunk + j;
             if (i >= N) break;
@@ -311,7 +279,7 @@ int main(int argc, char** argv) {
         for (uint64_t b = 0; b < N; b += kNT * 8) {
             for (uint64_t j = 0; j < uint64_t(kNT * 8); ++j)
                 xv[j] = b + j < N ? p[b + j] : 0.0;
-            s = wg_chunk<8>(s, xv.data(), ov.data());
+            s = wg_chunk<kNT, 8>(s, xv.data(), ov.data());
         }
         if (std::memcmp(&s, &S_ref, 8) != 0) {
             std::printf("FAIL wg_sum %s != %s\n", bits(s).c_str(), bits(S_ref).c_str());
