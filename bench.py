@@ -55,9 +55,23 @@ def profile_region(qa, fn):
     return ms.value
 
 
-def cpu_baseline(gates, n, sample_gates):
-    """Oracle (qpp qubit kernels restated, OpenMP) on the first
-    `sample_gates` gates of the same 30-qubit circuit."""
+def cpu_model():
+    """The host CPU's model name (the first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(gates, n, picks):
+    """Oracle (qpp qubit kernels restated, OpenMP) on the gates `picks`
+    (indices into the circuit) of the same circuit, applied in that order
+    to an n-qubit state; returns per-gate times and the circuit time
+    extrapolated from the mean time per gate kind (flagged as such)."""
     import ctypes
     from ctypes import POINTER, c_double, c_uint32
     import oracle
@@ -65,7 +79,7 @@ def cpu_baseline(gates, n, sample_gates):
     L.orc_time_gates.argtypes = [c_uint32, c_uint32, POINTER(c_uint32), POINTER(c_uint32),
                                  POINTER(c_uint32), POINTER(c_double), POINTER(c_double)]
     L.orc_time_gates.restype = ctypes.c_int
-    g = gates[:sample_gates]
+    g = [gates[i] for i in picks]
     kind = np.array([0 if x[0] == "u" else 1 for x in g], dtype=np.uint32)
     tg = np.array([x[1][0] for x in g], dtype=np.uint32)
     ct = np.array([x[2][0] if x[2] else 0 for x in g], dtype=np.uint32)
@@ -80,15 +94,31 @@ def cpu_baseline(gates, n, sample_gates):
     from qpp_amd.workloads import gate_bytes
     nbytes = sum(gate_bytes(x, n) for x in g)
     total = float(sec.sum())
+    n1, n2 = int((kind == 0).sum()), int((kind == 1).sum())
+    t1 = float(sec[kind == 0].mean()) if n1 else 0.0
+    t2 = float(sec[kind == 1].mean()) if n2 else 0.0
+    c1 = sum(1 for x in gates if x[0] == "u")
+    c2 = len(gates) - c1
+    full = len(picks) == len(gates)
+    est = total if full else c1 * t1 + c2 * t2
     return {
         "value": round(nbytes / total / 1e9, 3),
         "unit": "GB/s",
         "cores": int(threads),
+        "cpu_model": cpu_model(),
         "kind": "port",
-        "sample": f"first {len(g)} gates of the same {n}-qubit circuit "
-                  f"({int((kind == 0).sum())} dense 1q + {int((kind == 1).sum())} CNOT), "
-                  f"{total:.2f} s, {len(g) / total:.3f} gate-apps/s; qpp 7.0.3 qubit kernels "
-                  f"restated (apply_psi_1q / apply_ctrl_psi_1q, out-of-place)",
+        "sample": (f"the whole {n}-qubit circuit ({n1} dense 1q + {n2} CNOT gates): {total:.2f} s"
+                   if full else
+                   f"{len(g)} gates of the same {n}-qubit circuit ({n1} dense 1q at qubits "
+                   f"{[x[1][0] for x in g if x[0] == 'u']}, {n2} CNOT), {total:.2f} s: "
+                   f"{t1:.3f} s per dense gate, {t2:.3f} s per CNOT")
+                  + "; qpp 7.0.3 qubit kernels restated (apply_psi_1q / apply_ctrl_psi_1q, "
+                    "out-of-place, OpenMP)",
+        "circuit_s": round(est, 2),
+        "circuit_s_extrapolated": not full,
+        "circuit_s_definition": ("measured" if full else
+                                 f"EXTRAPOLATED: {c1} x mean dense-gate time + {c2} x mean "
+                                 f"CNOT time from the sample above"),
     }
 
 
@@ -443,9 +473,16 @@ def bench_rho(args, qa):
                "sample": f"first {len(sub)} of the {len(ops)} gates/channels ({t_ops:.2f} s, "
                          f"{t_ops / len(sub) * 1e3:.1f} ms each) and the three maps "
                          f"({t_maps:.2f} s) through the oracle (qpp's rho kernels restated)"}
+    # HBM bytes the pipeline streams: every fused pass reads and writes rho
+    # once, ptrace reads the entries it sums and writes the result, and
+    # ptranspose / syspermute read and write rho once each
+    streamed = pass_bytes + (algo - workloads.dm_bytes(n, ops))
     return {
-        "metric": "12-qubit density-matrix pipeline GB/s (algorithmic)",
-        "value": round(algo / wall / 1e9, 2),
+        "metric": "12-qubit density-matrix pipeline GB/s (HBM bytes streamed)",
+        "value": round(streamed / wall / 1e9, 2),
+        "value_definition": "bytes the fused passes and the three maps stream / pipeline wall "
+                            "time (the input copy excluded)",
+        "gate_apply_equiv_GBps": round(algo / wall / 1e9, 2),
         "unit": "GB/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3, 3),
@@ -493,76 +530,19 @@ def bench_copy(args, qa):
         "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
         "config": {"workload": f"copy of a {n}-qubit complex128 state ({nbytes / 2**30:.0f} GiB moved)",
                    "bytes_per_launch": nbytes},
-        "roofline": {"bound": "hbm", "kernel": "k_convert<double2,double2>", "achieved": round(gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_stream_copy", "achieved": round(gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": None, "avg_launch_ms": round(ms, 4)},
         "cpu_baseline": None,
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--qubits", type=int, default=30)
-    ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--seed", type=int, default=30)
-    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy"], default="fused")
-    ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
-    ap.add_argument("--cal-requests", type=int, default=40,
-                    help="serve mode: closed-loop calibration requests")
-    ap.add_argument("--lanes", type=int, default=8, help="serve mode: concurrent lanes")
-    ap.add_argument("--dtype", choices=["c128", "c64"], default="c64",
-                    help="batch mode: complex dtype of the trajectories")
-    ap.add_argument("--traj-scale", type=float, default=1.0,
-                    help="batch mode: fraction of the 4096/512/512/512 trajectories")
-    ap.add_argument("--traj-depth", type=int, default=30)
-    ap.add_argument("--oracle-traj", type=int, default=16,
-                    help="batch mode: 12-qubit trajectories checked against the oracle engine")
-    ap.add_argument("--oracle-traj16", type=int, default=6,
-                    help="batch mode: 16-qubit trajectories run on the oracle engine (parity + CPU baseline)")
-    ap.add_argument("--cpu-ops", type=int, default=24,
-                    help="rho mode: gates/channels of the pipeline timed on the oracle (CPU baseline)")
-    ap.add_argument("--cpu-requests", type=int, default=4,
-                    help="serve mode: small requests run on the oracle engine (parity + CPU baseline)")
-    ap.add_argument("--cpu-gates", type=int, default=-1,
-                    help="gates in the CPU baseline sample (0 = skip; default: the whole "
-                         "circuit up to 22 qubits, else the first 6 gates)")
-    ap.add_argument("--precision", type=int, default=1,
-                    help="fused mode: also run the circuit in complex64 and compare with complex128 (1/0)")
-    ap.add_argument("--reg-bits", type=int, default=4)
-    ap.add_argument("--low-bits", type=int, default=5)
-    ap.add_argument("--max-ops", type=int, default=64)
-    ap.add_argument("--grid-cap", type=int, default=0)
-    ap.add_argument("--autotune", type=int, default=1,
-                    help="fused mode: pick tile options with the autotuner first (1/0)")
-    ap.add_argument("--sweep", action="store_true",
-                    help="time fused options on the first layers and print them")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import qpp_amd as qa
+def bench_circuit(args, qa, n, depth, primary):
+    """The random circuit (n qubits, `depth` layers) on a resident complex128
+    state: one step applies the whole circuit.  The primary run (30 qubits,
+    depth 40) also tunes the fused options, splits off the complex64 run and
+    times the streaming copy ceiling."""
     from qpp_amd import workloads
-
-    info = qa.device_info()
-    log("device:", info)
-    if args.mode == "batch":
-        print(json.dumps(bench_batch(args, qa)), flush=True)
-        return
-    if args.mode == "serve":
-        print(json.dumps(bench_serve(args, qa)), flush=True)
-        return
-    if args.mode == "rho":
-        print(json.dumps(bench_rho(args, qa)), flush=True)
-        return
-    if args.mode == "copy":
-        print(json.dumps(bench_copy(args, qa)), flush=True)
-        return
-    n, depth = args.qubits, args.depth
     gates = workloads.random_layers(n, depth, args.seed)
     n1q = sum(1 for g in gates if g[0] == "u")
     ncx = len(gates) - n1q
@@ -585,7 +565,7 @@ def main():
                             f"({ms / npas:.2f} ms/pass, {gbs:.0f} GB/s per pass, "
                             f"{len(sub) / ms * 1e3:.0f} gates/s)")
     tuned = None
-    if mode == "fused" and args.autotune:
+    if mode == "fused" and args.autotune and primary:
         from qpp_amd import autotune
         cands = [dict(reg_bits=4, low_bits=lb, max_ops=mo, grid_cap=gc)
                  for lb in (3, 4, 5) for mo in (48, 64, 96) for gc in (16384, 32768, 65536)]
@@ -630,11 +610,12 @@ def main():
     # have to stream to match this time (not an HBM rate)
     equiv = algo_bytes / sec / 1e9
 
-    # dominant kernel, measured alone with events on the library stream
+    # dominant kernel: every pass timed with HIP events recorded between the
+    # pass launches on the state's stream
+    pass_ms = None
     if mode == "fused":
-        reps = 3
-        ms = profile_region(qa, lambda: [prog.run(st) for _ in range(reps)])
-        avg_ms = ms / (reps * npasses)
+        pass_ms = np.concatenate([prog.run_timed(st) for _ in range(3)])
+        avg_ms = float(pass_ms.mean())
         k_bytes = 2 * 16 * 2 ** n
     else:
         U = gates[0][3]
@@ -650,7 +631,7 @@ def main():
     # amplitudes, FP64 reductions) against complex128, compared on the device
     # in FP64
     precision = None
-    if mode == "fused" and args.precision:
+    if mode == "fused" and args.precision and primary:
         st.set_basis(0)
         prog.run(st)
         s64 = qa.State([2] * n, dtype=qa.C64)
@@ -723,9 +704,19 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "avg_launch_ms": round(avg_ms, 4),
+            "launch_ms_min_max": ([round(float(pass_ms.min()), 3), round(float(pass_ms.max()), 3)]
+                                  if pass_ms is not None else None),
+            "timing": ("HIP events between the pass launches on the state's stream, "
+                       f"{len(pass_ms)} launches" if pass_ms is not None else
+                       "HIP events around the launches on the library stream"),
         },
         "cpu_baseline": None,
     }
+    if primary and args.copy_ceiling:
+        cc = bench_copy(argparse.Namespace(qubits=n, steps=3, warmup=1), qa)
+        out["roofline"]["copy_ceiling_GBps"] = cc["value"]
+        out["roofline"]["frac_of_copy_ceiling"] = round(achieved / cc["value"], 4)
+        log(f"copy ceiling: {cc['value']} GB/s")
     # HBM bytes per launch of this kernel at this size, measured with
     # rocprofv3 FETCH_SIZE / WRITE_SIZE and scaled by the copy-kernel
     # calibration (profiles/traffic.json; the summaries beside it hold the
@@ -737,10 +728,129 @@ def main():
             out["roofline"]["traffic"] = tr["bytes_per_launch"]
     except (OSError, ValueError):
         pass
-    ncpu = args.cpu_gates if args.cpu_gates >= 0 else (len(gates) if n <= 22 else 6)
-    if ncpu > 0:
-        log(f"cpu baseline: {ncpu} gates at {n} qubits ...")
-        out["cpu_baseline"] = cpu_baseline(gates, n, ncpu)
+    if args.cpu_gates != 0:
+        if n <= 22:
+            picks = list(range(len(gates)))
+        else:  # dense gates on high and low qubits, and CNOTs, of layer 0
+            picks = [0, 1, n - 2, n - 1, n, n + 1]
+        log(f"cpu baseline: {len(picks)} gates at {n} qubits ...")
+        out["cpu_baseline"] = cpu_baseline(gates, n, picks)
+    return out
+
+
+def secondary_configs(args, qa):
+    """The other benchmark configurations, each at a bounded size so that the
+    default run stays within minutes; every object carries its own roofline
+    and CPU baseline.  Sizes are stated in each object's config."""
+    out = []
+    base = vars(args).copy()
+
+    def ns(**kw):
+        d = dict(base)
+        d.update(kw)
+        return argparse.Namespace(**d)
+
+    t = time.perf_counter()
+    # 20-qubit complex128 random circuit, depth 20 (the CPU-runnable plain
+    # method), fused with fixed options; the oracle runs the whole circuit
+    a0 = ns(qubits=20, depth=20, steps=20, warmup=3, reg_bits=4, low_bits=3, max_ops=48,
+            grid_cap=65536, autotune=0, precision=0, cpu_gates=-1, sweep=False)
+    r0 = bench_circuit(a0, qa, 20, 20, primary=False)
+    r0["config_id"] = 0
+    r0["metric"] = "20q random circuit GB/s of the fused passes"
+    out.append(r0)
+    log(f"secondary 20-qubit circuit: {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    r1 = bench_rho(ns(steps=3, warmup=1, cpu_ops=8), qa)
+    r1["config_id"] = 1
+    out.append(r1)
+    log(f"secondary 12-qubit rho pipeline: {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    r3 = bench_batch(ns(steps=2, warmup=1, traj_scale=0.25, dtype="c64", oracle_traj=6,
+                        oracle_traj16=2), qa)
+    r3["config_id"] = 3
+    out.append(r3)
+    log(f"secondary batched trajectories: {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    r4 = bench_serve(ns(requests=60, cal_requests=12, cpu_requests=4), qa)
+    r4["config_id"] = 4
+    out.append(r4)
+    log(f"secondary serving: {time.perf_counter() - t:.1f} s")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=30)
+    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy"], default="fused")
+    ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
+    ap.add_argument("--cal-requests", type=int, default=40,
+                    help="serve mode: closed-loop calibration requests")
+    ap.add_argument("--lanes", type=int, default=8, help="serve mode: concurrent lanes")
+    ap.add_argument("--dtype", choices=["c128", "c64"], default="c64",
+                    help="batch mode: complex dtype of the trajectories")
+    ap.add_argument("--traj-scale", type=float, default=1.0,
+                    help="batch mode: fraction of the 4096/512/512/512 trajectories")
+    ap.add_argument("--traj-depth", type=int, default=30)
+    ap.add_argument("--oracle-traj", type=int, default=16,
+                    help="batch mode: 12-qubit trajectories checked against the oracle engine")
+    ap.add_argument("--oracle-traj16", type=int, default=6,
+                    help="batch mode: 16-qubit trajectories run on the oracle engine (parity + CPU baseline)")
+    ap.add_argument("--cpu-ops", type=int, default=24,
+                    help="rho mode: gates/channels of the pipeline timed on the oracle (CPU baseline)")
+    ap.add_argument("--cpu-requests", type=int, default=4,
+                    help="serve mode: small requests run on the oracle engine (parity + CPU baseline)")
+    ap.add_argument("--secondary", type=int, default=1,
+                    help="fused mode: also run the other configurations (20-qubit circuit, "
+                         "12-qubit rho pipeline, batched trajectories, serving) at bounded "
+                         "sizes and report them under 'secondary'")
+    ap.add_argument("--copy-ceiling", type=int, default=1,
+                    help="time the streaming copy of the same state as the HBM ceiling")
+    ap.add_argument("--cpu-gates", type=int, default=-1,
+                    help="0 skips the CPU baseline; otherwise it runs the whole circuit up "
+                         "to 22 qubits, else 4 dense gates and 2 CNOTs of the first layer, "
+                         "extrapolated per gate kind")
+    ap.add_argument("--precision", type=int, default=1,
+                    help="fused mode: also run the circuit in complex64 and compare with complex128 (1/0)")
+    ap.add_argument("--reg-bits", type=int, default=4)
+    ap.add_argument("--low-bits", type=int, default=5)
+    ap.add_argument("--max-ops", type=int, default=64)
+    ap.add_argument("--grid-cap", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="fused mode: pick tile options with the autotuner first (1/0)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="time fused options on the first layers and print them")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import qpp_amd as qa
+    from qpp_amd import workloads
+
+    info = qa.device_info()
+    log("device:", info)
+    if args.mode == "batch":
+        print(json.dumps(bench_batch(args, qa)), flush=True)
+        return
+    if args.mode == "serve":
+        print(json.dumps(bench_serve(args, qa)), flush=True)
+        return
+    if args.mode == "rho":
+        print(json.dumps(bench_rho(args, qa)), flush=True)
+        return
+    if args.mode == "copy":
+        print(json.dumps(bench_copy(args, qa)), flush=True)
+        return
+    out = bench_circuit(args, qa, args.qubits, args.depth, primary=True)
+    if args.mode == "fused" and args.secondary:
+        out["secondary"] = secondary_configs(args, qa)
     print(json.dumps(out), flush=True)
 
 

@@ -373,6 +373,12 @@ int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates
                            const double* mats, qs_dtype dtype,
                            const qs_fuse_opts* opts, qs_program** out);
 int qs_program_run(qs_program* p, qs_state* st);
+/* qs_program_run with a HIP event recorded on the state's stream before the
+ * first pass and after each pass; waits for the last one and writes each
+ * pass's elapsed milliseconds to pass_ms[0 .. num_passes-1] (the kernel
+ * durations, launches being back to back).  No qpp counterpart:
+ * measurement. */
+int qs_program_run_timed(qs_program* p, qs_state* st, double* pass_ms);
 int qs_program_info(const qs_program* p, uint64_t* num_gates,
                     uint64_t* num_passes, uint64_t* num_phases);
 int qs_program_destroy(qs_program* p);

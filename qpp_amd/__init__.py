@@ -811,6 +811,7 @@ class Program:
         L.qs_program_info.argtypes = [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]
         L.qs_program_destroy.argtypes = [c_void_p]
         o = _FuseOpts(reg_bits, low_bits, max_ops, grid_cap)
+        self._dtype, self._reg_bits = dtype, reg_bits
         self._h = c_void_p()
         end = len(qc.steps) if step_end is None else step_end
         _check(L.qs_program_create(ctypes.byref(self._cs), step_begin, end, kind, dtype,
@@ -818,6 +819,15 @@ class Program:
 
     def run(self, st):
         _check(lib().qs_program_run(self._h, st.handle))
+
+    def run_timed(self, st):
+        """Run once; the milliseconds of each pass, from HIP events recorded
+        on the state's stream between the pass launches."""
+        L = lib()
+        L.qs_program_run_timed.argtypes = [c_void_p, c_void_p, POINTER(c_double)]
+        ms = np.zeros(max(1, self.num_passes()))
+        _check(L.qs_program_run_timed(self._h, st.handle, ms.ctypes.data_as(POINTER(c_double))))
+        return ms[: self.num_passes()]
 
     def info(self):
         g, p, h = c_uint64(), c_uint64(), c_uint64()
@@ -828,7 +838,8 @@ class Program:
         return self.info()["passes"]
 
     def kernel_name(self):
-        return "k_tile<double,16>"
+        r = "double" if self._dtype == C128 else "float"
+        return f"k_tile<{r},{16 if self._reg_bits == 4 else 8}>"
 
     def __del__(self):
         try:

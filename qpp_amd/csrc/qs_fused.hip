@@ -684,7 +684,7 @@ int launch_pass(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap
     return QS_OK;
 }
 
-int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap) {
+int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap, hipEvent_t* ev) {
     if (!P.dblob) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::fused", "program not uploaded");
     if (st->dtype != P.dtype) return fail(QS_ERR_TYPE_MISMATCH, "qpp_amd::fused", "dtype");
     if (st->stream != P.stream) {
@@ -694,6 +694,7 @@ int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap) {
     }
     if (!st->qubits || st->nbits != P.nb)
         return fail(QS_ERR_DIMS_MISMATCH_CVECTOR, "qpp_amd::fused", "state/program");
+    if (ev) QS_HIP(hipEventRecord(ev[0], st->stream), "qpp_amd::fused");
     for (size_t p = 0; p < P.passes.size(); ++p) {
         int rc;
         if (st->dtype == QS_C128)
@@ -703,6 +704,7 @@ int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap) {
             rc = P.reg_bits == 3 ? launch_pass<float, 8>(P, st, int(p), grid_cap)
                                  : launch_pass<float, 16>(P, st, int(p), grid_cap);
         if (rc) return rc;
+        if (ev) QS_HIP(hipEventRecord(ev[p + 1], st->stream), "qpp_amd::fused");
     }
     QS_HIP(hipGetLastError(), "qpp_amd::fused");
     return QS_OK;

@@ -27,23 +27,33 @@ enum FOpKind : int {
 //   register bits -> regok: bit rho set when register group rho satisfies it
 //   thread bits   -> (t & tmask) == tval on the thread number t
 //   other bits    -> (tile base & omask) == oval, uniform per tile
-// The kernels dispatch on `code` alone (one scalar load, with the matrix in
-// the same 80 leading bytes); the other fields are read only by the cases
+// The kernels dispatch on `code` alone (loaded with the matrix and the other
+// fields of the 96-byte head); the other fields are read only by the cases
 // that need them.  `kind`, `j0`, `j1`, `ctrl` are the host's description,
 // packed into `code` by seal_op().
 struct FOp {
+    // --- the 96-byte head (OpHead): everything the dispatch and the
+    // common cases read, fetched with two scalar loads
     uint32_t code;      // kind | j0 << 4 | j1 << 6 | kCodeCtrl | kCodeOut
     uint32_t regok;
     uint32_t tmask, tval;
+    uint32_t moff;      // OP_U2 / OP_U1B: matrix offset (complex elements)
+    uint32_t tbit;      // OP_DIAG_T: thread bit; OP_U1B: register bits
+    uint32_t pad0, pad1;
     double m[8];        // U1: u00 u10 u01 u11 (re, im); DIAG: d0 d1
+    // --- the rest, read only by the cases that need it
     uint64_t omask, oval;
     uint64_t dbit;      // OP_DIAG_O: global bit
-    uint32_t tbit;      // OP_DIAG_T: thread bit; OP_U1B: register bits
-    uint32_t moff;      // OP_U2 / OP_U1B: matrix offset (complex elements)
     int kind;
     int j0, j1;
     int ctrl;           // 1 when regok / tmask restrict the op per element
 };
+
+struct OpHead {
+    uint32_t code, regok, tmask, tval, moff, tbit, pad0, pad1;
+    double m[8];
+};
+static_assert(sizeof(OpHead) == 96, "OpHead is the first 96 bytes of FOp");
 
 constexpr uint32_t kCodeCtrl = 1u << 8;  // regok / tmask apply
 constexpr uint32_t kCodeOut = 1u << 9;   // omask / oval apply
@@ -126,7 +136,9 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
                   FusedProgram& P);
 int fused_upload(FusedProgram& P, qs_dtype dtype);
 void fused_free(FusedProgram& P);
-int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap);
+// ev: nullptr, or passes + 1 events recorded on the state's stream before
+// the first pass and after each pass (per-pass kernel timing)
+int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap, hipEvent_t* ev = nullptr);
 // circuit gate steps [b, e) -> bit gates (qs_program.hip)
 int lower_steps(const qs_circuit* qc, uint32_t b, uint32_t e, bool rho,
                 std::vector<BitGate>& out);

@@ -84,6 +84,20 @@ __global__ __launch_bounds__(kBlock) void k_convert(const TI* __restrict__ in,
     }
 }
 
+// Streaming copy of n 16-byte chunks, one per thread, consecutive threads on
+// consecutive chunks, one launch covering the whole buffer (no grid-stride
+// loop): the dispatcher then walks the buffer in order, which measured
+// faster on MI355X than persistent grid-stride copies (6.2 TB/s against at
+// most 5.5 TB/s with 1-8 chunks in flight per thread).  The practical HBM
+// ceiling every other kernel is compared with.
+constexpr uint64_t kCopyMaxGrid = uint64_t{1} << 30;  // workgroups per launch
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4* __restrict__ in,
+                                                        uint4* __restrict__ out, uint64_t n,
+                                                        uint64_t base) {
+    const uint64_t i = base + uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
 unsigned grid_for(uint64_t n) {
     return static_cast<unsigned>(std::max<uint64_t>(
         1, std::min<uint64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock)));
@@ -141,24 +155,31 @@ int qs_compare(const qs_state* a, const qs_state* b, double* out) {
     return QS_OK;
 }
 
-// dst := src for two states of the same shape and dtype, with a plain
-// streaming kernel (every byte read once and written once: the reference
-// point for memory-traffic counters).
+// dst := src for two states of the same shape and dtype, with the streaming
+// copy kernel (every byte read once and written once: the reference point
+// for memory-traffic counters and the practical HBM ceiling).
 int qs_state_copy(qs_state* dst, const qs_state* src) {
     constexpr const char* W = "qpp_amd::copy()";
     if (!valid(dst) || !valid(src)) return fail(QS_ERR_INVALID_HANDLE, W);
     if (dst->dtype != src->dtype || dst->kind != src->kind || dst->dims != src->dims)
         return fail(QS_ERR_TYPE_MISMATCH, W, "dst/src");
     if (src->stream != dst->stream) QS_HIP(hipStreamSynchronize(src->stream), W);
-    const uint64_t n = src->nelem;
-    const unsigned grid = grid_for(n);
-    if (src->dtype == QS_C128)
-        hipLaunchKernelGGL((k_convert<double2, double2>), dim3(grid), dim3(kBlock), 0, dst->stream,
-                           static_cast<const double2*>(src->d), static_cast<double2*>(dst->d), n);
-    else
-        hipLaunchKernelGGL((k_convert<float2, float2>), dim3(grid), dim3(kBlock), 0, dst->stream,
-                           static_cast<const float2*>(src->d), static_cast<float2*>(dst->d), n);
+    // 16-byte chunks; an odd number of complex64 elements leaves 8 bytes
+    // over, copied by the runtime
+    const uint64_t chunks = src->bytes() / 16;
+    for (uint64_t b = 0; b < chunks; b += kCopyMaxGrid * kBlock) {
+        const uint64_t left = chunks - b;
+        const uint64_t g = std::min<uint64_t>(kCopyMaxGrid, (left + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_stream_copy, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0,
+                           dst->stream, static_cast<const uint4*>(src->d),
+                           static_cast<uint4*>(dst->d), chunks, b);
+    }
     QS_HIP(hipGetLastError(), W);
+    if (src->bytes() % 16)
+        QS_HIP(hipMemcpyAsync(static_cast<char*>(dst->d) + chunks * 16,
+                              static_cast<const char*>(src->d) + chunks * 16, src->bytes() % 16,
+                              hipMemcpyDeviceToDevice, dst->stream),
+               W);
     return QS_OK;
 }
 

@@ -178,6 +178,28 @@ int qs_program_run(qs_program* p, qs_state* st) {
     return qs::fused_run(p->P, st, p->grid_cap);
 }
 
+int qs_program_run_timed(qs_program* p, qs_state* st, double* pass_ms) {
+    constexpr const char* W = "qpp_amd::program";
+    if (!p || !st || !st->d || !pass_ms) return fail(QS_ERR_INVALID_HANDLE, W);
+    const size_t np = p->P.passes.size();
+    std::vector<hipEvent_t> ev(np + 1, nullptr);
+    int rc = QS_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) rc = fail(QS_ERR_DEVICE, W, "hipEventCreate");
+    if (rc == QS_OK) rc = qs::fused_run(p->P, st, p->grid_cap, ev.data());
+    if (rc == QS_OK && hipEventSynchronize(ev[np]) != hipSuccess)
+        rc = fail(QS_ERR_DEVICE, W, "hipEventSynchronize");
+    for (size_t i = 0; rc == QS_OK && i < np; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess)
+            rc = fail(QS_ERR_DEVICE, W, "hipEventElapsedTime");
+        pass_ms[i] = ms;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
 int qs_program_info(const qs_program* p, uint64_t* num_gates, uint64_t* num_passes,
                     uint64_t* num_phases) {
     if (!p) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");

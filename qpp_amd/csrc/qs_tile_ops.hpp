@@ -87,11 +87,10 @@ struct RBof<16> {
 // gate arithmetic of a dense 1-bit op).
 // ---------------------------------------------------------------------------
 template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_u1(const cxt<R>* x, cxt<R>* y, const FOp* op,
+__device__ __forceinline__ void do_u1(const cxt<R>* x, cxt<R>* y, uint32_t regok,
                                       const double* mu, bool tok) {
     const cxt<R> u00 = opm<R>(mu, 0), u10 = opm<R>(mu, 1);
     const cxt<R> u01 = opm<R>(mu, 2), u11 = opm<R>(mu, 3);
-    const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if ((rho >> J) & 1) continue;
@@ -144,8 +143,7 @@ __device__ __forceinline__ void copy_regs(const cxt<R>* x, cxt<R>* y) {
 }
 
 template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_x(const cxt<R>* x, cxt<R>* y, const FOp* op, bool tok) {
-    const uint32_t regok = op->regok;
+__device__ __forceinline__ void do_x(const cxt<R>* x, cxt<R>* y, uint32_t regok, bool tok) {
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if ((rho >> J) & 1) continue;
@@ -162,10 +160,9 @@ __device__ __forceinline__ void do_x(const cxt<R>* x, cxt<R>* y, const FOp* op, 
 }
 
 template <typename R, int NR, int J, bool CT>
-__device__ __forceinline__ void do_diag_r(const cxt<R>* x, cxt<R>* y, const FOp* op,
+__device__ __forceinline__ void do_diag_r(const cxt<R>* x, cxt<R>* y, uint32_t regok,
                                           const double* mu, bool tok) {
     const cxt<R> d0 = opm<R>(mu, 0), d1 = opm<R>(mu, 1);
-    const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if (CT && !((regok >> rho) & 1u)) {
@@ -179,9 +176,8 @@ __device__ __forceinline__ void do_diag_r(const cxt<R>* x, cxt<R>* y, const FOp*
 
 // diagonal factor d (per lane or uniform) on every element
 template <typename R, int NR, bool CT>
-__device__ __forceinline__ void do_scale(const cxt<R>* x, cxt<R>* y, const FOp* op, cxt<R> d,
+__device__ __forceinline__ void do_scale(const cxt<R>* x, cxt<R>* y, uint32_t regok, cxt<R> d,
                                          bool tok) {
-    const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if (CT && !((regok >> rho) & 1u)) {
@@ -194,14 +190,12 @@ __device__ __forceinline__ void do_scale(const cxt<R>* x, cxt<R>* y, const FOp* 
 }
 
 template <typename R, int NR, int J0, int J1, bool CT>
-__device__ __forceinline__ void do_u2(const cxt<R>* x, cxt<R>* y, const FOp* op,
-                                      const cxt<R>* __restrict__ mats, bool tok) {
+__device__ __forceinline__ void do_u2(const cxt<R>* x, cxt<R>* y, uint32_t regok,
+                                      const cxt<R>* __restrict__ M, bool tok) {
     // column-major 4x4; gate index bit 0 <-> register bit J0, bit 1 <-> J1
-    const cxt<R>* M = mats + op->moff;
     cxt<R> m[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) m[i] = M[i];
-    const uint32_t regok = op->regok;
 #pragma unroll
     for (int rho = 0; rho < NR; ++rho) {
         if (((rho >> J0) & 1) || ((rho >> J1) & 1)) continue;
@@ -250,50 +244,49 @@ __device__ __forceinline__ void do_u2(const cxt<R>* x, cxt<R>* y, const FOp* op,
     }
 
 template <typename R, int NR, bool CT>
-__device__ __forceinline__ void run_op_ct(const cxt<R>* x, cxt<R>* y, const FOp* op,
-                                          uint32_t code, const double* mu,
+__device__ __forceinline__ void run_op_ct(const cxt<R>* x, cxt<R>* y, const OpHead& h,
+                                          const FOp* op, const double* mu,
                                           const cxt<R>* mats, uint32_t t, uint64_t base,
                                           bool tok) {
+    const uint32_t code = h.code, regok = h.regok;
     const int j0 = (code >> 4) & 3, j1 = (code >> 6) & 3;
     switch (code & 15u) {
         case OP_U1:
-            QS_J_CASES(do_u1, x, y, op, mu, tok)
+            QS_J_CASES(do_u1, x, y, regok, mu, tok)
             break;
         case OP_X:
-            QS_J_CASES(do_x, x, y, op, tok)
+            QS_J_CASES(do_x, x, y, regok, tok)
             break;
         case OP_DIAG_R:
-            QS_J_CASES(do_diag_r, x, y, op, mu, tok)
+            QS_J_CASES(do_diag_r, x, y, regok, mu, tok)
             break;
         case OP_DIAG_T: {
-            const bool one = (t & op->tbit) != 0;
+            const bool one = (t & h.tbit) != 0;
             const cxt<R> d = sel<R>(one, opm<R>(mu, 1), opm<R>(mu, 0));
-            do_scale<R, NR, CT>(x, y, op, d, tok);
+            do_scale<R, NR, CT>(x, y, regok, d, tok);
             break;
         }
         case OP_DIAG_O: {
             const cxt<R> d = (base & op->dbit) ? opm<R>(mu, 1) : opm<R>(mu, 0);
-            do_scale<R, NR, CT>(x, y, op, d, tok);
+            do_scale<R, NR, CT>(x, y, regok, d, tok);
             break;
         }
-        case OP_U1B:
-            do_u1b<R, NR>(x, y, mats + op->moff);
-            break;
         case OP_U2: {
+            const cxt<R>* M = mats + h.moff;
             switch (j0 * 4 + j1) {
-                case 0 * 4 + 1: do_u2<R, NR, 0, 1, CT>(x, y, op, mats, tok); break;
-                case 0 * 4 + 2: do_u2<R, NR, 0, 2, CT>(x, y, op, mats, tok); break;
-                case 1 * 4 + 2: do_u2<R, NR, 1, 2, CT>(x, y, op, mats, tok); break;
+                case 0 * 4 + 1: do_u2<R, NR, 0, 1, CT>(x, y, regok, M, tok); break;
+                case 0 * 4 + 2: do_u2<R, NR, 0, 2, CT>(x, y, regok, M, tok); break;
+                case 1 * 4 + 2: do_u2<R, NR, 1, 2, CT>(x, y, regok, M, tok); break;
                 case 0 * 4 + 3:
-                    if constexpr (NR > 8) do_u2<R, NR, 0, 3, CT>(x, y, op, mats, tok);
+                    if constexpr (NR > 8) do_u2<R, NR, 0, 3, CT>(x, y, regok, M, tok);
                     else copy_regs<R, NR>(x, y);
                     break;
                 case 1 * 4 + 3:
-                    if constexpr (NR > 8) do_u2<R, NR, 1, 3, CT>(x, y, op, mats, tok);
+                    if constexpr (NR > 8) do_u2<R, NR, 1, 3, CT>(x, y, regok, M, tok);
                     else copy_regs<R, NR>(x, y);
                     break;
                 case 2 * 4 + 3:
-                    if constexpr (NR > 8) do_u2<R, NR, 2, 3, CT>(x, y, op, mats, tok);
+                    if constexpr (NR > 8) do_u2<R, NR, 2, 3, CT>(x, y, regok, M, tok);
                     else copy_regs<R, NR>(x, y);
                     break;
                 default: copy_regs<R, NR>(x, y); break;
@@ -306,24 +299,42 @@ __device__ __forceinline__ void run_op_ct(const cxt<R>* x, cxt<R>* y, const FOp*
     }
 }
 
-// y := op(x).  `mu`: the 2x2 matrix (or the diagonal) the op applies: op->m,
-// or a per-trajectory override of it.  The dispatch needs one scalar load
-// (`code`); the case then loads its matrix.
+// y := op(x), the op's head already in registers.  `mu`: the 2x2 matrix (or
+// the diagonal) the op applies: h.m, or a per-trajectory override of it.
+// Bundles, the commonest op, are never controlled and go first, on a path
+// that holds nothing but their arithmetic.
 template <typename R, int NR>
-__device__ __forceinline__ void run_op(const cxt<R>* x, cxt<R>* y, const FOp* op,
-                                       const double* mu, const cxt<R>* mats, uint32_t t,
-                                       uint64_t base) {
-    const uint32_t code = op->code;
+__device__ __forceinline__ void run_op_h(const cxt<R>* x, cxt<R>* y, const OpHead& h,
+                                         const FOp* op, const double* mu, const cxt<R>* mats,
+                                         uint32_t t, uint64_t base) {
+    const uint32_t code = h.code;
+    if ((code & 15u) == OP_U1B) {
+        do_u1b<R, NR>(x, y, mats + h.moff);
+        return;
+    }
     if ((code & kCodeOut) && (base & op->omask) != op->oval) {  // out-of-tile control
         copy_regs<R, NR>(x, y);
         return;
     }
     if (code & kCodeCtrl) {
-        const bool tok = (t & op->tmask) == op->tval;
-        run_op_ct<R, NR, true>(x, y, op, code, mu, mats, t, base, tok);
+        const bool tok = (t & h.tmask) == h.tval;
+        run_op_ct<R, NR, true>(x, y, h, op, mu, mats, t, base, tok);
     } else {
-        run_op_ct<R, NR, false>(x, y, op, code, mu, mats, t, base, true);
+        run_op_ct<R, NR, false>(x, y, h, op, mu, mats, t, base, true);
     }
+}
+
+__device__ __forceinline__ OpHead load_head(const FOp* op) {
+    return *reinterpret_cast<const OpHead*>(op);
+}
+
+// y := op(x), loading the op's head here
+template <typename R, int NR>
+__device__ __forceinline__ void run_op(const cxt<R>* x, cxt<R>* y, const FOp* op,
+                                       const double* mu, const cxt<R>* mats, uint32_t t,
+                                       uint64_t base) {
+    const OpHead h = load_head(op);
+    run_op_h<R, NR>(x, y, h, op, mu, mats, t, base);
 }
 
 // Global index bits of tile number `tile` (the pass's out-of-tile bits as
