@@ -475,8 +475,8 @@ int qs_batch_destroy(qs_batch* b);
  * QEngine(qc, seed).execute(shots) (classes/qengine.hpp:1341-1405) in
  * complex128 or complex64, with the engine's statistics as result.  `lanes`
  * worker threads each own a HIP stream; a free lane takes the oldest queued
- * request whose device footprint (state, its pre-measurement copy and the
- * FP64 sampling vectors) fits what is left of `hbm_budget` bytes (0 = 90% of
+ * request whose device footprint (qs_server_footprint) fits what is left of
+ * `hbm_budget` bytes (0 = 90% of
  * the free device memory), so requests run concurrently and new ones are
  * admitted as lanes and memory free up.  Results are identical to running
  * the request alone. */
@@ -486,6 +486,13 @@ int qs_server_create(uint32_t lanes, uint64_t hbm_budget, qs_server** out);
  * can never fit the budget. */
 int qs_server_submit(qs_server* s, const qs_circuit* qc, qs_dtype dtype,
                      uint64_t shots, uint64_t seed, uint64_t* ticket);
+/* The device bytes a request is admitted against: the most it can hold at
+ * once on any path of execute(shots) (the state, the two saved states of
+ * post-selection retries, a measurement's contracted output, and the
+ * sampler's workspace: 3.5 s D + 16 D + small terms, s the bytes per
+ * amplitude). */
+int qs_server_footprint(const qs_circuit* qc, qs_dtype dtype, uint64_t shots,
+                        uint64_t* bytes);
 /* Blocks until done; times_ms[3] (optional): queued, running, total. */
 int qs_server_wait(qs_server* s, uint64_t ticket, double* times_ms);
 /* Results of a finished request: outcomes (n x nc dits, QEngine::get_stats

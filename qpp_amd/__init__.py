@@ -1203,6 +1203,16 @@ class Server:
         self._nc = {}
         _check(L.qs_server_create(int(lanes), int(hbm_budget), byref(self._h)))
 
+    @staticmethod
+    def footprint(qc, dtype=C128, shots=1024):
+        """Device bytes a request is admitted against (qs_server_footprint)."""
+        L = lib()
+        L.qs_server_footprint.argtypes = [POINTER(_Circuit), c_int, c_uint64, POINTER(c_uint64)]
+        cs = qc.struct()
+        b = c_uint64()
+        _check(L.qs_server_footprint(ctypes.byref(cs), dtype, shots, byref(b)))
+        return b.value
+
     def submit(self, qc, dtype=C128, shots=1024, seed=5489):
         cs = qc.struct()
         t = c_uint64()

@@ -585,18 +585,24 @@ struct qs_engine {
         return qs_state_clone(st, &sn.st);
     }
 
-    void restore(Snap& sn, bool take) {
-        qs_state_destroy(st);
+    // The engine's state becomes the snapshot's (taken over, or cloned).  A
+    // failed clone leaves the current state in place and reports why.
+    int restore(Snap& sn, bool take) {
         if (take) {
+            qs_state_destroy(st);
             st = sn.st;
             sn.st = nullptr;
         } else {
-            qs_state_clone(sn.st, &st);
+            qs_state* c = nullptr;
+            if (int rc = qs_state_clone(sn.st, &c)) return rc;
+            qs_state_destroy(st);
+            st = c;
         }
         dits = sn.dits;
         subsys = sn.subsys;
         probs = sn.probs;
         ps_ok = sn.ps_ok;
+        return QS_OK;
     }
 
     // execute_steps_with_postselection_retries_, qengine.hpp:316-345: every
@@ -614,9 +620,9 @@ struct qs_engine {
                 found = true;
                 if (rep + 1 < reps) rc = snapshot(ok);
             }
-            if (!rc && rep + 1 < reps) restore(copy, false);
+            if (!rc && rep + 1 < reps) rc = restore(copy, false);
         }
-        if (!rc && ok.st) restore(ok, true);
+        if (!rc && ok.st) rc = restore(ok, true);
         qs_state_destroy(ok.st);
         qs_state_destroy(copy.st);
         return rc;
@@ -967,7 +973,7 @@ int qs_engine_destroy(qs_engine* e) {
 }
 
 int qs_engine_execute(qs_engine* e, uint64_t reps) {
-    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::execute()");
+    if (!e || !e->st) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::execute()");
     qs::clear_error();
     if (reps == 0) return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()", "reps");
     return e->execute(reps);
@@ -1003,7 +1009,7 @@ int qs_engine_get_probs(const qs_engine* e, double* probs) {
 }
 
 int qs_engine_get_state(const qs_engine* e, qs_state** out) {
-    if (!e || !out) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_state()");
+    if (!e || !out || !e->st) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::get_state()");
     return qs_state_clone(e->st, out);
 }
 
@@ -1031,7 +1037,7 @@ int qs_engine_reset(qs_engine* e) {
     std::vector<uint64_t> dims(e->nq, e->d);
     qs_state* s = nullptr;
     {
-        const qs::StreamScope on(e->st->stream);
+        const qs::StreamScope on(e->st ? e->st->stream : qs::current_stream());
         if (int rc = qs_state_create(e->dtype, e->kind, e->nq, dims.data(), &s)) return rc;
     }
     qs_state_destroy(e->st);

@@ -79,14 +79,25 @@ struct Request {
     uint32_t lane = 0;
 };
 
-// Device bytes a request holds at most: the state, the pre-measurement
-// copy the sampling tail keeps (qengine.hpp:414-473), and the FP64
-// probability / exact-replay vectors of the sampler.
-uint64_t footprint_of(const qs_circuit* c, qs_dtype dt) {
+// Device bytes a request can hold at once, over every path QEngine's
+// execute(shots) takes (qengine.hpp:316-481), with D the dimension and s the
+// bytes per amplitude:
+//   the state                                               s D
+//   post-selection retries: the saved state and the first
+//   successful rep's state (execute_with_retries)           2 s D
+//   a destructive measurement's contracted output           s D / 2
+//   the sampler's workspace: FP64 probabilities and the
+//   exact cumulative table (2 x 8 D), block tables and
+//   per-shot slots (qs_measure.hip)                         16 D + 64 D / 4096 + 24 shots
+// plus 2 MiB for the compiled passes' tables and small buffers.  The
+// sampling-tail path holds less (the state, its pre-measurement copy and the
+// workspace), so this bound covers it.
+uint64_t footprint_of(const qs_circuit* c, qs_dtype dt, uint64_t shots) {
     double D = 1;
     for (uint32_t i = 0; i < c->nq; ++i) D *= c->d;
     const double s = dt == QS_C128 ? 16 : 8;
-    return static_cast<uint64_t>(2 * s * D + 16 * D + (1 << 20));
+    const double ws = 16 * D + 64 * (D / 4096 + 1) + 24 * double(shots) + 12 * 256;
+    return static_cast<uint64_t>(3.5 * s * D + ws + (2 << 20));
 }
 
 }  // namespace
@@ -202,6 +213,14 @@ int qs_server_create(uint32_t lanes, uint64_t hbm_budget, qs_server** out) {
     return QS_OK;
 }
 
+// The device bytes a request is admitted against (footprint_of).
+int qs_server_footprint(const qs_circuit* qc, qs_dtype dtype, uint64_t shots, uint64_t* bytes) {
+    if (!qc || !bytes) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::server");
+    if (dtype != QS_C128 && dtype != QS_C64) return fail(QS_ERR_TYPE_MISMATCH, "qpp_amd::server", "dtype");
+    *bytes = footprint_of(qc, dtype, shots);
+    return QS_OK;
+}
+
 // Queues QEngine(circuit, seed).execute(shots); the circuit is copied.
 int qs_server_submit(qs_server* s, const qs_circuit* qc, qs_dtype dtype, uint64_t shots,
                      uint64_t seed, uint64_t* ticket) {
@@ -214,7 +233,7 @@ int qs_server_submit(qs_server* s, const qs_circuit* qc, qs_dtype dtype, uint64_
     r->dtype = dtype;
     r->shots = shots;
     r->seed = seed;
-    r->footprint = footprint_of(qc, dtype);
+    r->footprint = footprint_of(qc, dtype, shots);
     if (r->footprint > s->budget) return fail(QS_ERR_OUT_OF_MEMORY, W, "request exceeds the HBM budget");
     r->t_submit = Clock::now();
     {

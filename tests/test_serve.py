@@ -43,10 +43,11 @@ def test_results_alone_equal_results_in_the_mix(orc):
 
 
 def test_admission_respects_the_budget():
-    """A budget that holds one 18-qubit c128 request at a time still runs
-    every request; the peak held never exceeds the budget."""
+    """A budget of exactly one 18-qubit c128 request's footprint still runs
+    every request, one at a time; the peak held never exceeds the budget."""
     reqs = [(workloads.measured_circuit(qa, 18, 3, 50 + i), qa.C128, 11 + i) for i in range(4)]
-    budget = 2 * 16 * 2 ** 18 + 16 * 2 ** 18 + (1 << 20) + 4096
+    budget = qa.Server.footprint(reqs[0][0], qa.C128, 64)
+    assert budget >= 3.5 * 16 * 2 ** 18 + 16 * 2 ** 18
     srv = qa.Server(lanes=4, hbm_budget=budget)
     ts = [srv.submit(qc, dt, shots=64, seed=sd) for qc, dt, sd in reqs]
     res = [srv.wait(t) for t in ts]
