@@ -168,10 +168,22 @@ int qs_apply_ctrl_fan(qs_state* st, const double* A, uint64_t dimA,
                       const uint64_t* target, const uint64_t* shift);
 
 /* qpp::apply(rho, Ks, target, dims)  operations.hpp:476-560: the channel
- * sum_K K rho K^dagger, fused into one pass.  Ks: nK matrices, each
- * dimK x dimK, stored back to back. */
+ * sum_K K rho K^dagger, fused into one pass (one term per operator when the
+ * targets span more than 32 basis states).  Ks: nK matrices, each
+ * dimK x dimK, stored back to back.  A ket fails with
+ * QS_ERR_MATRIX_NOT_SQUARE, as in qpp. */
 int qs_apply_kraus(qs_state* rho, const double* Ks, uint32_t nK, uint64_t dimK,
                    uint32_t ntarget, const uint64_t* target);
+
+/* qpp::apply(A, Ks)  operations.hpp:422-464: sum_K K rho K^dagger over the
+ * whole system.  The Kraus operators may be rectangular: each is
+ * rowsK x colsK (colsK = the dimension of rho), stored back to back
+ * column-major; the result is a new rowsK x rowsK density matrix (one
+ * subsystem of dimension rowsK) in *out.  Errors in qpp's order:
+ * MatrixNotSquare (a ket), ZeroSize (no operators), DimsMismatchMatrix
+ * (colsK != dimension of rho). */
+int qs_apply_kraus_full(const qs_state* rho, const double* Ks, uint32_t nK,
+                        uint64_t rowsK, uint64_t colsK, qs_state** out);
 
 /* ------------------------------------------------------------------------ */
 /* Reductions and index maps (out-of-place: create a new handle).            */

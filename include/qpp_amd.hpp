@@ -264,6 +264,29 @@ inline cmat apply(const cmat& rho, const std::vector<cmat>& Ks, const std::vecto
                   idx d = 2) {
     return apply(rho, Ks, target, detail::dims_of(rho, d));
 }
+// qpp::apply(A, Ks), operations.hpp:422-464: the channel on the whole
+// system; the Kraus operators may be rectangular (Dout x D), the result is
+// Dout x Dout
+inline cmat apply(const cmat& rho, const std::vector<cmat>& Ks) {
+    if (rho.size() == 0) throw exception::ZeroSize("[WHERE: qpp::apply()] Object has zero size! [CONTEXT: A]");
+    if (rho.rows() != rho.cols())
+        throw exception::MatrixNotSquare("[WHERE: qpp::apply()] Matrix is not square! [CONTEXT: A]");
+    if (Ks.empty()) throw exception::ZeroSize("[WHERE: qpp::apply()] Object has zero size! [CONTEXT: Ks]");
+    if (Ks[0].cols() != rho.rows())
+        throw exception::DimsMismatchMatrix(
+            "[WHERE: qpp::apply()] Dimension(s) mismatch matrix size! [CONTEXT: Ks[0]]");
+    std::vector<cplx> buf;
+    for (const cmat& K : Ks) {
+        if (K.rows() != Ks[0].rows() || K.cols() != Ks[0].cols())
+            throw exception::DimsNotEqual("[WHERE: qpp::apply()] Dimensions not equal! [CONTEXT: K]");
+        buf.insert(buf.end(), K.v.begin(), K.v.end());
+    }
+    State s(rho, std::vector<idx>{rho.rows()});
+    qs_state* out = nullptr;
+    check(qs_apply_kraus_full(s.get(), reinterpret_cast<const double*>(buf.data()),
+                              static_cast<uint32_t>(Ks.size()), Ks[0].rows(), Ks[0].cols(), &out));
+    return State(out).download();
+}
 inline cmat applyCTRL(const cmat& state, const cmat& A, const std::vector<idx>& ctrl,
                       const std::vector<idx>& target, const std::vector<idx>& dims,
                       std::optional<std::vector<idx>> shift = std::nullopt) {

@@ -37,6 +37,7 @@ def lib():
                                      c_uint32, u64p, u64p, dp]
         L.orc_apply_ctrl_fan.argtypes = L.orc_apply_ctrl.argtypes
         L.orc_apply_kraus.argtypes = [dp, c_uint32, u64p, c_uint32, dp, c_uint32, u64p, dp]
+        L.orc_apply_kraus_full.argtypes = [dp, c_uint64, c_uint32, dp, c_uint64, dp]
         L.orc_ptrace.argtypes = [dp, c_int, c_uint32, u64p, c_uint32, u64p, dp]
         L.orc_ptranspose.argtypes = [dp, c_int, c_uint32, u64p, c_uint32, u64p, c_int, dp]
         L.orc_syspermute.argtypes = [dp, c_int, c_uint32, u64p, u64p, c_int, dp]
@@ -144,6 +145,19 @@ def apply_kraus(rho, Ks, target, dims=2):
     lib().orc_apply_kraus(sp, len(d), dp_, len(Ks), kb.ctypes.data_as(POINTER(c_double)),
                           len(t), tp, op)
     return _shape(o, True, D)
+
+
+def apply_kraus_full(rho, Ks):
+    """apply(A, Ks), operations.hpp:422-464 (rectangular Kraus operators
+    allowed): sum_K K rho K^dagger."""
+    s, sp, _ = _buf(rho)
+    D = np.asarray(rho).shape[0]
+    Dout = np.asarray(Ks[0]).shape[0]
+    kb = np.ascontiguousarray(np.concatenate(
+        [np.asfortranarray(np.asarray(K, dtype=np.complex128)).reshape(-1, order="F") for K in Ks]))
+    o, op = _out(Dout * Dout)
+    lib().orc_apply_kraus_full(sp, D, len(Ks), kb.ctypes.data_as(POINTER(c_double)), Dout, op)
+    return _shape(o, True, Dout)
 
 
 def ptrace(A, target, dims=2):

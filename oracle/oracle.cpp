@@ -1575,6 +1575,33 @@ int orc_apply_kraus(const double* rho, uint32_t ndims, const uint64_t* dims,
     return 0;
 }
 
+// apply(A, Ks), operations.hpp:422-464: result (Dout x Dout) =
+// sum_K K * A * adjoint(K), K of shape Dout x D (rectangular allowed); the
+// products in the order K * A first, then times adjoint(K)
+int orc_apply_kraus_full(const double* rho, uint64_t D, uint32_t nK, const double* Ks,
+                         uint64_t Dout, double* out) {
+    const cvec A = load(rho, D * D);
+    cvec result(Dout * Dout, cplx(0, 0));
+    cvec T(Dout * D);
+    for (uint32_t k = 0; k < nK; ++k) {
+        const cvec K = load(Ks + 2 * uint64_t(k) * Dout * D, Dout * D);  // column-major
+        for (idx j = 0; j < D; ++j)
+            for (idx i = 0; i < Dout; ++i) {
+                cplx acc(0, 0);
+                for (idx a = 0; a < D; ++a) acc += K[i + a * Dout] * A[a + j * D];
+                T[i + j * Dout] = acc;
+            }
+        for (idx j = 0; j < Dout; ++j)
+            for (idx i = 0; i < Dout; ++i) {
+                cplx acc(0, 0);
+                for (idx b = 0; b < D; ++b) acc += T[i + b * Dout] * std::conj(K[j + b * Dout]);
+                result[i + j * Dout] += acc;
+            }
+    }
+    store(result, out);
+    return 0;
+}
+
 int orc_ptrace(const double* A, int is_rho, uint32_t ndims,
                const uint64_t* dims, uint32_t ntarget, const uint64_t* target,
                double* out) {

@@ -43,6 +43,11 @@ int orc_apply_ctrl_fan(const double* state, int is_rho, uint32_t ndims,
                        const uint64_t* target, const uint64_t* shift,
                        double* out);
 
+/* apply(A, Ks): operations.hpp:422-464; Ks are Dout x D (rectangular
+ * allowed), column-major, back to back; out is Dout x Dout. */
+int orc_apply_kraus_full(const double* rho, uint64_t D, uint32_t nK, const double* Ks,
+                         uint64_t Dout, double* out);
+
 /* apply(rho, Ks, target, dims): operations.hpp:476-560. */
 int orc_apply_kraus(const double* rho, uint32_t ndims, const uint64_t* dims,
                     uint32_t nK, const double* Ks, uint32_t ntarget,

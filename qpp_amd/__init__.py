@@ -323,6 +323,25 @@ class State:
                                     len(Ks), dim, len(t), tp))
         return self
 
+    def apply_kraus_full(self, Ks):
+        """qpp::apply(rho, Ks) over the whole system (operations.hpp:422-464):
+        a new density matrix of dimension Ks[0].shape[0] (one subsystem)."""
+        Ks = [np.asarray(K, dtype=np.complex128) for K in Ks]
+        if not Ks:
+            raise ZeroSize("[WHERE: qpp::apply()] Object has zero size! [CONTEXT: Ks]")
+        for K in Ks:
+            if K.shape != Ks[0].shape:
+                raise DimsNotEqual("[WHERE: qpp::apply()] Dimensions not equal! [CONTEXT: K]")
+        buf = np.ascontiguousarray(np.concatenate([np.asfortranarray(K).reshape(-1, order="F")
+                                                   for K in Ks]))
+        L = lib()
+        L.qs_apply_kraus_full.argtypes = [c_void_p, POINTER(c_double), c_uint32, c_uint64,
+                                          c_uint64, POINTER(c_void_p)]
+        out = c_void_p()
+        _check(L.qs_apply_kraus_full(self._h, buf.ctypes.data_as(POINTER(c_double)), len(Ks),
+                                     Ks[0].shape[0], Ks[0].shape[1], byref(out)))
+        return State._wrap(out)
+
     def ptrace(self, target):
         t, tp = _u64(target)
         h = c_void_p()
@@ -452,9 +471,18 @@ def _as_state(state, dims, dtype=C128):
     return State.from_array(A, dims, dtype)
 
 
-def apply(state, A, target, dims=2):
+def apply(state, A, target=None, dims=2):
     """qpp::apply(state, A, target, dims|d) -- or, when A is a list of Kraus
-    operators, qpp::apply(rho, Ks, target, dims|d)."""
+    operators, qpp::apply(rho, Ks, target, dims|d), and without a target
+    qpp::apply(rho, Ks) on the whole system (rectangular Ks allowed)."""
+    if isinstance(A, (list, tuple)) and target is None:
+        rho = np.asarray(state)
+        if rho.ndim != 2 or rho.shape[0] != rho.shape[1]:
+            raise MatrixNotSquare("[WHERE: qpp::apply()] Matrix is not square! [CONTEXT: A]")
+        st = State.from_array(rho, [rho.shape[0]])
+        return st.apply_kraus_full(A).download()
+    if target is None:
+        raise TypeError("apply() needs a target for a gate")
     if isinstance(A, (list, tuple)):
         dims = _dims_of(state, dims)
         st = _as_state(state, dims)

@@ -25,6 +25,7 @@ SOURCES = [
     "qs_batch.hip",
     "qs_serve.hip",
     "qs_io.hip",
+    "qs_kraus.hip",
 ]
 
 FLAGS = [

@@ -143,8 +143,8 @@ PYBIND11_MODULE(pyqpp, m) {
             if (py::isinstance<py::list>(A) || py::isinstance<py::tuple>(A)) {
                 std::vector<cmat> Ks;
                 for (auto k : A) Ks.push_back(to_cmat(k));
-                if (target.is_none())  // apply(A, Ks): the whole matrix
-                    return to_np(apply(s, Ks, {0}, std::vector<idx>{s.rows()}));
+                if (target.is_none())  // apply(A, Ks): the whole system
+                    return to_np(qpp_amd::apply(s, Ks));
                 return to_np(apply(s, Ks, idx_list(target), dims_for(s, dims, d)));
             }
             return to_np(apply(s, to_cmat(A), idx_list(target), dims_for(s, dims, d)));

@@ -558,13 +558,16 @@ int qs_apply_ctrl_fan(qs_state* st, const double* A, uint64_t dimA, uint32_t nct
 
 // qpp::apply(rho, Ks, target, dims), operations.hpp:476-560, fused into one
 // pass: the superoperator S = sum_K conj(K) (x) K acts on the (column digits,
-// row digits) of the targets of vec(rho).
+// row digits) of the targets of vec(rho).  Targets spanning more than 32
+// basis states take qs::kraus_per_operator (qs_kraus.hip).
 int qs_apply_kraus(qs_state* st, const double* Ks, uint32_t nK, uint64_t dimK,
                    uint32_t ntarget, const uint64_t* target) {
     constexpr const char* W = "qpp::apply()";
     if (!valid(st)) return fail(QS_ERR_INVALID_HANDLE, W, "A");
     qs::clear_error();
     if (ntarget == 0) return fail(QS_ERR_ZERO_SIZE, W, "target");
+    // a ket is not a square matrix (operations.hpp:493-496)
+    if (st->kind != QS_RHO) return fail(QS_ERR_MATRIX_NOT_SQUARE, W, "A");
     const auto tv = to_vec(target, ntarget);
     if (!check_subsys(tv, st->dims))
         return fail(QS_ERR_SUBSYS_MISMATCH_DIMS, W, "dims/target");
@@ -572,9 +575,9 @@ int qs_apply_kraus(qs_state* st, const double* Ks, uint32_t nK, uint64_t dimK,
     uint64_t DK = 1;
     for (uint64_t t : tv) DK *= st->dims[t];
     if (dimK != DK) return fail(QS_ERR_MATRIX_MISMATCH_SUBSYS, W, "dims/Ks[0]/target");
-    if (st->kind != QS_RHO)
-        return fail(QS_ERR_NOT_IMPLEMENTED, W, "Kraus channel on a ket");
-    if (DK > 32) return fail(QS_ERR_NOT_IMPLEMENTED, W, "Kraus operator above 5 qubits");
+    // above 32 target basis states the superoperator (DK^2 x DK^2) is too
+    // large for one pass: one term K rho K^dagger per operator instead
+    if (DK > 32) return qs::kraus_per_operator(st, Ks, nK, DK, ntarget, target);
     const uint64_t DS = DK * DK;
     // S[(a1, b1), (a2, b2)] = sum_K conj(K[a1, a2]) K[b1, b2], column-major
     std::vector<double> S(2 * DS * DS, 0.0);

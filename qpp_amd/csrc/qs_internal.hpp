@@ -68,6 +68,10 @@ bool force_exact_sampling();
 // qpp::measure_seq with the sampled basis index given (qs_measure.hip)
 int measure_seq_given(qs_state* st, uint32_t ntarget, const uint64_t* target,
                       int destructive, uint64_t index, uint64_t* results, double* probs);
+// sum_K K rho K^dagger on the targets, one term per operator (qs_kraus.hip);
+// validation done by the caller
+int kraus_per_operator(qs_state* st, const double* Ks, uint32_t nK, uint64_t DK,
+                       uint32_t ntarget, const uint64_t* target);
 // creates states on `s` for the scope (outputs derived from a state live on
 // that state's stream)
 struct StreamScope {
