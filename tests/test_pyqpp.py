@@ -57,6 +57,10 @@ def test_operations_vs_oracle(orc):
     g = 0.3
     Ks = [np.diag([1, np.sqrt(1 - g)]), np.array([[0, np.sqrt(g)], [0, 0]])]
     assert np.max(np.abs(pyqpp.apply(rho, Ks, [2], [2] * 4) - orc.apply_kraus(rho, Ks, [2], [2] * 4))) <= 1e-12
+    # apply(A, Ks) on the whole system, rectangular operators: <i| (x) I traces
+    # out the first qubit
+    Kt = [np.kron(np.eye(2)[i:i + 1, :], np.eye(8)) for i in range(2)]
+    assert np.max(np.abs(pyqpp.apply(rho, Kt) - orc.apply_kraus_full(rho, Kt))) <= 1e-12
     U1 = haar(2, rng)
     got = pyqpp.applyCTRL(psi, U1, [0, 5], [2], dims, shift=[1, 0])
     assert np.max(np.abs(got - col(orc.applyCTRL(psi, U1, [0, 5], [2], dims, [1, 0])))) <= 1e-12
