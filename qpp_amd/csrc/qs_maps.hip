@@ -6,7 +6,8 @@
 //                bit set holds the low output bits and the output bits fed by
 //                the low input bits, so both the gather from HBM and the
 //                scatter to HBM run over contiguous 2^L-element segments; the
-//                transposition happens in LDS.
+//                transposition happens in LDS.  The per-element index parts
+//                come from a table k_bitperm_table builds once per call.
 //   k_digitperm  the same map for qudit states (mixed-radix digits).
 //   k_ket_pt     ptranspose of a ket: out(i, j) = psi(i') conj(psi(j')).
 //   k_ptrace     partial trace; `LANES` lanes cooperate on one output entry
@@ -60,59 +61,95 @@ __device__ __forceinline__ uint64_t gather_bits(uint64_t y, const uint8_t* src,
 constexpr uint32_t kBpMaxTileBits = 12;
 constexpr int kBpPer = 1 << (kBpMaxTileBits - 8);
 
-template <typename T>
+// The part of every index that comes from an element's position e inside
+// the tile (the bits in T); G and the LDS packing act bit by bit, so the
+// part from the tile number (bits outside T) is simply OR-ed in.  Both
+// parts come from a table built once per call (it stays in L2): one entry
+// per position e, and the tile number's contribution byte by byte.
+struct BpEntry {
+    uint64_t in_off, out_off;  // input / output index bits from e
+    uint32_t ld_pk, st_pk;     // LDS slots of the e-th load / store
+};
+constexpr int kBpTileBytes = 8;  // up to 64 tile-number bits, 8 at a time
+struct BpTable {
+    uint64_t xt[kBpTileBytes][256];  // input bits of tile-number byte c = v
+    uint64_t yt[kBpTileBytes][256];  // output bits of tile-number byte c = v
+    BpEntry e[1 << kBpMaxTileBits];
+};
+
+__global__ __launch_bounds__(256) void k_bitperm_table(BitPerm bp, BpTable* __restrict__ tab) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x < kBpTileBytes) {  // tile-number byte blockIdx.x, value threadIdx.x
+        const uint32_t c = blockIdx.x, v = threadIdx.x;
+        uint64_t x = 0, y = 0;
+        for (uint32_t k = 8 * c; k < 8 * c + 8 && k < bp.nrest; ++k) {
+            const uint64_t bit = (v >> (k - 8 * c)) & 1;
+            y |= bit << bp.rest[k];
+            x |= bit << bp.src[bp.rest[k]];
+        }
+        tab->xt[c][v] = x;
+        tab->yt[c][v] = y;
+    }
+    if (e >= (1u << bp.tbits)) return;
+    uint64_t yl = 0, ys = 0;
+    for (uint32_t k = 0; k < bp.tbits; ++k) {
+        yl |= uint64_t((e >> k) & 1) << bp.load_out[k];
+        ys |= uint64_t((e >> k) & 1) << bp.store_out[k];
+    }
+    uint32_t pl = 0, pst = 0;
+    for (uint32_t k = 0; k < bp.tbits; ++k) {
+        pl |= uint32_t((yl >> bp.tpos[k]) & 1) << k;
+        pst |= uint32_t((ys >> bp.tpos[k]) & 1) << k;
+    }
+    BpEntry r;
+    r.in_off = gather_bits(yl, bp.src, bp.nb);
+    r.out_off = ys;
+    r.ld_pk = pl;
+    r.st_pk = pst;
+    tab->e[e] = r;
+}
+
+// One tile per workgroup over a grid of all tiles (the dispatcher then
+// walks the state in order, which streams HBM better than a persistent
+// grid): every load of the tile is issued, the elements go to their LDS
+// slots, and after one barrier every store.
+template <typename T, int PER>
 __global__ __launch_bounds__(256) void k_bitperm(const T* __restrict__ in,
                                                  T* __restrict__ out,
-                                                 uint64_t ntiles, BitPerm bp) {
+                                                 const BpTable* __restrict__ tab,
+                                                 uint64_t tile0, uint32_t tbits, uint32_t nrest) {
     extern __shared__ unsigned char smem_raw[];
     T* s = reinterpret_cast<T*>(smem_raw);
-    const uint32_t tsz = 1u << bp.tbits;
+    const uint32_t tsz = 1u << tbits;
     const uint32_t t = threadIdx.x;
-    // G and the LDS packing act bit by bit, so every index splits into a
-    // part from the tile number (bits outside T) and a part from the
-    // element's position e = t + 256 i inside the tile (bits in T).  The
-    // second part is the same for every tile: computed once here, for the
-    // tiles this (persistent) workgroup visits.
-    uint64_t in_off[kBpPer], out_off[kBpPer];
-    uint32_t ld_pk[kBpPer], st_pk[kBpPer];
-#pragma unroll
-    for (int i = 0; i < kBpPer; ++i) {
-        const uint32_t e = t + 256u * i;
-        in_off[i] = out_off[i] = 0;
-        ld_pk[i] = st_pk[i] = 0;
-        if (e >= tsz) continue;
-        uint64_t yl = 0, ys = 0;
-        for (uint32_t k = 0; k < bp.tbits; ++k) {
-            yl |= uint64_t((e >> k) & 1) << bp.load_out[k];
-            ys |= uint64_t((e >> k) & 1) << bp.store_out[k];
-        }
-        uint32_t pl = 0, pst = 0;
-        for (uint32_t k = 0; k < bp.tbits; ++k) {
-            pl |= uint32_t((yl >> bp.tpos[k]) & 1) << k;
-            pst |= uint32_t((ys >> bp.tpos[k]) & 1) << k;
-        }
-        in_off[i] = gather_bits(yl, bp.src, bp.nb);
-        out_off[i] = ys;
-        ld_pk[i] = pl;
-        st_pk[i] = pst;
+    const uint64_t tile = tile0 + blockIdx.x;
+    uint64_t ybase = 0, xbase = 0;
+    for (uint32_t c = 0; 8 * c < nrest; ++c) {
+        const uint32_t v = uint32_t(tile >> (8 * c)) & 255u;
+        xbase |= tab->xt[c][v];
+        ybase |= tab->yt[c][v];
     }
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        // output bits outside T from the tile number, and their input bits
-        uint64_t ybase = 0, xbase = 0;
-        for (uint32_t k = 0; k < bp.nrest; ++k) {
-            const uint64_t bit = (tile >> k) & 1;
-            ybase |= bit << bp.rest[k];
-            xbase |= bit << bp.src[bp.rest[k]];
-        }
+    // PER = tsz / 256 elements per thread when tsz >= 256; a smaller tile
+    // leaves threads without elements (they read entry 0 and store nothing)
+    uint64_t oo[PER];
+    uint32_t lp[PER], sp[PER];
+    T v[PER];
 #pragma unroll
-        for (int i = 0; i < kBpPer; ++i)
-            if (t + 256u * i < tsz) s[ld_pk[i]] = in[xbase | in_off[i]];
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < kBpPer; ++i)
-            if (t + 256u * i < tsz) out[ybase | out_off[i]] = s[st_pk[i]];
-        __syncthreads();
+    for (int i = 0; i < PER; ++i) {  // all loads in flight together
+        const uint32_t e = t + 256u * i < tsz ? t + 256u * i : 0u;
+        const BpEntry* en = tab->e + e;
+        oo[i] = en->out_off;
+        lp[i] = en->ld_pk;
+        sp[i] = en->st_pk;
+        v[i] = in[xbase | en->in_off];
     }
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (t + 256u * i < tsz) s[lp[i]] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (t + 256u * i < tsz) out[ybase | oo[i]] = s[sp[i]];
 }
 
 // ---------------------------------------------------------------------------
@@ -331,12 +368,30 @@ int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t
     if (bp.tbits > kBpMaxTileBits) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::bitperm", "tile");
     const uint64_t ntiles = uint64_t{1} << bp.nrest;
     const size_t shmem = sizeof(T) << bp.tbits;
-    // persistent: at most 8 workgroups per CU, each over several tiles
-    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(ntiles, 2048));
-    hipLaunchKernelGGL(k_bitperm<T>, dim3(grid), dim3(256), shmem,
-                       in->stream, static_cast<const T*>(in->d),
-                       static_cast<T*>(out->d), ntiles, bp);
-    QS_HIP(hipGetLastError(), "qpp_amd::bitperm");
+    constexpr const char* W = "qpp_amd::bitperm";
+    BpTable* tab = nullptr;
+    QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&tab), sizeof(BpTable), in->stream), W);
+    const uint32_t tgrid = std::max<uint32_t>(kBpTileBytes, ((1u << bp.tbits) + 255) / 256);
+    hipLaunchKernelGGL(k_bitperm_table, dim3(tgrid), dim3(256), 0, in->stream, bp, tab);
+    // one tile per workgroup, at most 2^30 workgroups per launch
+    for (uint64_t t0 = 0; t0 < ntiles; t0 += uint64_t{1} << 30) {
+        const uint64_t g = std::min<uint64_t>(ntiles - t0, uint64_t{1} << 30);
+        const dim3 gd(static_cast<uint32_t>(g));
+        const T* src = static_cast<const T*>(in->d);
+        T* dst = static_cast<T*>(out->d);
+        if (bp.tbits <= 10)
+            hipLaunchKernelGGL((k_bitperm<T, 4>), gd, dim3(256), shmem, in->stream, src, dst, tab,
+                               t0, bp.tbits, bp.nrest);
+        else if (bp.tbits == 11)
+            hipLaunchKernelGGL((k_bitperm<T, 8>), gd, dim3(256), shmem, in->stream, src, dst, tab,
+                               t0, bp.tbits, bp.nrest);
+        else
+            hipLaunchKernelGGL((k_bitperm<T, 16>), gd, dim3(256), shmem, in->stream, src, dst,
+                               tab, t0, bp.tbits, bp.nrest);
+    }
+    const hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(tab, in->stream);
+    if (e != hipSuccess) return qs::fail_hip(e, W);
     return QS_OK;
 }
 
