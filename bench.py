@@ -294,24 +294,51 @@ def bench_serve(args, qa):
         tickets.append(srv.submit(c, q["dtype"], 1024, q["seed"]))
         if (k + 1) % 50 == 0:
             log(f"serve: submitted {k + 1}/{nreq}, {srv.info()}")
-    lat, run_ms, stats = [], [], []
+    lat, run_ms, stats, bsz = [], [], [], []
     for t in tickets:
         w = srv.wait(t)
         lat.append(w[2])
         run_ms.append(w[1])
         stats.append(srv.stats(t))
+        bsz.append(srv.batch_size(t))
     wall = time.perf_counter() - start
     info = srv.info()
+    # a burst of small requests (12-14 qubits) submitted at once: they queue
+    # together and run batched
+    brng = np.random.default_rng(args.seed + 2)
+    burst = []
+    for k in range(32):
+        q = make(20000 + k, brng)
+        q["n"] = int(brng.integers(12, 15))
+        burst.append((workloads.measured_circuit(qa, q["n"], q["depth"], q["gseed"]), q))
+    tb = time.perf_counter()
+    bt = [srv.submit(c, q["dtype"], 1024, q["seed"]) for c, q in burst]
+    blat, bsizes = [], []
+    for t in bt:
+        blat.append(srv.wait(t)[2])
+        bsizes.append(srv.batch_size(t))
+    bwall = time.perf_counter() - tb
     srv.close()
+    burst_res = {"requests": len(burst), "circuits_per_s": round(len(burst) / bwall, 2),
+                 "p50_ms": round(float(np.percentile(blat, 50)), 2),
+                 "p99_ms": round(float(np.percentile(blat, 99)), 2),
+                 "batch_sizes": sorted(set(bsizes)),
+                 "launches": int(round(sum(1.0 / b for b in bsizes)))}
+    log(f"serve burst: {burst_res}")
     lat = np.array(lat)
     log(f"serve: {nreq} requests in {wall:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, "
         f"p99 {np.percentile(lat, 99):.1f} ms, {info}")
-    # each result alone == in the mix (a few requests re-run alone)
+    # each result alone == in the mix: a few requests (the batched ones
+    # first) served again, each on a server of its own
     same = 0
-    chk = list(range(0, nreq, max(1, nreq // 6)))[:6]
+    batched = [k for k in range(nreq) if bsz[k] > 1]
+    chk = (batched[:3] + [k for k in range(0, nreq, max(1, nreq // 6)) if k not in batched])[:6]
     for k in chk:
-        e = qa.QEngine(circs[k], dtype=reqs[k]["dtype"], seed=reqs[k]["seed"]).execute(1024)
-        same += int(e.get_stats() == stats[k][0])
+        s1 = qa.Server(lanes=1)
+        t1 = s1.submit(circs[k], reqs[k]["dtype"], 1024, reqs[k]["seed"])
+        s1.wait(t1)
+        same += int(s1.stats(t1) == stats[k])
+        s1.close()
     # precision of the complex64 requests: the gate part in c64 vs c128
     ok64, n64, worst = 0, 0, 0.0
     pass_bytes = 0.0
@@ -373,6 +400,12 @@ def bench_serve(args, qa):
                        "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
         "fraction_at_infidelity_le_1e-5": round(frac, 4),
+        "batched": {"requests": len(batched),
+                    "launches": int(sum(1.0 / bsz[k] for k in batched) + 0.5),
+                    "max_batch": int(max(bsz)),
+                    "rule": "12-14 qubits, gates then one measurement of every qubit: the gate "
+                            "steps of every such queued request of one dtype run in one launch"},
+        "burst_small": burst_res,
         "c64_requests": n64, "c64_worst_infidelity": worst,
         "alone_equals_mix": f"{same}/{len(chk)}",
         "server": info,
@@ -791,7 +824,7 @@ def main():
     ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
     ap.add_argument("--cal-requests", type=int, default=40,
                     help="serve mode: closed-loop calibration requests")
-    ap.add_argument("--lanes", type=int, default=8, help="serve mode: concurrent lanes")
+    ap.add_argument("--lanes", type=int, default=4, help="serve mode: concurrent lanes (the device has 4 hardware queues)")
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c64",
                     help="batch mode: complex dtype of the trajectories")
     ap.add_argument("--traj-scale", type=float, default=1.0,

@@ -477,6 +477,14 @@ int qs_batch_plan(const qs_batch* b, uint32_t circuit, uint64_t seed,
                   double noise_p, uint64_t cap, uint64_t* n_ovr,
                   uint32_t* ovr_step, double* ovr_mats, uint64_t* xmask,
                   uint64_t* zmask, uint32_t* phase, double* u);
+/* No qpp counterpart: the gate steps of circuit circuit_of[i] (everything
+ * before its measurement, noiseless) applied to |0..0> and left in out[i],
+ * a ket of that circuit's qubits in the batch's dtype, for all i in one
+ * persistent launch (one workgroup per state).  What QEngineT::execute runs
+ * before its sampling tail (qengine.hpp:1341-1405), for many circuits at
+ * once; the request server's path for small requests. */
+int qs_batch_run_states(qs_batch* b, uint64_t nstates, const uint32_t* circuit_of,
+                        qs_state* const* out);
 int qs_batch_destroy(qs_batch* b);
 
 /* ------------------------------------------------------------------------ */
@@ -490,8 +498,11 @@ int qs_batch_destroy(qs_batch* b);
  * request whose device footprint (qs_server_footprint) fits what is left of
  * `hbm_budget` bytes (0 = 90% of
  * the free device memory), so requests run concurrently and new ones are
- * admitted as lanes and memory free up.  Results are identical to running
- * the request alone. */
+ * admitted as lanes and memory free up.  Small requests (12-14 qubits,
+ * gates then one measurement of every qubit) run batched: a lane takes
+ * every queued one of the same dtype that fits and runs their gate steps
+ * in one launch (qs_batch_run_states), then each request's sampling tail.
+ * A request's result does not depend on the mix it ran in. */
 typedef struct qs_server qs_server;
 int qs_server_create(uint32_t lanes, uint64_t hbm_budget, qs_server** out);
 /* Non-blocking; the circuit is copied.  QS_ERR_OUT_OF_MEMORY if the request
@@ -512,6 +523,9 @@ int qs_server_wait(qs_server* s, uint64_t ticket, double* times_ms);
  * to read n_outcomes first. */
 int qs_server_stats(qs_server* s, uint64_t ticket, uint64_t* n_outcomes,
                     uint64_t* dits, uint64_t* counts, uint64_t* last_dits);
+/* How a finished request ran: the number of requests whose gate steps ran
+ * in one launch with it (1: alone or not batchable). */
+int qs_server_batch_size(qs_server* s, uint64_t ticket, uint32_t* batch_size);
 int qs_server_release(qs_server* s, uint64_t ticket);
 /* info[6]: completed, queued, running, peak running, bytes held, peak bytes */
 int qs_server_info(qs_server* s, uint64_t* info);

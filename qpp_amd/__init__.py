@@ -1151,6 +1151,7 @@ class Batch:
         L.qs_batch_destroy.argtypes = [c_void_p]
         arr = (POINTER(_Circuit) * len(self._cs))(*[ctypes.pointer(c) for c in self._cs])
         o = _FuseOpts(reg_bits, low_bits, max_ops, 0)
+        self._dtype = dtype
         self._h = c_void_p()
         _check(L.qs_batch_create(arr, len(self._cs), dtype, ctypes.byref(o), ctypes.byref(self._h)))
 
@@ -1167,6 +1168,19 @@ class Batch:
                                   idx.ctypes.data_as(POINTER(c_uint64)),
                                   pr.ctypes.data_as(POINTER(c_double))))
         return idx[: len(c)], pr[: len(c)]
+
+    def run_states(self, circuit_of, dtype=None):
+        """The gate steps of circuits[circuit_of[i]] applied to |0..0>, all in
+        one launch (qs_batch_run_states): one State per entry, in the batch's
+        dtype; no measurement is taken."""
+        L = lib()
+        L.qs_batch_run_states.argtypes = [c_void_p, c_uint64, POINTER(c_uint32), POINTER(c_void_p)]
+        c = np.ascontiguousarray(circuit_of, dtype=np.uint32)
+        dt = self._dtype if dtype is None else dtype
+        states = [State([2] * self.circuits[int(k)].nq, dtype=dt) for k in c]
+        hs = (c_void_p * max(1, len(states)))(*[st.handle for st in states])
+        _check(L.qs_batch_run_states(self._h, len(c), c.ctypes.data_as(POINTER(c_uint32)), hs))
+        return states
 
     def info(self, circuit=0):
         v = [c_uint64() for _ in range(3)]
@@ -1213,10 +1227,13 @@ class Batch:
 class Server:
     """Request server (qs_server_*): each request is QEngine(qc, seed)
     .execute(shots) in complex128 or complex64, admitted onto the GPU by
-    device footprint as lanes and memory free up; `lanes` requests run
-    concurrently on their own HIP streams."""
+    device footprint as lanes and memory free up; `lanes` lanes run
+    concurrently on their own HIP streams (4 by default: the device's
+    hardware queues).  Small requests (12-14 qubits, gates then one
+    measurement of every qubit) queued together run their gate steps in one
+    launch."""
 
-    def __init__(self, lanes=8, hbm_budget=0):
+    def __init__(self, lanes=4, hbm_budget=0):
         L = lib()
         L.qs_server_create.argtypes = [c_uint32, c_uint64, POINTER(c_void_p)]
         L.qs_server_submit.argtypes = [c_void_p, POINTER(_Circuit), c_int, c_uint64, c_uint64,
@@ -1268,6 +1285,15 @@ class Server:
                                      last.ctypes.data_as(POINTER(c_uint64))))
         st = {tuple(int(x) for x in d[i * nc:(i + 1) * nc]): int(c[i]) for i in range(n.value)}
         return st, [int(x) for x in last[:nc]]
+
+    def batch_size(self, ticket):
+        """Requests whose gate steps ran in one launch with this finished
+        request (1: alone, or not batchable)."""
+        L = lib()
+        L.qs_server_batch_size.argtypes = [c_void_p, c_uint64, POINTER(c_uint32)]
+        v = c_uint32()
+        _check(L.qs_server_batch_size(self._h, int(ticket), byref(v)))
+        return v.value
 
     def release(self, ticket):
         _check(lib().qs_server_release(self._h, int(ticket)))

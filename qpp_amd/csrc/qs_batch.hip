@@ -102,7 +102,8 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                                                    const Ovr* __restrict__ ovr,
                                                    const uint32_t* __restrict__ order,
                                                    uint32_t ntraj, TrajOut* __restrict__ out,
-                                                   uint32_t* __restrict__ next, int force_exact) {
+                                                   uint32_t* __restrict__ next, int force_exact,
+                                                   cxt<R>* const* __restrict__ dst) {
     extern __shared__ unsigned char smem_raw[];
     cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
     __shared__ double pre[kThreads + 1];
@@ -111,7 +112,6 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
     __shared__ int s_certain;
     __shared__ qs_exact::ChunkLds<kThreads> exl;
     const uint32_t t = threadIdx.x;
-    cxt<R>* __restrict__ a = ws + uint64_t(blockIdx.x) * slot;
     // work queue: trajectories in `order` (largest first), claimed one at a
     // time; every workgroup leaves once the queue is exhausted
     for (;;) {
@@ -120,6 +120,9 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         const uint32_t k = s_k;
         if (k >= ntraj) break;
         const uint32_t tr = order[k];
+        // the trajectory's state: the workgroup's slot, or (dst set) the
+        // caller's state of that trajectory, which keeps the final state
+        cxt<R>* __restrict__ a = dst ? dst[tr] : ws + uint64_t(blockIdx.x) * slot;
         const TrajIn ti = tin[tr];
         const BClass c = cls[ti.cls];
         uint32_t ov = ti.ovr_begin;
@@ -211,6 +214,10 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
             }
             ov = oc;
             __syncthreads();  // the pass's stores are visible to the next pass
+        }
+        if (dst) {
+            __syncthreads();  // every thread has read s_k before it is reused
+            continue;
         }
 
         // ---- measurement of every qubit: index of the outcome -------------
@@ -511,10 +518,10 @@ template <typename T>
 int dev_put(void*& dptr, size_t& cap, const std::vector<T>& v, hipStream_t s) {
     const size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
     if (bytes > cap) {
-        if (dptr) (void)hipFree(dptr);
+        if (dptr) (void)hipFreeAsync(dptr, s);
         dptr = nullptr;
         cap = 0;
-        QS_HIP(hipMalloc(&dptr, bytes), "qpp_amd::batch");
+        QS_HIP(hipMallocAsync(&dptr, bytes, s), "qpp_amd::batch");
         cap = bytes;
     }
     if (!v.empty())
@@ -525,8 +532,12 @@ int dev_put(void*& dptr, size_t& cap, const std::vector<T>& v, hipStream_t s) {
 
 }  // namespace
 
+// Device memory is stream-ordered on the stream of the first run (the
+// calling thread's stream: a serving lane's own), so creating and
+// destroying batches never synchronises other streams.
 struct qs_batch {
     qs_dtype dtype = QS_C64;
+    hipStream_t stream = nullptr;
     uint32_t reg_bits = 4;
     std::vector<Cls> cls;
     // device tables (all classes concatenated)
@@ -548,6 +559,8 @@ struct qs_batch {
     uint32_t* d_next = nullptr;  // work-queue counter
     void* d_ws = nullptr;
     size_t c_ws = 0;
+    void* d_dst = nullptr;  // per-trajectory destination states (run_states)
+    size_t c_dst = 0;
     // class -> offsets of its ops in the concatenated table
     std::vector<int> op_base;
     // last run
@@ -558,7 +571,7 @@ struct qs_batch {
 
 namespace {
 
-int upload_tables(qs_batch* b) {
+int upload_tables(qs_batch* b, hipStream_t s) {
     std::vector<BClass> bc;
     std::vector<FPass> ps;
     std::vector<FPhase> ph;
@@ -608,8 +621,9 @@ int upload_tables(qs_batch* b) {
             std::memcpy(blob.data() + o5 + i * 8, &v, 8);
         }
     }
-    QS_HIP(hipMalloc(&b->d_tab, tot), "qpp_amd::batch");
-    QS_HIP(hipMemcpy(b->d_tab, blob.data(), tot, hipMemcpyHostToDevice), "qpp_amd::batch");
+    QS_HIP(hipMallocAsync(&b->d_tab, tot, s), "qpp_amd::batch");
+    QS_HIP(hipMemcpyAsync(b->d_tab, blob.data(), tot, hipMemcpyHostToDevice, s), "qpp_amd::batch");
+    QS_HIP(hipStreamSynchronize(s), "qpp_amd::batch");  // blob is freed on return
     auto* base = static_cast<unsigned char*>(b->d_tab);
     b->d_cls = reinterpret_cast<BClass*>(base + o1);
     b->d_passes = reinterpret_cast<FPass*>(base + o2);
@@ -735,10 +749,9 @@ int qs_batch_create(const qs_circuit* const* circuits, uint32_t ncircuits, qs_dt
 
 int qs_batch_destroy(qs_batch* b) {
     if (!b) return QS_OK;
-    if (b->d_tab || b->d_ws) (void)hipStreamSynchronize(qs::default_stream());
-    for (void* p : {b->d_tab, b->d_tin, b->d_ovr, b->d_order, b->d_out, b->d_ws,
+    for (void* p : {b->d_tab, b->d_tin, b->d_ovr, b->d_order, b->d_out, b->d_ws, b->d_dst,
                     static_cast<void*>(b->d_next)})
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, b->stream);
     delete b;
     return QS_OK;
 }
@@ -774,20 +787,21 @@ int qs_batch_plan(const qs_batch* b, uint32_t circuit, uint64_t seed, double noi
     return QS_OK;
 }
 
-int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint64_t* seeds,
-                 double noise_p, uint64_t* out_index, double* out_prob) {
+}  // extern "C"
+
+namespace {
+
+// One launch of k_traj over ntraj trajectories (arguments validated by the
+// caller).  dst: nullptr (each trajectory ends in its measurement), or one
+// state per trajectory that receives its final state instead.
+int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint64_t* seeds,
+             double noise_p, const std::vector<void*>* dst, uint64_t* out_index, double* out_prob) {
     constexpr const char* W = "qpp_amd::batch";
-    if (!b) return fail(QS_ERR_INVALID_HANDLE, W);
-    if (ntraj == 0) return QS_OK;
-    if (!circuit_of || !seeds) return fail(QS_ERR_INVALID_HANDLE, W, "inputs");
-    if (ntraj > 0xFFFFFFFFull) return fail(QS_ERR_OUT_OF_RANGE, W, "ntraj");
-    if (!(noise_p <= 1.0)) return fail(QS_ERR_OUT_OF_RANGE, W, "noise_p");
-    for (uint64_t i = 0; i < ntraj; ++i)
-        if (circuit_of[i] >= b->cls.size()) return fail(QS_ERR_OUT_OF_RANGE, W, "circuit_of");
     if (int rc = qs::ensure_device()) return rc;
-    hipStream_t s = qs::default_stream();
+    if (!b->stream) b->stream = qs::current_stream();
+    hipStream_t s = b->stream;
     if (!b->d_tab)
-        if (int rc = upload_tables(b)) return rc;
+        if (int rc = upload_tables(b, s)) return rc;
 
     // ---- host: per-trajectory frame plans, in parallel --------------------
     const auto h0 = std::chrono::steady_clock::now();
@@ -850,29 +864,28 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
     QS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), W);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(ntraj, uint64_t(std::max(1, per_cu)) * ncu));
     const uint64_t slot = uint64_t{1} << maxnb;
-    const size_t ws_bytes = size_t(grid) * slot * es;
+    const size_t ws_bytes = dst ? 0 : size_t(grid) * slot * es;
     if (ws_bytes > b->c_ws) {
-        if (b->d_ws) {
-            QS_HIP(hipStreamSynchronize(s), W);
-            (void)hipFree(b->d_ws);
-        }
+        if (b->d_ws) (void)hipFreeAsync(b->d_ws, s);
         b->d_ws = nullptr;
         b->c_ws = 0;
-        QS_HIP(hipMalloc(&b->d_ws, ws_bytes), W);
+        QS_HIP(hipMallocAsync(&b->d_ws, ws_bytes, s), W);
         b->c_ws = ws_bytes;
     }
+    if (dst)
+        if (int rc = dev_put(b->d_dst, b->c_dst, *dst, s)) return rc;
     if (int rc = dev_put(b->d_tin, b->c_tin, tin, s)) return rc;
     if (int rc = dev_put(b->d_ovr, b->c_ovr, ovr, s)) return rc;
     if (int rc = dev_put(b->d_order, b->c_order, order, s)) return rc;
     const size_t ob = size_t(ntraj) * sizeof(TrajOut);
     if (ob > b->c_out) {
-        if (b->d_out) (void)hipFree(b->d_out);
+        if (b->d_out) (void)hipFreeAsync(b->d_out, s);
         b->d_out = nullptr;
         b->c_out = 0;
-        QS_HIP(hipMalloc(&b->d_out, ob), W);
+        QS_HIP(hipMallocAsync(&b->d_out, ob, s), W);
         b->c_out = ob;
     }
-    if (!b->d_next) QS_HIP(hipMalloc(&b->d_next, 256), W);
+    if (!b->d_next) QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&b->d_next), 256, s), W);
     hipEvent_t e0, e1;
     QS_HIP(hipEventCreate(&e0), W);
     QS_HIP(hipEventCreate(&e1), W);
@@ -889,20 +902,24 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<double, 8>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
+                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
+                               static_cast<double2* const*>(dst ? b->d_dst : nullptr));
         else
             hipLaunchKernelGGL((k_traj<double, 16>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
+                               static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
+                               static_cast<double2* const*>(dst ? b->d_dst : nullptr));
     } else {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<float, 8>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
+                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
+                               static_cast<float2* const*>(dst ? b->d_dst : nullptr));
         else
             hipLaunchKernelGGL((k_traj<float, 16>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
-                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force);
+                               static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
+                               static_cast<float2* const*>(dst ? b->d_dst : nullptr));
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, s);
@@ -924,6 +941,54 @@ int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const 
     }
     b->last_exact = nex;
     return QS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qs_batch_run(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint64_t* seeds,
+                 double noise_p, uint64_t* out_index, double* out_prob) {
+    constexpr const char* W = "qpp_amd::batch";
+    if (!b) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (ntraj == 0) return QS_OK;
+    if (!circuit_of || !seeds) return fail(QS_ERR_INVALID_HANDLE, W, "inputs");
+    if (ntraj > 0xFFFFFFFFull) return fail(QS_ERR_OUT_OF_RANGE, W, "ntraj");
+    if (!(noise_p <= 1.0)) return fail(QS_ERR_OUT_OF_RANGE, W, "noise_p");
+    for (uint64_t i = 0; i < ntraj; ++i)
+        if (circuit_of[i] >= b->cls.size()) return fail(QS_ERR_OUT_OF_RANGE, W, "circuit_of");
+    return run_impl(b, ntraj, circuit_of, seeds, noise_p, nullptr, out_index, out_prob);
+}
+
+// The gate steps of circuit circuit_of[i] applied to |0..0> into out[i] (a
+// ket of that circuit's qubits in the batch's dtype), all in one launch;
+// the measurement is not taken.
+int qs_batch_run_states(qs_batch* b, uint64_t nstates, const uint32_t* circuit_of,
+                        qs_state* const* out) {
+    constexpr const char* W = "qpp_amd::batch";
+    if (!b) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (nstates == 0) return QS_OK;
+    if (!circuit_of || !out) return fail(QS_ERR_INVALID_HANDLE, W, "inputs");
+    if (nstates > 0xFFFFFFFFull) return fail(QS_ERR_OUT_OF_RANGE, W, "nstates");
+    std::vector<void*> dst(nstates);
+    for (uint64_t i = 0; i < nstates; ++i) {
+        if (circuit_of[i] >= b->cls.size()) return fail(QS_ERR_OUT_OF_RANGE, W, "circuit_of");
+        const qs_state* st = out[i];
+        if (!st || !st->d) return fail(QS_ERR_INVALID_HANDLE, W, "out");
+        if (st->dtype != b->dtype) return fail(QS_ERR_TYPE_MISMATCH, W, "out");
+        if (st->kind != QS_KET || st->nelem != (uint64_t{1} << b->cls[circuit_of[i]].nb))
+            return fail(QS_ERR_DIMS_MISMATCH_CVECTOR, W, "out");
+        for (uint64_t j = 0; j < i; ++j)
+            if (out[j]->d == st->d) return fail(QS_ERR_OUT_OF_RANGE, W, "out (the same state twice)");
+        dst[i] = st->d;
+    }
+    // the states' own pending work (their initialisation) comes first
+    if (int rc = qs::ensure_device()) return rc;
+    if (!b->stream) b->stream = qs::current_stream();
+    for (uint64_t i = 0; i < nstates; ++i)
+        if (out[i]->stream != b->stream) QS_HIP(hipStreamSynchronize(out[i]->stream), W);
+    std::vector<uint64_t> seeds(nstates, 0);
+    return run_impl(b, nstates, circuit_of, seeds.data(), -1.0, &dst, nullptr, nullptr);
 }
 
 }  // extern "C"

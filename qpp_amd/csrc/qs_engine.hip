@@ -804,6 +804,10 @@ struct qs_engine {
         std::iota(subsys.begin(), subsys.end(), uint64_t{0});
     }
 
+    // set by qs::engine_execute_prefixed: the state after the gate prefix,
+    // which execute() takes instead of running the prefix
+    qs_state* prefixed = nullptr;
+
     int execute(uint64_t reps) {
         if (noise_p >= 0) return execute_with_retries(steps, 0, reps);
         // reset(qstate, false): keep the state and the statistics
@@ -829,12 +833,19 @@ struct qs_engine {
                 else
                     pre.push_back(app_of(s[i]));
             }
-        if (int rc = run_gates(pre, true)) return rc;
+        if (prefixed) {
+            qs_state_destroy(st);
+            st = prefixed;
+            prefixed = nullptr;
+        } else if (int rc = run_gates(pre, true)) {
+            return rc;
+        }
         if (reps > 1 && can_sample) return execute_sampling(s, pos, reps);
         return execute_with_retries(s, pos, reps);
     }
 
     ~qs_engine() {
+        if (prefixed) qs_state_destroy(prefixed);
         for (auto& kv : progs) qs_program_destroy(kv.second);
         if (st) qs_state_destroy(st);
     }
@@ -1097,3 +1108,36 @@ int qs_engine_post_select_ok(const qs_engine* e, int* ok) {
 }
 
 }  // extern "C"
+
+int qs::engine_execute_prefixed(qs_engine* e, qs_state* prefixed, uint64_t reps) {
+    constexpr const char* W = "qpp::QEngineT::execute()";
+    if (!e || !prefixed || !e->st) {
+        qs_state_destroy(prefixed);
+        return fail(QS_ERR_INVALID_HANDLE, W);
+    }
+    if (e->noise_p >= 0 || prefixed->dtype != e->dtype || prefixed->kind != e->kind ||
+        prefixed->nelem != e->st->nelem) {
+        qs_state_destroy(prefixed);
+        return fail(QS_ERR_DIMS_NOT_EQUAL, W, "state");
+    }
+    bool meas = false, ok = reps > 0;
+    for (const EStep& s : e->steps) {
+        const bool gate = s.type == QS_STEP_GATE || s.type == QS_STEP_FAN ||
+                          s.type == QS_STEP_CTRL || s.type == QS_STEP_CTRL_FAN ||
+                          s.type == QS_STEP_NOP;
+        if (is_meas(s) && s.type != QS_STEP_POST_SELECT) meas = true;
+        else if (!gate || meas) ok = false;
+    }
+    if (!ok) {
+        qs_state_destroy(prefixed);
+        return fail(QS_ERR_NOT_IMPLEMENTED, W, "a prefixed run needs gates, then measurements");
+    }
+    qs::clear_error();
+    e->prefixed = prefixed;
+    const int rc = e->execute(reps);
+    if (e->prefixed) {  // execute() returned before taking it
+        qs_state_destroy(e->prefixed);
+        e->prefixed = nullptr;
+    }
+    return rc;
+}

@@ -115,6 +115,13 @@ struct QuditGate {
 int launch_qudit_gate(qs_state* st, const QuditGate& g, const double* mats,
                       uint32_t nmats, uint64_t DA, bool conj);
 
+// QEngine::execute(reps) on an engine whose gate prefix (the steps before
+// the first measurement) has already been applied elsewhere: `prefixed`
+// holds that state and is taken over by the engine (destroyed on error
+// too).  Valid for circuits whose steps are gates followed by measurements
+// only (the prefix is then every gate step, in order, for any reps).
+int engine_execute_prefixed(qs_engine* e, qs_state* prefixed, uint64_t reps);
+
 }  // namespace qs
 
 #endif  // QPP_AMD_QS_INTERNAL_HPP_

@@ -12,6 +12,21 @@
 // requests is on the device at once.  All device memory of a request is
 // stream-ordered (hipMallocAsync from a caching pool), so admitting or
 // retiring a request never synchronises the device.
+//
+// Small requests run batched: a request of kBatchMinQ..kBatchMaxQ qubits
+// whose circuit is gates (dense 1-qubit or Clifford on <= 2 qubits)
+// followed by one measurement of every qubit is "batchable".  A lane that
+// takes one also takes every other queued batchable request of the same
+// dtype that fits (up to kMaxBatch), and their gate steps run in ONE
+// persistent launch (qs_batch_run_states: the k_traj work queue, one
+// workgroup per request, each writing its request's final state).  Each
+// request's QEngine::execute(shots) then continues from that state (the
+// sampling tail, qs::engine_execute_prefixed).  A batchable request always
+// takes this path, alone or in a mix, so its result does not depend on the
+// mix: every request's computation is its own workgroup's, in the same
+// order of operations.  Above kBatchMaxQ a single workgroup streaming a
+// whole state is slower than the per-pass launches that spread it over the
+// chip, so larger requests run alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +37,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -77,7 +93,28 @@ struct Request {
     std::vector<uint64_t> stat_dits, stat_counts;  // outcomes (n x nc) and counts
     std::vector<uint64_t> dits;
     uint32_t lane = 0;
+    bool batchable = false;
+    uint32_t batch_size = 0;  // requests in the launch that ran it (1: alone)
 };
+
+constexpr uint32_t kBatchMinQ = 12;  // one 2^12-amplitude tile
+constexpr uint32_t kBatchMaxQ = 14;
+constexpr size_t kMaxBatch = 64;
+
+// What qs_batch accepts, checked on the steps: qubits, gate steps on at most
+// two qubits (controls included), then one measurement of every qubit.
+bool batchable(const qs_circuit* c) {
+    if (c->d != 2 || c->nq < kBatchMinQ || c->nq > kBatchMaxQ || c->nsteps == 0) return false;
+    for (uint32_t i = 0; i + 1 < c->nsteps; ++i) {
+        const qs_step& st = c->steps[i];
+        if (st.type == QS_STEP_NOP) continue;
+        if (st.type != QS_STEP_GATE && st.type != QS_STEP_CTRL) return false;
+        if (st.nctrl + st.ntarget > 2 || st.ntarget == 0) return false;
+        if (st.shift_off != UINT32_MAX) return false;
+    }
+    const qs_step& m = c->steps[c->nsteps - 1];
+    return m.type == QS_STEP_MEASURE_MANY && m.ntarget == c->nq;
+}
 
 // Device bytes a request can hold at once, over every path QEngine's
 // execute(shots) takes (qengine.hpp:316-481), with D the dimension and s the
@@ -118,6 +155,7 @@ struct qs_server {
 
     void work(uint32_t lane) {
         qs::set_thread_stream(streams[lane]);
+        std::vector<Request*> group;
         for (;;) {
             Request* r = nullptr;
             {
@@ -137,21 +175,42 @@ struct qs_server {
                 if (it == queue.end()) continue;
                 r = *it;
                 queue.erase(it);
+                group.assign(1, r);
                 in_use += r->footprint;
+                if (r->batchable)  // the other batchable requests that fit
+                    for (auto jt = queue.begin(); jt != queue.end() && group.size() < kMaxBatch;) {
+                        Request* q = *jt;
+                        if (q->batchable && q->dtype == r->dtype && in_use + q->footprint <= budget) {
+                            in_use += q->footprint;
+                            group.push_back(q);
+                            jt = queue.erase(jt);
+                        } else {
+                            ++jt;
+                        }
+                    }
                 peak_in_use = std::max(peak_in_use, in_use);
-                ++running;
+                running += group.size();
                 peak_running = std::max(peak_running, running);
             }
-            r->lane = lane;
-            r->t_start = Clock::now();
-            run(r);
-            r->t_done = Clock::now();
+            const auto t_start = Clock::now();
+            for (Request* q : group) {
+                q->lane = lane;
+                q->t_start = t_start;
+            }
+            if (r->batchable)
+                run_batch(group);
+            else
+                run(r);
+            const auto t_done = Clock::now();
             {
                 std::lock_guard<std::mutex> lk(m);
-                in_use -= r->footprint;
-                --running;
-                r->done = true;
-                ++completed;
+                for (Request* q : group) {
+                    q->t_done = t_done;
+                    in_use -= q->footprint;
+                    --running;
+                    q->done = true;
+                    ++completed;
+                }
             }
             cv_done.notify_all();
             cv_work.notify_all();
@@ -162,6 +221,51 @@ struct qs_server {
         qs_engine* e = nullptr;
         int rc = qs_engine_create(&r->circ.c, QS_KET, r->dtype, -1.0, r->seed, &e);
         if (!rc) rc = qs_engine_execute(e, r->shots);
+        r->batch_size = 1;
+        finish(r, e, rc);
+    }
+
+    // The gate steps of every request of `g` in one launch, then each one's
+    // sampling tail from its final state.  If the batch cannot be built, the
+    // requests run one by one.
+    void run_batch(const std::vector<Request*>& g) {
+        std::vector<const qs_circuit*> circs;
+        for (Request* q : g) circs.push_back(&q->circ.c);
+        qs_batch* b = nullptr;
+        std::vector<qs_state*> sts(g.size(), nullptr);
+        int rc = qs_batch_create(circs.data(), uint32_t(circs.size()), g[0]->dtype, nullptr, &b);
+        for (size_t i = 0; i < g.size() && !rc; ++i) {
+            const std::vector<uint64_t> dims(g[i]->circ.c.nq, 2);
+            rc = qs_state_create(g[i]->dtype, QS_KET, uint32_t(dims.size()), dims.data(), &sts[i]);
+        }
+        if (!rc) {
+            std::vector<uint32_t> circuit_of(g.size());
+            std::iota(circuit_of.begin(), circuit_of.end(), 0u);
+            rc = qs_batch_run_states(b, g.size(), circuit_of.data(), sts.data());
+        }
+        qs_batch_destroy(b);
+        if (rc) {  // not batchable after all: each request alone
+            for (qs_state* st : sts) qs_state_destroy(st);
+            for (Request* q : g) run(q);
+            return;
+        }
+        for (size_t i = 0; i < g.size(); ++i) {
+            Request* q = g[i];
+            qs_engine* e = nullptr;
+            int rq = qs_engine_create(&q->circ.c, QS_KET, q->dtype, -1.0, q->seed, &e);
+            if (!rq) {
+                rq = qs::engine_execute_prefixed(e, sts[i], q->shots);
+            } else {
+                qs_state_destroy(sts[i]);
+            }
+            sts[i] = nullptr;
+            q->batch_size = uint32_t(g.size());
+            finish(q, e, rq);
+        }
+    }
+
+    // the request's results from its engine (rc: the run's status)
+    void finish(Request* r, qs_engine* e, int rc) {
         if (!rc) {
             uint64_t n = 0;
             rc = qs_engine_num_outcomes(e, &n);
@@ -234,6 +338,7 @@ int qs_server_submit(qs_server* s, const qs_circuit* qc, qs_dtype dtype, uint64_
     r->shots = shots;
     r->seed = seed;
     r->footprint = footprint_of(qc, dtype, shots);
+    r->batchable = batchable(qc);
     if (r->footprint > s->budget) return fail(QS_ERR_OUT_OF_MEMORY, W, "request exceeds the HBM budget");
     r->t_submit = Clock::now();
     {
@@ -281,6 +386,18 @@ int qs_server_stats(qs_server* s, uint64_t ticket, uint64_t* n_outcomes, uint64_
     if (dits) std::copy(r->stat_dits.begin(), r->stat_dits.end(), dits);
     if (counts) std::copy(r->stat_counts.begin(), r->stat_counts.end(), counts);
     if (last_dits) std::copy(r->dits.begin(), r->dits.end(), last_dits);
+    return QS_OK;
+}
+
+// How a finished request ran: *batch_size = the number of requests whose
+// gate steps ran in the same launch as its own (1: alone, or not batchable).
+int qs_server_batch_size(qs_server* s, uint64_t ticket, uint32_t* batch_size) {
+    constexpr const char* W = "qpp_amd::server";
+    if (!s || !batch_size) return fail(QS_ERR_INVALID_HANDLE, W);
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end() || !it->second->done) return fail(QS_ERR_OUT_OF_RANGE, W, "ticket");
+    *batch_size = it->second->batch_size;
     return QS_OK;
 }
 

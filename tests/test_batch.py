@@ -15,10 +15,10 @@ import qpp_amd as qa
 PAULI = [I2, X, Y, Z]
 
 
-def noisy_circuit(n, depth, seed, cz=False, extra=False):
+def noisy_circuit(n, depth, seed, cz=False, extra=False, measure=True):
     """Haar 1-qubit gates on every qubit, then a CNOT ladder (i, i+1)
     (optionally CZ / X / Z / H Clifford gates too), ending in one
-    measurement of every qubit."""
+    measurement of every qubit (unless measure is False)."""
     rng = np.random.default_rng(seed)
     qc = qa.QCircuit(n, n)
     for _ in range(depth):
@@ -30,7 +30,8 @@ def noisy_circuit(n, depth, seed, cz=False, extra=False):
             qc.gate(X, 0).gate(Z, n - 1).gate(H, n // 2)
             qc.CTRL(X, n - 1, 0, shift=[1])
             qc.gate(CZ, 1, 3)
-    qc.measure(list(range(n)), 0)
+    if measure:
+        qc.measure(list(range(n)), 0)
     return qc
 
 
@@ -202,3 +203,50 @@ def test_batch_exact_replay_path_vs_oracle(orc, monkeypatch):
     monkeypatch.delenv("QPP_AMD_FORCE_EXACT")
     i2, _ = qa.Batch(circs, dtype=qa.C128).run(cof, seeds, 0.01)
     assert np.array_equal(i2, idx)
+
+
+@gpu
+@pytest.mark.parametrize("dt", [qa.C128, qa.C64])
+def test_run_states_vs_oracle(orc, dt):
+    """qs_batch_run_states: every entry's final state (the gate steps only,
+    no measurement) equals the oracle engine's state for the same circuit
+    without its measurement; entries of one circuit are bit-identical."""
+    specs = [(12, 3, 71, dict(cz=True)), (13, 2, 72, dict(extra=True)), (14, 2, 73, {})]
+    circs = [noisy_circuit(n, d, s, **kw) for n, d, s, kw in specs]
+    bare = [noisy_circuit(n, d, s, measure=False, **kw) for n, d, s, kw in specs]
+    b = qa.Batch(circs, dtype=dt)
+    cof = [0, 1, 2, 1, 0]
+    states = b.run_states(cof)
+    tol = 1e-12 if dt == qa.C128 else 2e-6
+    for k, st in zip(cof, states):
+        print(f"run_states: circuit {k} ({circs[k].nq} qubits), dtype {dt}", flush=True)
+        _, _, _, ref = orc.engine_run(bare[k], reps=1)
+        got = st.download()
+        assert np.max(np.abs(got - ref)) <= tol
+    assert np.array_equal(states[1].download(), states[3].download())
+    assert np.array_equal(states[0].download(), states[4].download())
+
+
+@gpu
+def test_run_states_errors():
+    circs = [noisy_circuit(12, 1, 81)]
+    b = qa.Batch(circs, dtype=qa.C128)
+    L = qa.lib()
+    import ctypes
+    L.qs_batch_run_states.argtypes = [ctypes.c_void_p, ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_void_p)]
+    cof = np.zeros(1, dtype=np.uint32)
+
+    def run(states, c=cof):
+        hs = (ctypes.c_void_p * len(states))(*[st.handle.value for st in states])
+        return L.qs_batch_run_states(b._h, len(states), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), hs)
+
+    with pytest.raises(qa.TypeMismatch):
+        qa._check(run([qa.State([2] * 12, dtype=qa.C64)]))
+    with pytest.raises(qa.DimsMismatchCvector):
+        qa._check(run([qa.State([2] * 11)]))
+    st = qa.State([2] * 12)
+    with pytest.raises(qa.OutOfRange):
+        qa._check(run([st, st], np.zeros(2, dtype=np.uint32)))
+    with pytest.raises(qa.OutOfRange):
+        qa._check(run([qa.State([2] * 12)], np.ones(1, dtype=np.uint32)))
