@@ -538,6 +538,55 @@ def bench_rho(args, qa):
     }
 
 
+def bench_block(args, qa, n=30, reps=3):
+    """Dense 4- and 5-qubit gates (16x16 and 32x32 unitaries, the widest
+    blocks qpp::apply takes on the matrix cores here) on an n-qubit
+    complex64 ket, every amplitude read and written once per launch: the
+    FP32 MFMA kernel (k_gatek_mfma) against the VALU kernel (k_gatek, forced
+    with QPP_AMD_GATEK_VALU=1), plus the complex128 VALU kernel.  HBM
+    roofline: 2 s 2^n bytes per launch; MFMA work: 4 DIM real FMA per
+    amplitude (the complex product as a real GEMM)."""
+    import os
+    from qpp_amd.workloads import haar
+    rng = np.random.default_rng(args.seed)
+    out = []
+    for k in (4, 5):
+        U = haar(2 ** k, rng)
+        tg = [0, 7, 13, 21, 29][:k]  # low and high target bits
+        row = {"qubits_in_gate": k, "targets": tg}
+        for label, dt, valu in (("c64_mfma", qa.C64, False), ("c64_valu", qa.C64, True),
+                                ("c128_valu", qa.C128, True)):
+            st = qa.State([2] * n, dtype=dt)
+            if valu:
+                os.environ["QPP_AMD_GATEK_VALU"] = "1"
+            try:
+                st.apply(U, tg)  # warm-up
+                qa.synchronize()
+                ms = profile_region(qa, lambda: [st.apply(U, tg) for _ in range(reps)]) / reps
+            finally:
+                os.environ.pop("QPP_AMD_GATEK_VALU", None)
+            es = 8 if dt == qa.C64 else 16
+            gbs = 2 * es * 2 ** n / (ms / 1e3) / 1e9
+            fl = 2 * (4 * 2 ** k if label == "c64_mfma" else 8 * 2 ** k) * 2 ** n
+            row[label] = {"ms": round(ms, 3), "GBps": round(gbs, 1),
+                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "TFLOPs": round(fl / (ms / 1e3) / 1e12, 2)}
+            del st
+        row["c64_mfma"]["mfma_frac"] = round(row["c64_mfma"]["TFLOPs"] / 157.3, 4)
+        row["mfma_speedup_vs_valu"] = round(row["c64_valu"]["ms"] / row["c64_mfma"]["ms"], 3)
+        log(f"block k={k}: {row}")
+        out.append(row)
+    return {
+        "metric": "dense k-qubit gate on a 30-qubit complex64 ket: MFMA vs VALU",
+        "workload": f"{n}-qubit ket, one dense 2^k x 2^k unitary per launch (k = 4, 5), "
+                    f"{reps} launches timed with HIP events (matrix upload included)",
+        "flops_definition": "MFMA: 2 x 4 x 2^k real flops per amplitude (complex product as a "
+                            "real GEMM); VALU: 2 x 8 x 2^k (complex FMA); peaks 157.3 TFLOP/s "
+                            "FP32 matrix, 78.6 TFLOP/s FP32 / FP64 vector",
+        "blocks": out,
+    }
+
+
 def bench_copy(args, qa):
     """Reference streaming kernel: a 30-qubit complex128 state (16 GiB) copied
     into another with qs_state_copy (each byte read once, written once).
@@ -809,6 +858,11 @@ def secondary_configs(args, qa):
     r4["config_id"] = 4
     out.append(r4)
     log(f"secondary serving: {time.perf_counter() - t:.1f} s")
+    t = time.perf_counter()
+    rb = bench_block(ns(), qa, n=30)
+    rb["config_id"] = 2
+    out.append(rb)
+    log(f"secondary dense blocks on the matrix cores: {time.perf_counter() - t:.1f} s")
     return out
 
 
@@ -820,7 +874,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=30)
-    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy"], default="fused")
+    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy", "block"], default="fused")
     ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
     ap.add_argument("--cal-requests", type=int, default=40,
                     help="serve mode: closed-loop calibration requests")
@@ -880,6 +934,9 @@ def main():
         return
     if args.mode == "copy":
         print(json.dumps(bench_copy(args, qa)), flush=True)
+        return
+    if args.mode == "block":
+        print(json.dumps(bench_block(args, qa, n=args.qubits)), flush=True)
         return
     out = bench_circuit(args, qa, args.qubits, args.depth, primary=True)
     if args.mode == "fused" and args.secondary:

@@ -126,6 +126,11 @@ bool force_exact_sampling() {
     return v && v[0] == '1';
 }
 
+bool gatek_valu_forced() {
+    const char* v = std::getenv("QPP_AMD_GATEK_VALU");
+    return v && v[0] == '1';
+}
+
 StreamScope::StreamScope(hipStream_t s) : old(t_stream) { t_stream = s; }
 StreamScope::~StreamScope() { t_stream = old; }
 

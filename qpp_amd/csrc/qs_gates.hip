@@ -15,6 +15,9 @@
 //   k_gatek     k-bit dense gate (k <= 10) through LDS: a workgroup stages a
 //               batch of amplitude groups, each thread then forms outputs
 //               from the staged group and the gate matrix.
+//   k_gatek_mfma  4- and 5-bit dense gates on complex64 states with FP32
+//               MFMA: Y = U X over the state's amplitude groups as a real
+//               GEMM (2^(k+1) x 2^(k+1) times 16 groups per wave step).
 //   k_qudit     mixed-radix gate for qudit states (multi-index strides).
 //
 // Arithmetic: complex128 in double, complex64 in float; gate matrices are
@@ -23,6 +26,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "qs_internal.hpp"
@@ -227,6 +231,125 @@ __global__ __launch_bounds__(kBlock) void k_gatek(
 }
 
 // ---------------------------------------------------------------------------
+// k-bit dense gate with MFMA (complex64, k = 4, 5)
+// ---------------------------------------------------------------------------
+// The groups (columns) are as in k_gatek.  Over the real and imaginary parts
+// the gate is one real GEMM, [Yr; Yi] = W [Xr; Xi] with W = [[Ur, -Ui],
+// [Ui, Ur]] (2 DIM x 2 DIM, DIM = 2^K), and every wave step multiplies W by
+// 16 groups with v_mfma_f32_16x16x4f32 (D[16x16] += A[16x4] B[4x16]; lane l
+// holds A[l%16][l/16], B[l/16][l%16] and D[4(l/16)+v][l%16], v = 0..3):
+//   B: lane (g = l/16, j = l%16) holds real row 4s+g of group j for slice
+//      s; it loads the complex amplitudes 4q+g of its group, whose real
+//      parts are slices q and whose imaginary parts are slices DIM/4+q;
+//   A: W in 16-row blocks mb and 4-column slices s, each lane's entries
+//      packed by the host in that order and read from LDS;
+//   D: row block mb < NB/2 holds real parts and mb + NB/2 the imaginary
+//      parts of the same complex rows 16 mb + 4 g + v, so each lane writes
+//      whole complex amplitudes.
+// Arithmetic per amplitude: 4 DIM FP32 FMA on the matrix cores (twice the
+// FMA count of the complex product on VALU, at twice the VALU rate).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kMfmaGrid = 2048;
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_gatek_mfma(float2* __restrict__ a, uint64_t ncols,
+                                                       const uint64_t* __restrict__ inner,
+                                                       const float* __restrict__ wpack, Fixed fx,
+                                                       uint64_t cval) {
+    constexpr int DIM = 1 << K;
+    constexpr int NX = DIM / 4;       // complex inputs per lane
+    constexpr int NS = 2 * NX;        // slices of 4 real columns of W
+    constexpr int NB = 2 * DIM / 16;  // blocks of 16 real rows of W
+    constexpr int NO = NB / 2;        // output row blocks (complex)
+    const int lane = threadIdx.x & 63;
+    const int j = lane & 15, g = lane >> 4;
+    // the packed matrix in LDS, shared by the workgroup's waves (registers
+    // go to the amplitudes in flight instead)
+    __shared__ float wl[NB * NS * 64];
+    for (int i = threadIdx.x; i < NB * NS * 64; i += kBlock) wl[i] = wpack[i];
+    __syncthreads();
+    uint64_t in_off[NX], out_off[NO][4];
+#pragma unroll
+    for (int q = 0; q < NX; ++q) in_off[q] = inner[4 * q + g];
+#pragma unroll
+    for (int mh = 0; mh < NO; ++mh)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out_off[mh][v] = inner[16 * mh + 4 * g + v];
+    constexpr int kWaves = kBlock / 64;
+    const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
+    // a wave's group `grp` covers columns 16 grp .. 16 grp + 15; the next
+    // group's amplitudes are loaded before the current group's products, so
+    // one group's loads are in flight while the matrix cores run
+    auto fetch = [&](uint64_t grp, float (&xr)[NX], float (&xi)[NX], uint64_t& base, bool& ok) {
+        const uint64_t col = grp * 16 + j;
+        ok = col < ncols;
+        base = ok ? (insert_zeros(col, fx.pos, fx.np) | cval) : 0;
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            float2 v = make_float2(0.f, 0.f);
+            if (ok) v = a[base + in_off[q]];
+            xr[q] = v.x;
+            xi[q] = v.y;
+        }
+    };
+    uint64_t grp = uint64_t(blockIdx.x) * kWaves + threadIdx.x / 64;
+    if (grp * 16 >= ncols) return;
+    float xr[NX], xi[NX], yr[NX], yi[NX];
+    uint64_t base, nbase = 0;
+    bool ok, nok = false;
+    fetch(grp, xr, xi, base, ok);
+    for (;;) {
+        const uint64_t next = grp + nwaves;
+        const bool more = next * 16 < ncols;  // uniform over the wave
+        if (more) fetch(next, yr, yi, nbase, nok);
+        floatx4 acc[NB];
+#pragma unroll
+        for (int mb = 0; mb < NB; ++mb) acc[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // slice s's matrix entries are read while slice s-1's products run.
+        // The matrix is read again at every group: an offset the compiler
+        // cannot see through (zero, from an instruction it must keep) stops
+        // it from hoisting all 2 DIM^2 / 16 entries into registers, and the
+        // scheduling barrier from issuing every slice's reads at once.
+        int z;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+        const float* wv = wl + z;
+        float wc[NB], wn[NB];
+#pragma unroll
+        for (int mb = 0; mb < NB; ++mb) wc[mb] = wv[mb * NS * 64 + lane];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s + 1 < NS) {
+#pragma unroll
+                for (int mb = 0; mb < NB; ++mb) wn[mb] = wv[(mb * NS + s + 1) * 64 + lane];
+            }
+            const float b = s < NX ? xr[s] : xi[s - NX];
+#pragma unroll
+            for (int mb = 0; mb < NB; ++mb)
+                acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[mb], b, acc[mb], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int mb = 0; mb < NB; ++mb) wc[mb] = wn[mb];
+        }
+        if (ok) {
+#pragma unroll
+            for (int mh = 0; mh < NO; ++mh)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    a[base + out_off[mh][v]] = make_float2(acc[mh][v], acc[mh + NO][v]);
+        }
+        if (!more) break;
+        grp = next;
+        base = nbase;
+        ok = nok;
+#pragma unroll
+        for (int q = 0; q < NX; ++q) {
+            xr[q] = yr[q];
+            xi[q] = yi[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // qudit (mixed-radix) gate
 // ---------------------------------------------------------------------------
 struct QG {
@@ -366,6 +489,9 @@ int upload(qs_state* st, const void* host, size_t bytes, void** dptr) {
     return QS_OK;
 }
 
+int gatek_mfma(qs_state* st, uint32_t k, const std::vector<uint64_t>& inner, const double* m,
+               bool conj, const Fixed& fx, uint64_t cval, uint64_t ncols);
+
 template <typename R>
 int gatek_impl(qs_state* st, const uint32_t* bits, uint32_t k, const double* m,
                uint64_t cmask, uint64_t cval, bool conj) {
@@ -384,6 +510,9 @@ int gatek_impl(qs_state* st, const uint32_t* bits, uint32_t k, const double* m,
     for (uint32_t b : bits_of(cmask)) fixed.push_back(b);
     const Fixed fx = make_fixed(fixed);
     const uint64_t ngroups = uint64_t{1} << (st->nbits - fx.np);
+    if constexpr (std::is_same<R, float>::value)
+        if ((k == 4 || k == 5) && !qs::gatek_valu_forced())
+            return gatek_mfma(st, k, inner, m, conj, fx, cval, ngroups);
     const size_t inner_bytes = inner.size() * sizeof(uint64_t);
     const size_t mat_bytes = mm.size() * sizeof(cxt<R>);
     std::vector<unsigned char> blob(inner_bytes + mat_bytes);
@@ -404,6 +533,46 @@ int gatek_impl(qs_state* st, const uint32_t* bits, uint32_t k, const double* m,
                        reinterpret_cast<const cxt<R>*>(
                            static_cast<unsigned char*>(dblob) + inner_bytes),
                        fx, cval, uint64_t{0}, uint64_t{0});
+    QS_HIP(hipFreeAsync(dblob, st->stream), "qpp_amd::apply");
+    return QS_OK;
+}
+
+// W = [[Ur, -Ui], [Ui, Ur]] (m column-major complex, conj: conj(U)) packed
+// in k_gatek_mfma's A-operand order: entry (mb, s, lane) = W[16 mb + lane %
+// 16][4 s + lane / 16].
+int gatek_mfma(qs_state* st, uint32_t k, const std::vector<uint64_t>& inner, const double* m,
+               bool conj, const Fixed& fx, uint64_t cval, uint64_t ncols) {
+    const int dim = 1 << k, nr = 2 * dim, ns = nr / 4, nb = nr / 16;
+    auto W = [&](int r, int c) -> float {
+        const int rr = r % dim, cc = c % dim;
+        const double re = m[2 * (rr + cc * dim)];
+        const double im = conj ? -m[2 * (rr + cc * dim) + 1] : m[2 * (rr + cc * dim) + 1];
+        if ((r < dim) == (c < dim)) return static_cast<float>(re);
+        return static_cast<float>(r < dim ? -im : im);
+    };
+    std::vector<float> wp(size_t(nb) * ns * 64);
+    for (int mb = 0; mb < nb; ++mb)
+        for (int s = 0; s < ns; ++s)
+            for (int l = 0; l < 64; ++l) wp[(size_t(mb) * ns + s) * 64 + l] = W(16 * mb + l % 16, 4 * s + l / 16);
+    const size_t inner_bytes = inner.size() * sizeof(uint64_t);
+    std::vector<unsigned char> blob(inner_bytes + wp.size() * sizeof(float));
+    std::memcpy(blob.data(), inner.data(), inner_bytes);
+    std::memcpy(blob.data() + inner_bytes, wp.data(), wp.size() * sizeof(float));
+    void* dblob = nullptr;
+    if (int rc = upload(st, blob.data(), blob.size(), &dblob)) return rc;
+    // persistent waves (each walks many groups, prefetching the next): a
+    // few workgroups per CU at most
+    const uint64_t groups16 = (ncols + 15) / 16;
+    const uint32_t grid = std::min<uint32_t>(grid_for(groups16, kBlock / 64), kMfmaGrid);
+    auto* inn = static_cast<const uint64_t*>(dblob);
+    auto* wpk = reinterpret_cast<const float*>(static_cast<unsigned char*>(dblob) + inner_bytes);
+    if (k == 4)
+        hipLaunchKernelGGL(k_gatek_mfma<4>, dim3(grid), dim3(kBlock), 0, st->stream,
+                           static_cast<float2*>(st->d), ncols, inn, wpk, fx, cval);
+    else
+        hipLaunchKernelGGL(k_gatek_mfma<5>, dim3(grid), dim3(kBlock), 0, st->stream,
+                           static_cast<float2*>(st->d), ncols, inn, wpk, fx, cval);
+    QS_HIP(hipGetLastError(), "qpp::apply()");
     QS_HIP(hipFreeAsync(dblob, st->stream), "qpp_amd::apply");
     return QS_OK;
 }

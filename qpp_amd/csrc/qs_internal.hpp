@@ -65,6 +65,9 @@ void set_thread_stream(hipStream_t s);
 // table (the reference's sequential sums) instead of accepting the
 // error-bounded parallel search
 bool force_exact_sampling();
+// measurement hook: QPP_AMD_GATEK_VALU=1 applies 4- and 5-qubit dense gates
+// on complex64 states with the VALU kernel instead of the MFMA one
+bool gatek_valu_forced();
 // qpp::measure_seq with the sampled basis index given (qs_measure.hip)
 int measure_seq_given(qs_state* st, uint32_t ntarget, const uint64_t* target,
                       int destructive, uint64_t index, uint64_t* results, double* probs);

@@ -106,3 +106,63 @@ def test_c64_normalize_in_fp64():
     got = S.download()
     assert np.max(np.abs(got - ref)) <= 2 * np.finfo(np.float32).eps
     assert abs(S.norm2() - 1) < 1e-6
+
+
+def _dense_cases(n, rng):
+    """(target, control) sets for 4- and 5-qubit gates: low, high and mixed
+    target bits, with and without controls."""
+    return [([0, 1, 2, 3], []), ([n - 1, n - 2, n - 3, n - 4, n - 5], []),
+            ([2, n - 1, 5, 0], [n - 3]), ([7, 3, n - 2, 1, 9], [4, n - 1]),
+            (list(rng.permutation(n)[:5]), [])]
+
+
+@pytest.mark.parametrize("valu", [False, True])
+def test_c64_dense_gates_mfma_vs_oracle(orc, monkeypatch, valu):
+    """4- and 5-qubit dense gates on complex64 kets and density matrices
+    (FP32 MFMA kernel; valu: the VALU kernel forced by QPP_AMD_GATEK_VALU)
+    against the complex128 oracle: max-abs <= 2e-6 (FP32 sums of up to 32
+    terms of a unit column)."""
+    if valu:
+        monkeypatch.setenv("QPP_AMD_GATEK_VALU", "1")
+    rng = np.random.default_rng(41)
+    n = 12
+    psi = randket(2 ** n, rng)
+    for tg, ct in _dense_cases(n, rng):
+        U = haar(2 ** len(tg), rng)
+        print(f"ket n={n} target={tg} ctrl={ct} valu={valu}", flush=True)
+        st = qa.State.from_array(psi, [2] * n, dtype=qa.C64)
+        if ct:
+            st.apply_ctrl(U, ct, tg)
+            ref = orc.applyCTRL(psi, U, ct, tg, [2] * n)
+        else:
+            st.apply(U, tg)
+            ref = orc.apply(psi, U, tg, 2)
+        assert np.max(np.abs(st.download() - ref)) <= 2e-6
+    nr = 6
+    rho = randrho(2 ** nr, rng)
+    for tg in ([0, 1, 2, 3], [5, 1, 3, 0, 2]):
+        U = haar(2 ** len(tg), rng)
+        print(f"rho n={nr} target={tg} valu={valu}", flush=True)
+        st = qa.State.from_array(rho, [2] * nr, dtype=qa.C64)
+        st.apply(U, tg)
+        assert np.max(np.abs(st.download() - orc.apply(rho, U, tg, 2))) <= 2e-6
+
+
+def test_c64_dense_gate_mfma_equals_valu_at_size(monkeypatch):
+    """A 5-qubit dense gate on a 24-qubit complex64 ket: the MFMA and VALU
+    kernels agree to FP32 rounding (max-abs <= 1e-6 on a normalised state),
+    and unitarity keeps the norm within 1e-6."""
+    rng = np.random.default_rng(42)
+    n = 24
+    U = haar(32, rng)
+    a = qa.State([2] * n, dtype=qa.C64)
+    a.apply(haar(2, rng), [0])
+    for q in range(n):
+        a.apply(haar(2, rng), [q])
+    b = a.astype(qa.C64)
+    a.apply(U, [3, 20, 9, 0, 15])
+    monkeypatch.setenv("QPP_AMD_GATEK_VALU", "1")
+    b.apply(U, [3, 20, 9, 0, 15])
+    r = a.compare(b)
+    print(r, flush=True)
+    assert r["max_abs"] <= 1e-6 and abs(r["norm2_a"] - 1) <= 1e-6
