@@ -544,8 +544,8 @@ def bench_block(args, qa, n=30, reps=3):
     complex64 ket, every amplitude read and written once per launch: the
     FP32 MFMA kernel (k_gatek_mfma) against the VALU kernel (k_gatek, forced
     with QPP_AMD_GATEK_VALU=1), plus the complex128 VALU kernel.  HBM
-    roofline: 2 s 2^n bytes per launch; MFMA work: 4 DIM real FMA per
-    amplitude (the complex product as a real GEMM)."""
+    roofline: 2 s 2^n bytes per launch; arithmetic: 4 x 2^k real FMA per
+    amplitude on either unit (the complex product as a real GEMM)."""
     import os
     from qpp_amd.workloads import haar
     rng = np.random.default_rng(args.seed)
@@ -567,7 +567,7 @@ def bench_block(args, qa, n=30, reps=3):
                 os.environ.pop("QPP_AMD_GATEK_VALU", None)
             es = 8 if dt == qa.C64 else 16
             gbs = 2 * es * 2 ** n / (ms / 1e3) / 1e9
-            fl = 2 * (4 * 2 ** k if label == "c64_mfma" else 8 * 2 ** k) * 2 ** n
+            fl = 8 * 2 ** k * 2 ** n  # 4 * 2^k real FMA per amplitude on either unit
             row[label] = {"ms": round(ms, 3), "GBps": round(gbs, 1),
                           "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                           "TFLOPs": round(fl / (ms / 1e3) / 1e12, 2)}
@@ -580,9 +580,9 @@ def bench_block(args, qa, n=30, reps=3):
         "metric": "dense k-qubit gate on a 30-qubit complex64 ket: MFMA vs VALU",
         "workload": f"{n}-qubit ket, one dense 2^k x 2^k unitary per launch (k = 4, 5), "
                     f"{reps} launches timed with HIP events (matrix upload included)",
-        "flops_definition": "MFMA: 2 x 4 x 2^k real flops per amplitude (complex product as a "
-                            "real GEMM); VALU: 2 x 8 x 2^k (complex FMA); peaks 157.3 TFLOP/s "
-                            "FP32 matrix, 78.6 TFLOP/s FP32 / FP64 vector",
+        "flops_definition": "8 x 2^k real flops per amplitude (4 x 2^k FMA: the complex "
+                            "product, or the same count as a real GEMM on the matrix cores); "
+                            "peaks 157.3 TFLOP/s FP32 matrix, 78.6 TFLOP/s FP32 / FP64 vector",
         "blocks": out,
     }
 

@@ -246,8 +246,8 @@ __global__ __launch_bounds__(kBlock) void k_gatek(
 //   D: row block mb < NB/2 holds real parts and mb + NB/2 the imaginary
 //      parts of the same complex rows 16 mb + 4 g + v, so each lane writes
 //      whole complex amplitudes.
-// Arithmetic per amplitude: 4 DIM FP32 FMA on the matrix cores (twice the
-// FMA count of the complex product on VALU, at twice the VALU rate).
+// Arithmetic per amplitude: 4 DIM FP32 FMA, as many as the complex product
+// takes on VALU, at twice the VALU FP32 rate (packed FP32 aside).
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kMfmaGrid = 2048;
 
