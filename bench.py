@@ -587,6 +587,43 @@ def bench_block(args, qa, n=30, reps=3):
     }
 
 
+def bench_sample(args, qa, sizes=(20, 24, 26), shots=1024, reps=5):
+    """qpp::sample of every qubit (1024 shots) on resident random complex128
+    kets of 2^20..2^26 amplitudes: the error-bounded parallel search, and the
+    exact libstdc++ table (binade scans) forced for every shot with
+    QPP_AMD_FORCE_EXACT=1.  Host wall time per call (the call returns the
+    indices to the host)."""
+    import os
+    rng = np.random.default_rng(args.seed)
+    rows = []
+    for n in sizes:
+        D = 2 ** n
+        psi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+        psi /= np.linalg.norm(psi)
+        st = qa.State.from_array(psi, [2] * n)
+        u = rng.random(shots)
+        row = {"qubits": n}
+        for label, force in (("bounded_search_ms", "0"), ("exact_table_ms", "1")):
+            os.environ["QPP_AMD_FORCE_EXACT"] = force
+            try:
+                st.sample(list(range(n)), u)
+                qa.synchronize()
+                t = time.perf_counter()
+                for _ in range(reps):
+                    st.sample(list(range(n)), u)
+                qa.synchronize()
+                row[label] = round((time.perf_counter() - t) / reps * 1e3, 3)
+            finally:
+                os.environ.pop("QPP_AMD_FORCE_EXACT", None)
+        # exact table traffic: p read 4 times (sum, q scan, table, search
+        # neighbourhood) and two tables written: about 5 x 8 D bytes
+        row["exact_table_GBps"] = round(5 * 8 * D / (row["exact_table_ms"] / 1e3) / 1e9, 1)
+        log(f"sample n={n}: {row}")
+        rows.append(row)
+        del st
+    return {"metric": "qpp::sample of 1024 shots, ms per call", "shots": shots, "rows": rows}
+
+
 def bench_copy(args, qa):
     """Reference streaming kernel: a 30-qubit complex128 state (16 GiB) copied
     into another with qs_state_copy (each byte read once, written once).
@@ -874,7 +911,8 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=30)
-    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy", "block"], default="fused")
+    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy", "block", "sample"],
+                    default="fused")
     ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
     ap.add_argument("--cal-requests", type=int, default=40,
                     help="serve mode: closed-loop calibration requests")
@@ -937,6 +975,9 @@ def main():
         return
     if args.mode == "block":
         print(json.dumps(bench_block(args, qa, n=args.qubits)), flush=True)
+        return
+    if args.mode == "sample":
+        print(json.dumps(bench_sample(args, qa)), flush=True)
         return
     out = bench_circuit(args, qa, args.qubits, args.depth, primary=True)
     if args.mode == "fused" and args.secondary:
