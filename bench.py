@@ -576,8 +576,35 @@ def bench_block(args, qa, n=30, reps=3):
         row["mfma_speedup_vs_valu"] = round(row["c64_valu"]["ms"] / row["c64_mfma"]["ms"], 3)
         log(f"block k={k}: {row}")
         out.append(row)
+    # the 5-qubit gate is the MFMA-heavy case: its roofline against the FP32
+    # matrix peak, beside its HBM rate
+    k5 = out[-1]["c64_mfma"]
+    roof = {"bound": "mfma", "kernel": "k_gatek_mfma<5>", "achieved": k5["TFLOPs"], "peak": 157.3,
+            "unit": "TFLOP/s", "frac": round(k5["TFLOPs"] / 157.3, 4), "traffic": None,
+            "avg_launch_ms": k5["ms"], "hbm_GBps": k5["GBps"], "hbm_frac": k5["hbm_frac"]}
+    # CPU: the oracle's qpp apply (dense k-qubit kernel restated, OpenMP) of
+    # the same 32x32 gate on a 24-qubit complex128 ket, scaled by 2^6
+    cpu = None
+    try:
+        import oracle
+        m = 24
+        psi = np.zeros(2 ** m, dtype=np.complex128)
+        psi[0] = 1
+        U5 = haar(32, rng)
+        t0 = time.perf_counter()
+        oracle.apply(psi, U5, [0, 7, 13, 21, 23], 2)
+        dt_cpu = time.perf_counter() - t0
+        cpu = {"value": round(2 * 16 * 2 ** m / dt_cpu / 1e9, 3), "unit": "GB/s", "cores": oracle_threads(),
+               "kind": "port",
+               "sample": f"one 32x32 gate on a {m}-qubit complex128 ket through the oracle's apply "
+                         f"(qpp's apply_psi_kq restated, OpenMP): {dt_cpu:.2f} s; at {n} qubits about "
+                         f"{dt_cpu * 2 ** (n - m):.0f} s (EXTRAPOLATED x 2^{n - m})"}
+    except Exception as e:  # the oracle is test infrastructure; report its absence
+        cpu = {"value": None, "unit": "GB/s", "cores": 0, "kind": "port", "sample": f"not run: {e}"}
     return {
         "metric": "dense k-qubit gate on a 30-qubit complex64 ket: MFMA vs VALU",
+        "roofline": roof,
+        "cpu_baseline": cpu,
         "workload": f"{n}-qubit ket, one dense 2^k x 2^k unitary per launch (k = 4, 5), "
                     f"{reps} launches timed with HIP events (matrix upload included)",
         "flops_definition": "8 x 2^k real flops per amplitude (4 x 2^k FMA: the complex "
