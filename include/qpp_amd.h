@@ -436,6 +436,14 @@ int qs_engine_get_measured(const qs_engine* e, uint8_t* measured);
 int qs_engine_reset_stats(qs_engine* e);
 /* QEngineT::post_select_ok(): 0 after a run whose post-selection failed. */
 int qs_engine_post_select_ok(const qs_engine* e, int* ok);
+/* No qpp counterpart: where qpp would run for ever, execute stops with
+ * QS_ERR_OUT_OF_RANGE -- a runtime while loop whose body has run
+ * max_loop_reps times in one run with its condition still true (default
+ * 2^24), and a post-selection retried without limit (max_reps unset, see
+ * qs_engine_set_post_selection) that has failed max_ps_failures times
+ * (default 2^16).  0 removes a limit (qpp's behaviour: qengine.hpp:414-473,
+ * 837-880). */
+int qs_engine_set_limits(qs_engine* e, uint64_t max_loop_reps, uint64_t max_ps_failures);
 /* QNoisyEngineT::get_noise_results (qnoisy_engine.hpp:104-121, 174-176):
  * the Kraus indices drawn before circuit step `step` (0 the identity, 1, 2, 3
  * X, Y, Z of the depolarizing channel), one per live qubit, in ascending

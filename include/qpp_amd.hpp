@@ -648,6 +648,13 @@ class QEngine {
     }
     bool get_ensure_post_selection() const { return ensure_; }
     idx get_max_post_selection_reps() const { return max_ps_; }
+    // no qpp counterpart: where qpp runs for ever, stop with OutOfRange after
+    // max_loop_reps runs of a while loop's body / max_ps_failures failures
+    // of an unlimited post-selection (0: never, qpp's behaviour)
+    QEngine& set_limits(idx max_loop_reps, idx max_ps_failures) {
+        check(qs_engine_set_limits(e_, max_loop_reps, max_ps_failures));
+        return *this;
+    }
     // qengine.hpp:1035-1164: measured qudits, dits, statistics
     QEngine& set_dits(const std::vector<idx>& dits) {
         std::vector<uint64_t> d(dits.begin(), dits.end());

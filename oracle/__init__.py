@@ -63,6 +63,7 @@ def lib():
                                         POINTER(c_int)]
         L.orc_reset.argtypes = [dp, c_int, c_uint32, u64p, c_uint32, u64p, c_double, dp]
         L.orc_last_noise_results.argtypes = [c_uint64, u64p, u64p]
+        L.orc_set_engine_limits.argtypes = [c_uint64, c_uint64]
         _lib = L
     return _lib
 
@@ -264,10 +265,14 @@ def reset(A, target, dims=2, u=0.5):
 
 
 def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1 << 16,
-               ensure_post_selection=False, max_post_selection_reps=None, with_ps_ok=False):
+               ensure_post_selection=False, max_post_selection_reps=None, with_ps_ok=False,
+               max_loop_reps=1 << 24, max_ps_failures=1 << 16):
     """QEngine / QNoisyEngine execute(reps) on a qpp_amd.QCircuit; returns
     (stats dict, dits, probs, final state), plus post_select_ok() when
-    with_ps_ok."""
+    with_ps_ok.  max_loop_reps / max_ps_failures: where qpp runs for ever,
+    give up with OutOfRange after that many while-loop body runs or failed
+    unlimited post-selections (0: never)."""
+    lib().orc_set_engine_limits(int(max_loop_reps), int(max_ps_failures))
     cs = qc.struct()
     nc = qc.nc
     n_out = np.zeros(1, dtype=np.uint64)
@@ -287,6 +292,7 @@ def engine_run(qc, reps=1, noise_p=-1.0, seed=5489, is_rho=False, max_outcomes=1
                             sc.ctypes.data_as(POINTER(c_uint64)), dits.ctypes.data_as(POINTER(c_uint64)),
                             probs.ctypes.data_as(POINTER(c_double)), st.ctypes.data_as(POINTER(c_double)),
                             olen.ctypes.data_as(POINTER(c_uint64)), ctypes.byref(ok))
+    lib().orc_set_engine_limits(1 << 24, 1 << 16)
     if rc != 0:
         err = RuntimeError(f"oracle engine stopped with qs_status {rc}")
         err.code = rc

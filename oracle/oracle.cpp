@@ -1106,6 +1106,9 @@ struct Engine {
     std::vector<std::vector<idx>> noise_results;
     idx cur_ip = 0;  // the step being executed
     idx loops = 0;   // while-loop repetitions in the current run
+    // where qpp runs for ever: give up after this many body runs of a while
+    // loop / failures of an unlimited post-selection (0: never)
+    idx loop_give_up = idx{1} << 24, ps_give_up = idx{1} << 16;
 
     double canonical() { return std::generate_canonical<double, 53>(gen); }
 
@@ -1199,8 +1202,8 @@ struct Engine {
                 } else if (attempts < st.max_ps) {
                     // unlimited retries of a result that never comes: give
                     // up (qpp loops for ever) after 2^16 attempts
-                    if (st.max_ps == std::numeric_limits<idx>::max() &&
-                        attempts >= (idx{1} << 16))
+                    if (st.max_ps == std::numeric_limits<idx>::max() && ps_give_up &&
+                        attempts >= ps_give_up)
                         throw EngineError{QS_ERR_OUT_OF_RANGE};
                     continue;
                 } else {
@@ -1327,9 +1330,9 @@ struct Engine {
             return cond_true(s) ? i : s.jump;
         if (s.type == QS_STEP_COND_ELSE) return s.jump;
         if (s.type == QS_STEP_COND_END && s.loop_end && cond_true(steps_[s.jump])) {
-            // a loop that never ends (qpp runs it for ever) stops after 2^24
-            // repetitions in one run
-            if (++loops >= (idx{1} << 24)) throw EngineError{QS_ERR_OUT_OF_RANGE};
+            // a loop that never ends (qpp runs it for ever) stops once its
+            // body has run loop_give_up times in one run
+            if (++loops >= loop_give_up && loop_give_up) throw EngineError{QS_ERR_OUT_OF_RANGE};
             return s.jump;  // back to the loop's body
         }
         return i;
@@ -1435,7 +1438,7 @@ struct Engine {
                 // unlimited retries of a sample that never succeeds: give up
                 // (qpp loops for ever) after 2^16 attempts
                 if (failed && st.ensure_ps && st.max_ps == std::numeric_limits<idx>::max() &&
-                    attempts >= (idx{1} << 16))
+                    ps_give_up && attempts >= ps_give_up)
                     throw EngineError{QS_ERR_OUT_OF_RANGE};
             } while (failed && st.ensure_ps && attempts < st.max_ps);
             if (failed) continue;
@@ -1746,6 +1749,14 @@ int orc_engine_run(const qs_circuit* qc, int is_rho, double noise_p,
                              nullptr);
 }
 
+// give-up counts for the following engine runs (0: never give up)
+static idx g_loop_give_up = idx{1} << 24, g_ps_give_up = idx{1} << 16;
+int orc_set_engine_limits(uint64_t max_loop_reps, uint64_t max_ps_failures) {
+    g_loop_give_up = max_loop_reps;
+    g_ps_give_up = max_ps_failures;
+    return 0;
+}
+
 int orc_last_noise_results(uint64_t step, uint64_t* out, uint64_t* count) {
     if (step >= g_last_noise.size()) return QS_ERR_OUT_OF_RANGE;
     *count = g_last_noise[step].size();
@@ -1765,6 +1776,8 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
     e.d = qc->d;
     e.is_rho = is_rho != 0;
     e.noise_p = noise_p;
+    e.loop_give_up = g_loop_give_up;
+    e.ps_give_up = g_ps_give_up;
     e.gen.seed(static_cast<std::mt19937::result_type>(seed));
     for (uint32_t m = 0; m < qc->nmats; ++m)
         e.mats.push_back(from_raw(qc->mats + 2 * qc->mat_off[m], qc->mat_dim[m]));

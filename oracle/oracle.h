@@ -138,6 +138,11 @@ int orc_engine_run(const struct qs_circuit* qc, int is_rho, double noise_p,
  * for a step on a destructively measured qudit (qengine.hpp:514-526),
  * QS_ERR_OUT_OF_RANGE for a post-selection retried without limit that
  * failed 2^16 times. */
+/* give-up counts of the following engine runs: a while loop whose body has
+ * run max_loop_reps times with its condition true, an unlimited
+ * post-selection that failed max_ps_failures times (2^24, 2^16 by
+ * default; 0 = never, qpp's behaviour) */
+int orc_set_engine_limits(uint64_t max_loop_reps, uint64_t max_ps_failures);
 int orc_engine_run_ps(const struct qs_circuit* qc, int is_rho, double noise_p,
                       uint64_t seed, uint64_t reps, int ensure_ps,
                       uint64_t max_ps, uint64_t max_outcomes,

@@ -1045,6 +1045,17 @@ class QEngine:
         self._max_ps = int(n)
         return self._set_ps()
 
+    def set_limits(self, max_loop_reps=1 << 24, max_ps_failures=1 << 16):
+        """Where qpp runs for ever, stop with OutOfRange: after a while
+        loop's body has run max_loop_reps times in one run with its
+        condition still true, and after max_ps_failures failures of a
+        post-selection retried without limit.  0 removes a limit (qpp's
+        behaviour)."""
+        L = lib()
+        L.qs_engine_set_limits.argtypes = [c_void_p, c_uint64, c_uint64]
+        _check(L.qs_engine_set_limits(self._h, int(max_loop_reps), int(max_ps_failures)))
+        return self
+
     def _set_ps(self):
         L = lib()
         L.qs_engine_set_post_selection.argtypes = [c_void_p, c_int, c_uint64]

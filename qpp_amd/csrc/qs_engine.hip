@@ -48,11 +48,13 @@ constexpr uint64_t kMeasured = std::numeric_limits<uint64_t>::max();
 // successful rep of the sampling tail) never ends when the selected result
 // has probability 0, e.g. after a measurement that contradicts it; qpp
 // then loops for ever.  Here such a step gives up with an error after
-// kPsGiveUp attempts (a result of probability p fails that often with
-// probability (1-p)^65536, below 1e-28 for p >= 1e-3).
+// ps_give_up failed attempts, 2^16 unless set (a result of probability p
+// fails that often with probability (1-p)^65536, below 1e-28 for p >=
+// 1e-3).  The same for a runtime while loop whose condition never turns
+// false: loop_give_up repetitions of its body in one run, 2^24 unless set.
+// qs_engine_set_limits changes both; 0 means no limit (qpp's behaviour).
 constexpr uint64_t kUnlimited = std::numeric_limits<uint64_t>::max();
 constexpr uint64_t kPsGiveUp = uint64_t{1} << 16;
-// the same for a runtime while loop whose condition never turns false
 constexpr uint64_t kLoopGiveUp = uint64_t{1} << 24;
 
 struct EStep {
@@ -177,6 +179,7 @@ struct qs_engine {
     // internal/classes/qengine_state.hpp:66-72
     bool ps_ok = true, ensure_ps = false;
     uint64_t max_ps = std::numeric_limits<uint64_t>::max();
+    uint64_t ps_give_up = kPsGiveUp, loop_give_up = kLoopGiveUp;  // 0: none
 
     double canonical() { return std::generate_canonical<double, 53>(gen); }
 
@@ -445,9 +448,9 @@ struct qs_engine {
                 if (!ensure_ps) {
                     ps_ok = false;
                 } else if (attempts < max_ps) {
-                    if (max_ps == kUnlimited && attempts >= kPsGiveUp)
+                    if (max_ps == kUnlimited && ps_give_up && attempts >= ps_give_up)
                         return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
-                                    "post-selection failed 65536 times with unlimited retries");
+                                    "post-selection failed the give-up count of times with unlimited retries");
                     continue;
                 } else {
                     ps_ok = false;
@@ -533,12 +536,13 @@ struct qs_engine {
                 // ENDWHILE (:869-880): while the loop's condition holds, go
                 // on at the first step of its body.  A loop whose condition
                 // never turns false would run for ever (as in qpp); it stops
-                // here after kLoopGiveUp repetitions in one run.
+                // here once its body has run loop_give_up times in one run
+                // and the condition still holds.
                 if (x.loop_end && cond_true(s[x.jump])) {
-                    if (++loops >= kLoopGiveUp) {
+                    if (++loops >= loop_give_up && loop_give_up) {
                         run_gates(pending, false);
                         return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
-                                    "while loop repeated 2^24 times");
+                                    "while loop repeated the give-up count of times");
                     }
                     i = x.jump;
                 }
@@ -748,9 +752,9 @@ struct qs_engine {
                 if (!(ensure_ps && attempts < max_ps)) {
                     ++rep;
                     attempts = 0;
-                } else if (max_ps == kUnlimited && attempts >= kPsGiveUp) {
+                } else if (max_ps == kUnlimited && ps_give_up && attempts >= ps_give_up) {
                     rc = fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
-                              "post-selection failed 65536 times with unlimited retries");
+                              "post-selection failed the give-up count of times with unlimited retries");
                 }
                 continue;
             }
@@ -777,9 +781,9 @@ struct qs_engine {
                     if (!(ensure_ps && attempts < max_ps)) {
                         ++rep;
                         attempts = 0;
-                    } else if (max_ps == kUnlimited && attempts >= kPsGiveUp) {
+                    } else if (max_ps == kUnlimited && ps_give_up && attempts >= ps_give_up) {
                         rc = fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
-                                  "post-selection failed 65536 times with unlimited retries");
+                                  "post-selection failed the give-up count of times with unlimited retries");
                     }
                     continue;
                 }
@@ -1098,6 +1102,15 @@ int qs_engine_set_post_selection(qs_engine* e, int ensure, uint64_t max_reps) {
     if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::set_ensure_post_selection()");
     e->ensure_ps = ensure != 0;
     e->max_ps = max_reps;
+    return QS_OK;
+}
+
+// The give-up counts of unlimited post-selection retries and of runtime
+// while loops (2^16 and 2^24 by default; 0 = no limit, as in qpp).
+int qs_engine_set_limits(qs_engine* e, uint64_t max_loop_reps, uint64_t max_ps_failures) {
+    if (!e) return fail(QS_ERR_INVALID_HANDLE, "qpp::QEngineT::set_limits()");
+    e->loop_give_up = max_loop_reps;
+    e->ps_give_up = max_ps_failures;
     return QS_OK;
 }
 

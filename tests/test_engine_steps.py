@@ -446,6 +446,65 @@ def test_oracle_while_loop_iterations(orc):
         assert dits == [1] and abs(abs(np.vdot(exp, st)) - 1) < 1e-12
 
 
+def _loop_runs(orc, seed):
+    """Body runs of until_one() for this seed: 1 + the draws below 1/2
+    before the first one at or above it."""
+    r = orc.Rng(seed)
+    k = 1
+    while r.canonical() < 0.5:
+        k += 1
+    return k
+
+
+def _ps_attempts(orc, seed):
+    """Attempts until post-selecting q0 = 1 of H|0> succeeds: the first
+    draw at or above 1/2 (one draw per attempt)."""
+    r = orc.Rng(seed)
+    k = 1
+    while r.canonical() < 0.5:
+        k += 1
+    return k
+
+
+def post_select_one():
+    qc = qa.QCircuit(2, 1)
+    return qc.gate(H, 0).post_select(0, 1, 0, destructive=False).gate(ry(THETA), 1)
+
+
+def test_oracle_give_up_counts_are_exact(orc):
+    """The while-loop limit L stops a run whose loop body has run L times
+    with its condition still true: a loop that needs k runs passes at
+    L = k and stops with OutOfRange at L = k - 1.  An unlimited
+    post-selection that succeeds at attempt k has k - 1 failures first: it
+    passes with a failure limit P = k and stops at P = k - 1.  0 removes a
+    limit."""
+    done = 0
+    for seed in range(40):
+        k = _loop_runs(orc, seed)
+        if k < 2:
+            continue
+        _, dits, _, st = orc.engine_run(until_one(), reps=1, seed=seed, max_loop_reps=k)
+        exp = np.kron([0, 1], ry(k * THETA) @ [1, 0])
+        assert dits == [1] and abs(abs(np.vdot(exp, st)) - 1) < 1e-12
+        orc.engine_run(until_one(), reps=1, seed=seed, max_loop_reps=0)
+        with pytest.raises(RuntimeError) as ex:
+            orc.engine_run(until_one(), reps=1, seed=seed, max_loop_reps=k - 1)
+        assert ex.value.code == qa.OutOfRange.code
+        k = _ps_attempts(orc, seed)
+        _, dits, _, _, ok = orc.engine_run(post_select_one(), reps=1, seed=seed,
+                                           ensure_post_selection=True, max_ps_failures=k,
+                                           with_ps_ok=True)
+        assert ok and dits == [1]
+        orc.engine_run(post_select_one(), reps=1, seed=seed, ensure_post_selection=True,
+                       max_ps_failures=0)
+        with pytest.raises(RuntimeError) as ex:
+            orc.engine_run(post_select_one(), reps=1, seed=seed, ensure_post_selection=True,
+                           max_ps_failures=k - 1)
+        assert ex.value.code == qa.OutOfRange.code
+        done += 1
+    assert done >= 10
+
+
 def test_oracle_while_rejects_destructive_steps(orc):
     bad = qa.QCircuit(2, 1).cond_while([0], [0]).gate(H, 0).measure(0, 0).cond_end()
     with pytest.raises(RuntimeError):
@@ -469,6 +528,30 @@ def test_engine_while_vs_oracle(orc):
     forever = qa.QCircuit(1, 1).cond_while([0], [0]).nop().cond_end()
     with pytest.raises(qa.OutOfRange):
         qa.QEngine(forever).execute(1)
+
+
+@gpu
+def test_engine_give_up_counts_vs_oracle(orc):
+    """The device engine's settable limits cut off exactly where the
+    oracle's do (see test_oracle_give_up_counts_are_exact)."""
+    checked = 0
+    for seed in range(40):
+        k = _loop_runs(orc, seed)
+        if k < 2:
+            continue
+        e = qa.QEngine(until_one(), seed=seed).set_limits(k, 1 << 16).execute(1)
+        assert e.get_dits() == [1]
+        with pytest.raises(qa.OutOfRange):
+            qa.QEngine(until_one(), seed=seed).set_limits(k - 1, 1 << 16).execute(1)
+        k = _ps_attempts(orc, seed)
+        e = qa.QEngine(post_select_one(), seed=seed).set_ensure_post_selection(True)
+        e.set_limits(1 << 24, k).execute(1)
+        assert e.get_dits() == [1] and e.post_select_ok()
+        e = qa.QEngine(post_select_one(), seed=seed).set_ensure_post_selection(True)
+        with pytest.raises(qa.OutOfRange):
+            e.set_limits(1 << 24, k - 1).execute(1)
+        checked += 1
+    assert checked >= 10
 
 
 # ---------------------------------------------------------------------------
