@@ -721,9 +721,18 @@ def bench_circuit(args, qa, n, depth, primary):
         log(f"autotune: {tuned} in {time.perf_counter() - tt:.1f} s")
         args.reg_bits, args.low_bits = tuned["reg_bits"], tuned["low_bits"]
         args.max_ops, args.grid_cap = tuned["max_ops"], tuned["grid_cap"]
+    native_s = None
     if mode == "fused":
         prog = qa.Program(n, gates, reg_bits=args.reg_bits, low_bits=args.low_bits,
                           max_ops=args.max_ops, grid_cap=args.grid_cap)
+        if args.native and args.reg_bits == 4:
+            tn = time.perf_counter()
+            try:
+                prog.compile_native()
+                native_s = time.perf_counter() - tn
+                log(f"specialised pass kernels compiled in {native_s:.1f} s")
+            except qa.QppError as e:  # stays on the general tile kernel
+                log(f"specialised pass kernels not built: {e}")
         run = lambda: prog.run(st)  # noqa: E731
         npasses = prog.num_passes()
         pass_bytes = npasses * 2 * 16 * 2 ** n
@@ -783,6 +792,8 @@ def bench_circuit(args, qa, n, depth, primary):
         s64 = qa.State([2] * n, dtype=qa.C64)
         p64 = qa.Program(n, gates, dtype=qa.C64, reg_bits=args.reg_bits, low_bits=args.low_bits,
                          max_ops=args.max_ops, grid_cap=args.grid_cap)
+        if native_s is not None:
+            p64.compile_native()
         p64.run(s64)
         cmp = st.compare(s64)
         s64.set_basis(0)
@@ -819,6 +830,9 @@ def bench_circuit(args, qa, n, depth, primary):
             "fuse_opts": {"reg_bits": args.reg_bits, "low_bits": args.low_bits,
                           "max_ops": args.max_ops, "grid_cap": args.grid_cap,
                           "autotuned": tuned is not None},
+            "pass_kernels": ("specialised per pass (generated HIP, hipRTC), compiled in "
+                             f"{native_s:.1f} s outside the timed region") if native_s is not None
+                            else "general tile kernel k_tile",
         },
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "value_definition": "HBM bytes the circuit's passes stream (2*16*2^n per pass: the state "
@@ -973,6 +987,8 @@ def main():
     ap.add_argument("--low-bits", type=int, default=5)
     ap.add_argument("--max-ops", type=int, default=64)
     ap.add_argument("--grid-cap", type=int, default=0)
+    ap.add_argument("--native", type=int, default=1,
+                    help="fused mode: run the passes as specialised kernels (compiled once per program)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="fused mode: pick tile options with the autotuner first (1/0)")
     ap.add_argument("--sweep", action="store_true",

@@ -385,6 +385,12 @@ int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates
                            const double* mats, qs_dtype dtype,
                            const qs_fuse_opts* opts, qs_program** out);
 int qs_program_run(qs_program* p, qs_state* st);
+/* No qpp counterpart: generate the program's passes as specialised kernels
+ * (matrices and layouts as literals, register-only X gates as renamings, no
+ * op dispatch), compile them with hipRTC for gfx950 and use them in later
+ * runs (reg_bits 4 programs).  Takes seconds; a failure leaves the program
+ * on the general tile kernel. */
+int qs_program_compile_native(qs_program* p);
 /* qs_program_run with a HIP event recorded on the state's stream before the
  * first pass and after each pass; waits for the last one and writes each
  * pass's elapsed milliseconds to pass_ms[0 .. num_passes-1] (the kernel

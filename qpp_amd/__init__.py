@@ -848,6 +848,16 @@ class Program:
     def run(self, st):
         _check(lib().qs_program_run(self._h, st.handle))
 
+    def compile_native(self):
+        """Generate this program's passes as specialised kernels and compile
+        them with hipRTC (qs_program_compile_native; seconds).  Later runs
+        use them."""
+        L = lib()
+        L.qs_program_compile_native.argtypes = [c_void_p]
+        _check(L.qs_program_compile_native(self._h))
+        self._native = True
+        return self
+
     def run_timed(self, st):
         """Run once; the milliseconds of each pass, from HIP events recorded
         on the state's stream between the pass launches."""
@@ -866,6 +876,8 @@ class Program:
         return self.info()["passes"]
 
     def kernel_name(self):
+        if getattr(self, "_native", False):
+            return "qs_pass_<i> (specialised tile passes, one kernel per pass)"
         r = "double" if self._dtype == C128 else "float"
         return f"k_tile<{r},{16 if self._reg_bits == 4 else 8}>"
 
@@ -958,6 +970,14 @@ class BitProgram:
 
     def run(self, st):
         _check(lib().qs_program_run(self._h, st.handle))
+
+    def compile_native(self):
+        """Specialised pass kernels for the compiled program (see
+        Program.compile_native)."""
+        L = lib()
+        L.qs_program_compile_native.argtypes = [c_void_p]
+        _check(L.qs_program_compile_native(self._h))
+        return self
 
     def info(self):
         g, p, h = c_uint64(), c_uint64(), c_uint64()

@@ -26,6 +26,7 @@ SOURCES = [
     "qs_serve.hip",
     "qs_io.hip",
     "qs_kraus.hip",
+    "qs_native.hip",
 ]
 
 FLAGS = [
@@ -71,7 +72,7 @@ def build(verbose=False, jobs=None):
             sys.stderr.write(f"--- {s}\n{text}\n")
         raise RuntimeError("HIP compilation failed: " + ", ".join(s for s, _ in failed))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -695,9 +695,12 @@ int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap, hipEvent_t
     if (!st->qubits || st->nbits != P.nb)
         return fail(QS_ERR_DIMS_MISMATCH_CVECTOR, "qpp_amd::fused", "state/program");
     if (ev) QS_HIP(hipEventRecord(ev[0], st->stream), "qpp_amd::fused");
+    const bool native = !P.native_fn.empty() && P.native_dtype == st->dtype;
     for (size_t p = 0; p < P.passes.size(); ++p) {
         int rc;
-        if (st->dtype == QS_C128)
+        if (native)
+            rc = native_launch(P, st, int(p), grid_cap);
+        else if (st->dtype == QS_C128)
             rc = P.reg_bits == 3 ? launch_pass<double, 8>(P, st, int(p), grid_cap)
                                  : launch_pass<double, 16>(P, st, int(p), grid_cap);
         else

@@ -4,6 +4,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -129,6 +130,13 @@ struct FusedProgram {
     FPhase* d_phases = nullptr;
     FOp* d_ops = nullptr;
     void* d_mats = nullptr;
+    // specialised pass kernels (qs_native.hip): hipModule_t per group of
+    // passes and one hipFunction_t per pass, for states of native_dtype
+    std::vector<void*> native_modules;
+    std::vector<void*> native_fn;
+    qs_dtype native_dtype = QS_C128;
+    void* native_tab = nullptr;          // the passes' constants (device)
+    std::vector<uint64_t> native_off;   // each pass's first constant
 };
 
 namespace qs {
@@ -139,6 +147,15 @@ void fused_free(FusedProgram& P);
 // ev: nullptr, or passes + 1 events recorded on the state's stream before
 // the first pass and after each pass (per-pass kernel timing)
 int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap, hipEvent_t* ev = nullptr);
+// specialised pass kernels generated from the program and compiled with
+// hipRTC (qs_native.hip); fused_run uses them once built
+std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double>* tab = nullptr,
+                          std::vector<uint64_t>* off = nullptr,
+                          std::vector<std::string>* pass_src = nullptr,
+                          std::string* preamble = nullptr);
+int native_build(FusedProgram& P, qs_dtype dt);
+void native_free(FusedProgram& P);
+int native_launch(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap);
 // circuit gate steps [b, e) -> bit gates (qs_program.hip)
 int lower_steps(const qs_circuit* qc, uint32_t b, uint32_t e, bool rho,
                 std::vector<BitGate>& out);

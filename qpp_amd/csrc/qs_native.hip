@@ -1,0 +1,520 @@
+// qs_native.hip -- specialised pass kernels for a tile program, generated as
+// HIP source and compiled at run time with hipRTC.
+//
+// k_tile (qs_fused.hip) interprets a pass: for every op it loads the op's
+// head, dispatches on its code and writes all registers of the other of two
+// register arrays.  A specialised kernel has the same structure (persistent
+// workgroups, the next tile loaded while the current one is computed, LDS
+// layout changes between phases) with the pass's ops fixed when the code is
+// generated:
+//  - matrix entries and diagonal factors come from a per-pass table at fixed
+//    offsets, read with scalar loads where each op needs them (the address
+//    is made opaque once per tile, so the compiler cannot hoist every
+//    pass's constants out of the tile loop into registers);
+//  - an X whose control lies in register bits only is a renaming of
+//    registers at generation time and costs no instruction;
+//  - a controlled op computes only the register groups its control admits;
+//  - no dispatch and no copies between register arrays;
+//  - tile-base runs and the global and LDS offsets of every register group
+//    and thread bit are literals.
+// The arithmetic of each op is written as in qs_tile_ops.hpp (the same
+// complex products in the same order), so results agree with k_tile to
+// rounding.  Programs use it on request (qs_program_compile_native); every
+// other path keeps k_tile.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "qs_fused.hpp"
+#include "qs_internal.hpp"
+
+namespace {
+
+using qs::fail;
+using C = std::complex<double>;
+
+constexpr int kNR = 16;  // register groups (reg_bits 4)
+
+struct Gen {
+    std::ostringstream o;
+    bool dbl = true;
+    int perm[kNR];
+    std::vector<double> tab;  // the pass's constants (re, im pairs)
+
+    // a complex constant: its place in the pass's table
+    std::string cx(double re, double im) {
+        const size_t i = tab.size();
+        tab.push_back(re);
+        tab.push_back(im);
+        return "mk(cm[" + std::to_string(i) + "], cm[" + std::to_string(i + 1) + "])";
+    }
+    std::string cx(const C& c) { return cx(c.real(), c.imag()); }
+    std::string X(int rho) const { return "x[" + std::to_string(perm[rho]) + "]"; }
+    void identity() {
+        for (int i = 0; i < kNR; ++i) perm[i] = i;
+    }
+};
+
+std::string u64(uint64_t v) { return std::to_string(v) + "ull"; }
+
+// thread part of a phase's global index (single-bit offsets per thread bit)
+std::string thr_g(const FPhase& ph) {
+    std::string e = "(0ull";
+    for (int j = 0; j < 8; ++j)
+        e += " | ((unsigned long long)((t >> " + std::to_string(j) + ") & 1u) * " + u64(ph.g_thr[j]) + ")";
+    return e + ")";
+}
+
+// thread part of a phase's swizzled LDS index
+std::string thr_s(const FPhase& ph) {
+    std::string e = "(0u";
+    for (int j = 0; j < 8; ++j)
+        e += " ^ (((t >> " + std::to_string(j) + ") & 1u) * " + std::to_string(ph.s_thr[j]) + "u)";
+    return e + ")";
+}
+
+uint64_t greg(const FPhase& ph, int rho) {
+    uint64_t g = 0;
+    for (int k = 0; k < 4; ++k)
+        if ((rho >> k) & 1) g |= ph.g_rb[k];
+    return g;
+}
+
+uint32_t sreg(const FPhase& ph, int rho) {
+    uint32_t s = 0;
+    for (int k = 0; k < 4; ++k)
+        if ((rho >> k) & 1) s ^= ph.s_rb[k];
+    return s;
+}
+
+std::string tile_base(const FPass& ps, const char* tile) {
+    std::string e = "(0ull";
+    for (int i = 0; i < ps.nruns; ++i) {
+        const uint32_t r = ps.runs[i];
+        const uint32_t src = r & 255u, dst = (r >> 8) & 255u, len = r >> 16;
+        const uint64_t mask = len >= 64 ? ~uint64_t{0} : ((uint64_t{1} << len) - 1);
+        e += " | (((" + std::string(tile) + " >> " + std::to_string(src) + ") & " + u64(mask) + ") << " +
+             std::to_string(dst) + ")";
+    }
+    return e + ")";
+}
+
+// y = op(x) for one op, in place on the register array under the current
+// renaming (qs_tile_ops.hpp run_op_h / run_op_ct / do_*)
+void gen_op(Gen& g, const FusedProgram& P, const FOp& op) {
+    std::ostringstream& o = g.o;
+    const uint32_t R = op.regok;
+    const bool ct = op.ctrl != 0;
+    const bool has_t = op.tmask != 0;
+    auto rok = [&](int rho) { return !ct || ((R >> rho) & 1u); };
+    o << "    {\n";
+    if (op.omask) o << "    if ((base & " << u64(op.omask) << ") == " << u64(op.oval) << ") {\n";
+    if (has_t) o << "    const bool tk = (t & " << op.tmask << "u) == " << op.tval << "u;\n";
+    auto put = [&](int rho, const std::string& v) {
+        if (has_t)
+            o << "    " << g.X(rho) << " = sel(tk, " << v << ", " << g.X(rho) << ");\n";
+        else
+            o << "    " << g.X(rho) << " = " << v << ";\n";
+    };
+    auto u1 = [&](int J, const std::string& u00, const std::string& u10, const std::string& u01,
+                  const std::string& u11, bool controlled) {
+        for (int rho = 0; rho < kNR; ++rho) {
+            if ((rho >> J) & 1) continue;
+            if (controlled && !rok(rho)) continue;
+            const int r1 = rho | (1 << J);
+            o << "    { const T a0 = " << g.X(rho) << ", b0 = " << g.X(r1) << ";\n";
+            const std::string y0 = "mac2(" + u00 + ", a0, " + u01 + ", b0)";
+            const std::string y1 = "mac2(" + u10 + ", a0, " + u11 + ", b0)";
+            if (controlled && has_t) {
+                o << "    " << g.X(rho) << " = sel(tk, " << y0 << ", a0);\n";
+                o << "    " << g.X(r1) << " = sel(tk, " << y1 << ", b0); }\n";
+            } else {
+                o << "    " << g.X(rho) << " = " << y0 << ";\n";
+                o << "    " << g.X(r1) << " = " << y1 << "; }\n";
+            }
+        }
+    };
+    switch (op.kind) {
+        case OP_U1: {
+            o << "    const T u00 = " << g.cx(op.m[0], op.m[1]) << ", u10 = " << g.cx(op.m[2], op.m[3])
+              << ", u01 = " << g.cx(op.m[4], op.m[5]) << ", u11 = " << g.cx(op.m[6], op.m[7]) << ";\n";
+            u1(op.j0, "u00", "u10", "u01", "u11", true);
+            break;
+        }
+        case OP_X: {
+            const int J = op.j0;
+            for (int rho = 0; rho < kNR; ++rho) {
+                if ((rho >> J) & 1) continue;
+                if (!rok(rho)) continue;
+                const int r1 = rho | (1 << J);
+                if (has_t) {
+                    o << "    { const T a0 = " << g.X(rho) << ", b0 = " << g.X(r1) << ";\n";
+                    o << "    " << g.X(rho) << " = sel(tk, b0, a0);\n";
+                    o << "    " << g.X(r1) << " = sel(tk, a0, b0); }\n";
+                } else if (op.omask) {
+                    o << "    { const T a0 = " << g.X(rho) << "; " << g.X(rho) << " = " << g.X(r1) << "; "
+                      << g.X(r1) << " = a0; }\n";
+                } else {
+                    std::swap(g.perm[rho], g.perm[r1]);  // a renaming: no code
+                }
+            }
+            break;
+        }
+        case OP_DIAG_R: {
+            const int J = op.j0;
+            o << "    const T d0 = " << g.cx(op.m[0], op.m[1]) << ", d1 = " << g.cx(op.m[2], op.m[3]) << ";\n";
+            for (int rho = 0; rho < kNR; ++rho) {
+                if (!rok(rho)) continue;
+                put(rho, std::string("cmul(") + (((rho >> J) & 1) ? "d1" : "d0") + ", " + g.X(rho) + ")");
+            }
+            break;
+        }
+        case OP_DIAG_T:
+        case OP_DIAG_O: {
+            o << "    const T d0 = " << g.cx(op.m[0], op.m[1]) << ", d1 = " << g.cx(op.m[2], op.m[3]) << ";\n";
+            if (op.kind == OP_DIAG_T)
+                o << "    const T dd = sel((t & " << op.tbit << "u) != 0u, d1, d0);\n";
+            else
+                o << "    const T dd = (base & " << u64(op.dbit) << ") ? d1 : d0;\n";
+            for (int rho = 0; rho < kNR; ++rho) {
+                if (!rok(rho)) continue;
+                put(rho, "cmul(dd, " + g.X(rho) + ")");
+            }
+            break;
+        }
+        case OP_U2: {
+            const int J0 = op.j0, J1 = op.j1;
+            const C* M = P.mats.data() + op.moff;
+            o << "    const T m[16] = {";
+            for (int i = 0; i < 16; ++i) o << (i ? ", " : "") << g.cx(M[i]);
+            o << "};\n";
+            for (int rho = 0; rho < kNR; ++rho) {
+                if (((rho >> J0) & 1) || ((rho >> J1) & 1)) continue;
+                if (!rok(rho)) continue;
+                const int i[4] = {rho, rho | (1 << J0), rho | (1 << J1), rho | (1 << J0) | (1 << J1)};
+                o << "    { const T v0 = " << g.X(i[0]) << ", v1 = " << g.X(i[1]) << ", v2 = " << g.X(i[2])
+                  << ", v3 = " << g.X(i[3]) << ";\n";
+                for (int r = 0; r < 4; ++r) {
+                    o << "    T o" << r << " = mac2(m[" << r << "], v0, m[" << r + 4 << "], v1); { const T t2 = mac2(m["
+                      << r + 8 << "], v2, m[" << r + 12 << "], v3); o" << r << ".x += t2.x; o" << r
+                      << ".y += t2.y; }\n";
+                }
+                for (int r = 0; r < 4; ++r) {
+                    if (has_t)
+                        o << "    " << g.X(i[r]) << " = sel(tk, o" << r << ", v" << r << ");\n";
+                    else
+                        o << "    " << g.X(i[r]) << " = o" << r << ";\n";
+                }
+                o << "    }\n";
+            }
+            break;
+        }
+        case OP_U1B: {
+            const C* M = P.mats.data() + op.moff;
+            for (int k = 0; k < 4; ++k) {
+                o << "    { const T u00 = " << g.cx(M[4 * k]) << ", u10 = " << g.cx(M[4 * k + 1])
+                  << ", u01 = " << g.cx(M[4 * k + 2]) << ", u11 = " << g.cx(M[4 * k + 3]) << ";\n";
+                u1(k, "u00", "u10", "u01", "u11", false);
+                o << "    }\n";
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    if (op.omask) o << "    }\n";
+    o << "    }\n";
+    // each op's constants are loaded next to its arithmetic
+    o << "    __builtin_amdgcn_sched_barrier(0);\n";
+}
+
+void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& sfrom,
+               const std::string& sto) {
+    std::ostringstream& o = g.o;
+    for (int rho = 0; rho < kNR; ++rho)
+        o << "    lds[" << sfrom << " ^ " << sreg(from, rho) << "u] = " << g.X(rho) << ";\n";
+    o << "    __syncthreads();\n";
+    g.identity();
+    for (int rho = 0; rho < kNR; ++rho)
+        o << "    x[" << rho << "] = lds[" << sto << " ^ " << sreg(to, rho) << "u];\n";
+    o << "    __syncthreads();\n";
+}
+
+void gen_pass(Gen& g, const FusedProgram& P, int pi) {
+    std::ostringstream& o = g.o;
+    const FPass& ps = P.passes[pi];
+    const int pb = ps.phase_begin, pe = ps.phase_end;
+    const FPhase& ld = P.phases[pb];
+    g.tab.clear();
+    o << "extern \"C\" __global__ __launch_bounds__(256) void qs_pass_" << pi
+      << "(T* __restrict__ a, unsigned long long ntiles, const R* __restrict__ mtab) {\n";
+    o << "    extern __shared__ unsigned char smem_raw[];\n";
+    o << "    T* lds = reinterpret_cast<T*>(smem_raw);\n";
+    o << "    const unsigned t = threadIdx.x;\n";
+    o << "    const unsigned long long g_ld = " << thr_g(ld) << ";\n";
+    // the LDS thread parts of the phases that take part in a layout change
+    for (int ph = pb; ph < pe; ++ph)
+        o << "    const unsigned s_" << ph - pb << " = " << thr_s(P.phases[ph]) << ";\n";
+    o << "    T x[16], y[16];\n";
+    o << "    unsigned long long tile = blockIdx.x;\n";
+    o << "    if (tile < ntiles) {\n";
+    o << "        const T* p = a + (" << tile_base(ps, "tile") << " | g_ld);\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = p[" << u64(greg(ld, rho)) << "];\n";
+    o << "    }\n";
+    o << "    __builtin_amdgcn_s_waitcnt(0x0F70);\n";
+    o << "    for (; tile < ntiles; tile += gridDim.x) {\n";
+    o << "    const unsigned long long base = " << tile_base(ps, "tile") << ";\n";
+    o << "    int z;\n    asm volatile(\"s_mov_b32 %0, 0\" : \"=s\"(z));\n";
+    o << "    const R* __restrict__ cm = mtab + z;\n";
+    o << "    for (int r = 0; r < 16; ++r) x[r] = y[r];\n";
+    o << "    { const unsigned long long nx = tile + gridDim.x;\n";
+    o << "    if (nx < ntiles) {\n";
+    o << "        const T* p = a + (" << tile_base(ps, "nx") << " | g_ld);\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = p[" << u64(greg(ld, rho)) << "];\n";
+    o << "    } }\n";
+    g.identity();
+    int ci = pb;
+    for (int ph = pb; ph < pe; ++ph) {
+        if (ph != ci) {
+            gen_remap(g, P.phases[ci], P.phases[ph], "s_" + std::to_string(ci - pb),
+                      "s_" + std::to_string(ph - pb));
+            ci = ph;
+        }
+        for (int oi = P.phases[ph].op_begin; oi < P.phases[ph].op_end; ++oi) gen_op(g, P, P.ops[oi]);
+    }
+    const FPhase* cur = &P.phases[ci];
+    if (!cur->coalesced) {
+        gen_remap(g, *cur, ld, "s_" + std::to_string(ci - pb), "s_0");
+        cur = &ld;
+    }
+    if (cur == &ld)
+        o << "    T* q = a + (base | g_ld);\n";
+    else
+        o << "    T* q = a + (base | " << thr_g(*cur) << ");\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "    q[" << u64(greg(*cur, rho)) << "] = " << g.X(rho) << ";\n";
+    o << "    }\n}\n\n";
+}
+
+const char* kPreamble = R"(
+__device__ __forceinline__ T mk(R x, R y) { T r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ T mac2(T u0, T a, T u1, T b) {
+    T r;
+    r.x = u0.x * a.x - u0.y * a.y + u1.x * b.x - u1.y * b.y;
+    r.y = u0.x * a.y + u0.y * a.x + u1.x * b.y + u1.y * b.x;
+    return r;
+}
+__device__ __forceinline__ T cmul(T u, T a) { return mk(u.x * a.x - u.y * a.y, u.x * a.y + u.y * a.x); }
+__device__ __forceinline__ T sel(bool c, T a, T b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
+)";
+
+}  // namespace
+
+namespace qs {
+
+// The program as HIP source: a preamble, then one kernel per pass (pass_src
+// has each pass's kernel alone, for compiling groups of passes apart).
+std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double>* tab,
+                          std::vector<uint64_t>* off, std::vector<std::string>* pass_src,
+                          std::string* preamble) {
+    Gen g;
+    g.dbl = dt == QS_C128;
+    std::string pre = g.dbl ? "typedef double2 T;\ntypedef double R;\n" : "typedef float2 T;\ntypedef float R;\n";
+    pre += kPreamble;
+    pre += "\n";
+    if (preamble) *preamble = pre;
+    if (tab) tab->clear();
+    if (off) off->clear();
+    if (pass_src) pass_src->clear();
+    std::string all = pre;
+    for (size_t p = 0; p < P.passes.size(); ++p) {
+        g.o.str("");
+        gen_pass(g, P, int(p));
+        const std::string k = g.o.str();
+        all += k;
+        if (pass_src) pass_src->push_back(k);
+        if (off) off->push_back(tab ? tab->size() : 0);
+        if (tab) tab->insert(tab->end(), g.tab.begin(), g.tab.end());
+    }
+    return all;
+}
+
+}  // namespace qs
+
+namespace {
+
+// code objects of compiled sources, by the source text (a program compiled
+// again -- the same circuit and options -- is not recompiled)
+std::mutex g_code_mu;
+std::map<std::string, std::vector<char>> g_code;
+
+// compiles one source with hipRTC for gfx950; "" on success, else the log
+std::string rtc_compile(const std::string& src, std::vector<char>& code) {
+    {
+        std::lock_guard<std::mutex> lk(g_code_mu);
+        auto it = g_code.find(src);
+        if (it != g_code.end()) {
+            code = it->second;
+            return "";
+        }
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "qs_native.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return "hiprtcCreateProgram failed";
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    if (hiprtcCompileProgram(prog, 6, opts) != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return log.empty() ? std::string("compile failed") : log.substr(0, 2000);
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    std::lock_guard<std::mutex> lk(g_code_mu);
+    g_code.emplace(src, code);
+    return "";
+}
+
+}  // namespace
+
+namespace qs {
+
+// Generates and compiles the program's pass kernels, in groups of passes
+// compiled in parallel threads; P.native_* hold the modules, one function
+// per pass and the constant table afterwards.
+int native_build(FusedProgram& P, qs_dtype dt) {
+    constexpr const char* W = "qpp_amd::program";
+    if (P.reg_bits != 4) return fail(QS_ERR_NOT_IMPLEMENTED, W, "native passes need reg_bits 4");
+    for (const FPass& ps : P.passes)
+        if (ps.m != 12) return fail(QS_ERR_NOT_IMPLEMENTED, W, "native passes need 2^12-amplitude tiles");
+    if (int rc = ensure_device()) return rc;
+    std::vector<double> tab;
+    std::vector<uint64_t> off;
+    std::vector<std::string> psrc;
+    std::string pre;
+    (void)native_source(P, dt, &tab, &off, &psrc, &pre);
+    // groups of passes of about equal source size, one per thread
+    const size_t np = P.passes.size();
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t ng = std::max<size_t>(1, std::min<size_t>({np, size_t(hw), 16}));
+    std::vector<std::string> gsrc(ng, pre);
+    std::vector<size_t> gsize(ng, 0), group_of(np, 0);
+    std::vector<size_t> order(np);
+    for (size_t p = 0; p < np; ++p) order[p] = p;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return psrc[a].size() > psrc[b].size(); });
+    for (size_t p : order) {
+        const size_t gi = size_t(std::min_element(gsize.begin(), gsize.end()) - gsize.begin());
+        gsrc[gi] += psrc[p];
+        gsize[gi] += psrc[p].size();
+        group_of[p] = gi;
+    }
+    std::vector<std::vector<char>> code(ng);
+    std::vector<std::string> err(ng);
+    {
+        std::vector<std::thread> th;
+        for (size_t gi = 0; gi < ng; ++gi)
+            th.emplace_back([&, gi] { err[gi] = rtc_compile(gsrc[gi], code[gi]); });
+        for (auto& t : th) t.join();
+    }
+    for (size_t gi = 0; gi < ng; ++gi)
+        if (!err[gi].empty())
+            return fail_runtime(QS_ERR_DEVICE, "qpp_amd::program: native pass compile failed: " + err[gi]);
+    std::vector<void*> mods(ng, nullptr);
+    auto unload = [&] {
+        for (void* m : mods)
+            if (m) (void)hipModuleUnload(reinterpret_cast<hipModule_t>(m));
+    };
+    for (size_t gi = 0; gi < ng; ++gi) {
+        hipModule_t mod = nullptr;
+        const hipError_t e = hipModuleLoadData(&mod, code[gi].data());
+        if (e != hipSuccess) {
+            unload();
+            return fail_hip(e, W);
+        }
+        mods[gi] = reinterpret_cast<void*>(mod);
+    }
+    std::vector<void*> fns(np);
+    for (size_t p = 0; p < np; ++p) {
+        const std::string name = "qs_pass_" + std::to_string(p);
+        hipFunction_t f = nullptr;
+        const hipError_t e = hipModuleGetFunction(&f, reinterpret_cast<hipModule_t>(mods[group_of[p]]), name.c_str());
+        if (e != hipSuccess) {
+            unload();
+            return fail_hip(e, W);
+        }
+        fns[p] = reinterpret_cast<void*>(f);
+    }
+    // the constants, in the state's precision
+    const size_t es = dt == QS_C128 ? 8 : 4;
+    std::vector<unsigned char> hb(std::max<size_t>(1, tab.size()) * es);
+    for (size_t i = 0; i < tab.size(); ++i) {
+        if (es == 8) {
+            std::memcpy(hb.data() + 8 * i, &tab[i], 8);
+        } else {
+            const float f = static_cast<float>(tab[i]);
+            std::memcpy(hb.data() + 4 * i, &f, 4);
+        }
+    }
+    void* dtab = nullptr;
+    hipError_t e = hipMalloc(&dtab, hb.size());
+    if (e == hipSuccess) e = hipMemcpy(dtab, hb.data(), hb.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (dtab) (void)hipFree(dtab);
+        unload();
+        return fail_hip(e, W);
+    }
+    native_free(P);
+    P.native_modules = std::move(mods);
+    P.native_fn = std::move(fns);
+    P.native_dtype = dt;
+    P.native_tab = dtab;
+    P.native_off = std::move(off);
+    return QS_OK;
+}
+
+void native_free(FusedProgram& P) {
+    if (!P.native_modules.empty()) {
+        if (P.last) (void)hipStreamSynchronize(P.last);
+        if (P.stream) (void)hipStreamSynchronize(P.stream);
+        for (void* m : P.native_modules) (void)hipModuleUnload(reinterpret_cast<hipModule_t>(m));
+    }
+    if (P.native_tab) (void)hipFree(P.native_tab);
+    P.native_modules.clear();
+    P.native_tab = nullptr;
+    P.native_fn.clear();
+    P.native_off.clear();
+}
+
+// One pass through its specialised kernel.
+int native_launch(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap) {
+    const FPass& ps = P.passes[pass];
+    const size_t es = st->dtype == QS_C128 ? 16 : 8;
+    const size_t shmem = es << ps.m;
+    uint64_t grid = ps.ntiles;
+    if (grid > grid_cap) grid = grid_cap;
+    void* a = st->d;
+    unsigned long long nt = ps.ntiles;
+    void* tab = static_cast<unsigned char*>(P.native_tab) + P.native_off[pass] * (es / 2);
+    void* args[] = {&a, &nt, &tab};
+    QS_HIP(hipModuleLaunchKernel(reinterpret_cast<hipFunction_t>(P.native_fn[pass]),
+                                 static_cast<unsigned>(grid), 1, 1, 256, 1, 1,
+                                 static_cast<unsigned>(shmem), st->stream, args, nullptr),
+           "qpp_amd::fused");
+    return QS_OK;
+}
+
+}  // namespace qs

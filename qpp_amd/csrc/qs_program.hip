@@ -173,6 +173,14 @@ int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates
     return QS_OK;
 }
 
+// The program's passes as specialised kernels (generated source compiled
+// with hipRTC); later runs on states of the program's dtype use them.
+int qs_program_compile_native(qs_program* p) {
+    if (!p) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
+    qs::clear_error();
+    return qs::native_build(p->P, p->P.dtype);
+}
+
 int qs_program_run(qs_program* p, qs_state* st) {
     if (!p || !st || !st->d) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
     return qs::fused_run(p->P, st, p->grid_cap);
@@ -211,6 +219,7 @@ int qs_program_info(const qs_program* p, uint64_t* num_gates, uint64_t* num_pass
 
 int qs_program_destroy(qs_program* p) {
     if (!p) return QS_OK;
+    qs::native_free(p->P);
     qs::fused_free(p->P);
     delete p;
     return QS_OK;
