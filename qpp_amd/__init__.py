@@ -877,7 +877,8 @@ class Program:
 
     def kernel_name(self):
         if getattr(self, "_native", False):
-            return "qs_pass_<i> (specialised tile passes, one kernel per pass)"
+            return ("qs_pass_c128_<i>" if self._dtype == C128 else "qs_pass_c64_<i>") + \
+                " (specialised tile passes, one kernel per pass)"
         r = "double" if self._dtype == C128 else "float"
         return f"k_tile<{r},{16 if self._reg_bits == 4 else 8}>"
 

@@ -255,7 +255,7 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi) {
     const int pb = ps.phase_begin, pe = ps.phase_end;
     const FPhase& ld = P.phases[pb];
     g.tab.clear();
-    o << "extern \"C\" __global__ __launch_bounds__(256) void qs_pass_" << pi
+    o << "extern \"C\" __global__ __launch_bounds__(256) void qs_pass_" << (g.dbl ? "c128_" : "c64_") << pi
       << "(T* __restrict__ a, unsigned long long ntiles, const R* __restrict__ mtab) {\n";
     o << "    extern __shared__ unsigned char smem_raw[];\n";
     o << "    T* lds = reinterpret_cast<T*>(smem_raw);\n";
@@ -449,7 +449,7 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     }
     std::vector<void*> fns(np);
     for (size_t p = 0; p < np; ++p) {
-        const std::string name = "qs_pass_" + std::to_string(p);
+        const std::string name = std::string(dt == QS_C128 ? "qs_pass_c128_" : "qs_pass_c64_") + std::to_string(p);
         hipFunction_t f = nullptr;
         const hipError_t e = hipModuleGetFunction(&f, reinterpret_cast<hipModule_t>(mods[group_of[p]]), name.c_str());
         if (e != hipSuccess) {
