@@ -154,6 +154,9 @@ std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double
                           std::vector<std::string>* pass_src = nullptr,
                           std::string* preamble = nullptr);
 int native_build(FusedProgram& P, qs_dtype dt);
+// the program with runs of ops on one or two register bits of a phase
+// multiplied into one op where that costs fewer FP64 instructions
+FusedProgram native_fuse(const FusedProgram& P);
 void native_free(FusedProgram& P);
 int native_launch(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap);
 // circuit gate steps [b, e) -> bit gates (qs_program.hip)

@@ -17,14 +17,16 @@
 //  - no dispatch and no copies between register arrays;
 //  - tile-base runs and the global and LDS offsets of every register group
 //    and thread bit are literals.
-// The arithmetic of each op is written as in qs_tile_ops.hpp (the same
-// complex products in the same order), so results agree with k_tile to
-// rounding.  Programs use it on request (qs_program_compile_native); every
+// Before generation, the ops of each phase that act on one or two register
+// bits are multiplied into fewer ops where that saves FP64 instructions
+// (native_fuse).  The arithmetic of each op is written as in
+// qs_tile_ops.hpp, so results agree with k_tile to rounding.  Programs use it on request (qs_program_compile_native); every
 // other path keeps k_tile.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -107,6 +109,324 @@ std::string tile_base(const FPass& ps, const char* tile) {
     }
     return e + ")";
 }
+
+// ---------------------------------------------------------------------------
+// Pair fusion: within a phase, the ops that act only on register bits are
+// gathered per bit (a 2x2 product) and per pair of bits (a 4x4 product, once
+// an op joins the two bits: a CNOT or controlled gate between them, or a
+// dense 2-bit gate).  Ops on other bits commute with a pending product and
+// pass it; an op that involves a pending product's bits emits the product
+// first.  A product goes out as one op when that is cheaper in FP64
+// instructions than the ops it holds (U1 8, U2 16, diagonal 4, X 0 per
+// register group), else its ops go out unchanged and in order.  On the
+// random circuit a layer of 1-qubit gates, the CNOTs joining register bits
+// and the next layer become 4x4 blocks: two layers of 8 U1 ops cost 1024
+// instructions per tile-wave, the same as two 4x4 ops cost 512.
+// ---------------------------------------------------------------------------
+using M4 = std::array<C, 16>;  // column-major, index bit 0 <-> lower bit
+using M2 = std::array<C, 4>;
+
+M2 m2_mul(const M2& A, const M2& B) {
+    M2 r{};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) r[i + 2 * j] = A[i] * B[2 * j] + A[i + 2] * B[1 + 2 * j];
+    return r;
+}
+M4 m4_mul(const M4& A, const M4& B) {
+    M4 r{};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            C acc(0, 0);
+            for (int k = 0; k < 4; ++k) acc += A[i + 4 * k] * B[k + 4 * j];
+            r[i + 4 * j] = acc;
+        }
+    return r;
+}
+M4 m4_id() {
+    M4 r{};
+    for (int i = 0; i < 4; ++i) r[i + 4 * i] = C(1, 0);
+    return r;
+}
+// U on index bit `which` (0: the lower bit of the pair) of a 4x4
+M4 m4_on(const M2& U, int which) {
+    M4 r{};
+    for (int rr = 0; rr < 4; ++rr)
+        for (int cc = 0; cc < 4; ++cc) {
+            const int r0 = rr & 1, r1 = rr >> 1, c0 = cc & 1, c1 = cc >> 1;
+            r[rr + 4 * cc] = which == 0 ? (r1 == c1 ? U[r0 + 2 * c0] : C(0, 0))
+                                        : (r0 == c0 ? U[r1 + 2 * c1] : C(0, 0));
+        }
+    return r;
+}
+// U on index bit `which`, applied where the other bit equals v
+M4 m4_ctrl(const M2& U, int which, int v) {
+    M4 r = m4_id();
+    for (int rr = 0; rr < 4; ++rr)
+        for (int cc = 0; cc < 4; ++cc) {
+            const int cr = which == 0 ? rr >> 1 : rr & 1, ccb = which == 0 ? cc >> 1 : cc & 1;
+            if (cr != v || ccb != v) continue;
+            const int tr = which == 0 ? rr & 1 : rr >> 1, tc = which == 0 ? cc & 1 : cc >> 1;
+            r[rr + 4 * cc] = U[tr + 2 * tc];
+        }
+    return r;
+}
+
+constexpr int kCostU1 = 8, kCostU2 = 16, kCostDiag = 4;
+
+struct Pending {
+    bool pair = false;
+    int a = -1, b = -1;       // bits (a < b for a pair)
+    M2 s{};                   // single-bit product
+    M4 m{};                   // pair product
+    bool diag = true;         // single: every op diagonal
+    int cost = 0;             // the held ops' FP64 cost
+    std::vector<FOp> held;    // the held ops, in order
+};
+
+// the 2x2 an uncontrolled 1-bit op applies (U1, DIAG_R, X)
+M2 op_m2(const FOp& o) {
+    if (o.kind == OP_X) return {C(0, 0), C(1, 0), C(1, 0), C(0, 0)};
+    if (o.kind == OP_DIAG_R) return {C(o.m[0], o.m[1]), C(0, 0), C(0, 0), C(o.m[2], o.m[3])};
+    return {C(o.m[0], o.m[1]), C(o.m[2], o.m[3]), C(o.m[4], o.m[5]), C(o.m[6], o.m[7])};
+}
+
+int op_cost(const FOp& o) {
+    switch (o.kind) {
+        case OP_U1: return kCostU1;
+        case OP_DIAG_R: case OP_DIAG_T: case OP_DIAG_O: return kCostDiag;
+        case OP_U2: return kCostU2;
+        case OP_U1B: return 4 * kCostU1;
+        default: return 0;
+    }
+}
+
+// register bits the op's register-group predicate depends on
+uint32_t regok_bits(const FOp& o) {
+    uint32_t m = 0;
+    for (int b = 0; b < 4; ++b)
+        for (int rho = 0; rho < kNR; ++rho)
+            if (((o.regok >> rho) & 1u) != ((o.regok >> (rho ^ (1 << b))) & 1u)) {
+                m |= 1u << b;
+                break;
+            }
+    return m;
+}
+
+uint32_t op_bits(const FOp& o) {
+    uint32_t m = regok_bits(o);
+    switch (o.kind) {
+        case OP_U1: case OP_X: case OP_DIAG_R: m |= 1u << o.j0; break;
+        case OP_U2: m |= (1u << o.j0) | (1u << o.j1); break;
+        case OP_U1B: m |= 15u; break;
+        default: break;
+    }
+    return m;
+}
+
+struct Fuser {
+    std::vector<C>& mats;
+    std::vector<FOp> out;
+    std::vector<Pending> pend;  // disjoint bit sets
+    explicit Fuser(std::vector<C>& m) : mats(m) {}
+
+    int find(int bit) const {
+        for (size_t i = 0; i < pend.size(); ++i)
+            if (pend[i].a == bit || pend[i].b == bit) return int(i);
+        return -1;
+    }
+    void emit(size_t i) {
+        Pending p = std::move(pend[i]);
+        pend.erase(pend.begin() + long(i));
+        FOp o{};
+        o.regok = 0xFFFFu;
+        if (!p.pair) {
+            const int cost = p.diag ? kCostDiag : kCostU1;
+            if (p.held.size() == 1 || cost >= p.cost) {
+                out.insert(out.end(), p.held.begin(), p.held.end());
+                return;
+            }
+            o.kind = p.diag ? OP_DIAG_R : OP_U1;
+            o.j0 = p.a;
+            if (p.diag) {
+                o.m[0] = p.s[0].real(), o.m[1] = p.s[0].imag();
+                o.m[2] = p.s[3].real(), o.m[3] = p.s[3].imag();
+            } else {
+                for (int t = 0; t < 4; ++t) o.m[2 * t] = p.s[t].real(), o.m[2 * t + 1] = p.s[t].imag();
+            }
+            out.push_back(o);
+            return;
+        }
+        if (kCostU2 >= p.cost) {
+            out.insert(out.end(), p.held.begin(), p.held.end());
+            return;
+        }
+        o.kind = OP_U2;
+        o.j0 = p.a;
+        o.j1 = p.b;
+        o.moff = static_cast<uint32_t>(mats.size());
+        mats.insert(mats.end(), p.m.begin(), p.m.end());
+        out.push_back(o);
+    }
+    void flush_bits(uint32_t bits) {
+        for (size_t i = 0; i < pend.size();) {
+            const Pending& p = pend[i];
+            const bool hit = ((bits >> p.a) & 1u) || (p.pair && ((bits >> p.b) & 1u));
+            if (hit)
+                emit(i);
+            else
+                ++i;
+        }
+    }
+    void flush_all() {
+        while (!pend.empty()) emit(0);
+    }
+    // a 4x4 on bits (a < b) that the held op `o` applies
+    void pair_op(int a, int b, const M4& G, const FOp& o) {
+        int ia = find(a), ib = find(b);
+        if (ia >= 0 && ia == ib && pend[ia].pair) {
+            Pending& p = pend[ia];
+            p.m = m4_mul(G, p.m);
+            p.cost += op_cost(o);
+            p.held.push_back(o);
+            return;
+        }
+        // a pair with another bit is emitted; singles of a and b join
+        if (ia >= 0 && pend[ia].pair) {
+            emit(ia);
+        }
+        ib = find(b);
+        if (ib >= 0 && pend[ib].pair) emit(ib);
+        Pending np;
+        np.pair = true;
+        np.a = a;
+        np.b = b;
+        np.m = m4_id();
+        ia = find(a);
+        if (ia >= 0) {
+            np.m = m4_mul(m4_on(pend[ia].s, 0), np.m);
+            np.cost += pend[ia].cost;
+            np.held.insert(np.held.end(), pend[ia].held.begin(), pend[ia].held.end());
+            pend.erase(pend.begin() + ia);
+        }
+        ib = find(b);
+        if (ib >= 0) {
+            np.m = m4_mul(m4_on(pend[ib].s, 1), np.m);
+            np.cost += pend[ib].cost;
+            np.held.insert(np.held.end(), pend[ib].held.begin(), pend[ib].held.end());
+            pend.erase(pend.begin() + ib);
+        }
+        np.m = m4_mul(G, np.m);
+        np.cost += op_cost(o);
+        np.held.push_back(o);
+        pend.push_back(std::move(np));
+    }
+    void add(const FOp& o) {
+        if (o.kind == OP_U1B) {  // a 2x2 on every register bit: four 1-bit ops
+            for (int k = 0; k < 4; ++k) {
+                FOp u{};
+                u.kind = OP_U1;
+                u.j0 = k;
+                u.regok = 0xFFFFu;
+                for (int t = 0; t < 4; ++t) {
+                    u.m[2 * t] = mats[o.moff + 4 * k + t].real();
+                    u.m[2 * t + 1] = mats[o.moff + 4 * k + t].imag();
+                }
+                add(u);
+            }
+            return;
+        }
+        const bool plain = o.tmask == 0 && o.omask == 0;
+        const uint32_t rb = regok_bits(o);
+        const bool one_bit = o.kind == OP_U1 || o.kind == OP_X || o.kind == OP_DIAG_R;
+        if (plain && one_bit && rb == 0 && (o.regok & 0xFFFFu) == 0xFFFFu) {
+            const int j = o.j0;
+            const int i = find(j);
+            if (i < 0 && o.kind == OP_X) {  // a renaming, nothing to gain
+                out.push_back(o);
+                return;
+            }
+            const M2 U = op_m2(o);
+            if (i >= 0 && pend[i].pair) {
+                Pending& p = pend[i];
+                p.m = m4_mul(m4_on(U, j == p.a ? 0 : 1), p.m);
+                p.cost += op_cost(o);
+                p.held.push_back(o);
+            } else if (i >= 0) {
+                Pending& p = pend[i];
+                p.s = m2_mul(U, p.s);
+                p.diag = p.diag && o.kind == OP_DIAG_R;
+                p.cost += op_cost(o);
+                p.held.push_back(o);
+            } else {
+                Pending p;
+                p.a = j;
+                p.s = U;
+                p.diag = o.kind == OP_DIAG_R;
+                p.cost = op_cost(o);
+                p.held.push_back(o);
+                pend.push_back(std::move(p));
+            }
+            return;
+        }
+        // a 1-bit op controlled by exactly one other register bit, or an
+        // uncontrolled dense 2-bit op: a 4x4 on the pair
+        if (plain && one_bit && __builtin_popcount(rb) == 1 && !((rb >> o.j0) & 1u)) {
+            const int c = __builtin_ctz(rb);
+            int v = -1;
+            uint32_t r1 = 0, r0 = 0;
+            for (int rho = 0; rho < kNR; ++rho) ((rho >> c) & 1 ? r1 : r0) |= 1u << rho;
+            if ((o.regok & 0xFFFFu) == r1) v = 1;
+            if ((o.regok & 0xFFFFu) == r0) v = 0;
+            if (v >= 0) {
+                const int a = std::min(c, o.j0), b = std::max(c, o.j0);
+                pair_op(a, b, m4_ctrl(op_m2(o), o.j0 == a ? 0 : 1, v), o);
+                return;
+            }
+        }
+        if (plain && o.kind == OP_U2 && rb == 0 && (o.regok & 0xFFFFu) == 0xFFFFu) {
+            M4 G;
+            for (int t = 0; t < 16; ++t) G[t] = mats[o.moff + t];
+            pair_op(o.j0, o.j1, G, o);
+            return;
+        }
+        flush_bits(op_bits(o));
+        out.push_back(o);
+    }
+};
+
+}  // namespace
+
+namespace qs {
+
+// The program with each phase's ops pair-fused (see Fuser); phases keep
+// their layouts, their op ranges follow the new op list.
+FusedProgram native_fuse(const FusedProgram& P) {
+    if (P.reg_bits != 4) return P;
+    FusedProgram Q;
+    Q.nb = P.nb;
+    Q.reg_bits = P.reg_bits;
+    Q.passes = P.passes;
+    Q.phases = P.phases;
+    Q.mats = P.mats;
+    Q.gates_per_pass = P.gates_per_pass;
+    Q.dtype = P.dtype;
+    for (FPhase& ph : Q.phases) {
+        Fuser f(Q.mats);
+        for (int i = ph.op_begin; i < ph.op_end; ++i) f.add(P.ops[i]);
+        f.flush_all();
+        ph.op_begin = static_cast<int>(Q.ops.size());
+        for (FOp& o : f.out) {
+            o.ctrl = (o.regok & 0xFFFFu) != 0xFFFFu || o.tmask != 0;
+            Q.ops.push_back(o);
+        }
+        ph.op_end = static_cast<int>(Q.ops.size());
+    }
+    return Q;
+}
+
+}  // namespace qs
+
+namespace {
 
 // y = op(x) for one op, in place on the register array under the current
 // renaming (qs_tile_ops.hpp run_op_h / run_op_ct / do_*)
@@ -406,7 +726,7 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     std::vector<uint64_t> off;
     std::vector<std::string> psrc;
     std::string pre;
-    (void)native_source(P, dt, &tab, &off, &psrc, &pre);
+    (void)native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre);
     // groups of passes of about equal source size, one per thread
     const size_t np = P.passes.size();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
