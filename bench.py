@@ -730,7 +730,8 @@ def bench_circuit(args, qa, n, depth, primary):
                  for lb in (3, 4, 5) for mo in (48, 64, 96) for gc in (16384, 32768, 65536)]
         tt = time.perf_counter()
         tuned = autotune.tune(n, gates, candidates=cands, time_budget_s=90,
-                              prefix_gates=12 * (n + n // 2))
+                              prefix_gates=12 * (n + n // 2), native=bool(args.native),
+                              verbose=True)
         log(f"autotune: {tuned} in {time.perf_counter() - tt:.1f} s")
         args.reg_bits, args.low_bits = tuned["reg_bits"], tuned["low_bits"]
         args.max_ops, args.grid_cap = tuned["max_ops"], tuned["grid_cap"]

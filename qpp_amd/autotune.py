@@ -50,13 +50,17 @@ def _key(n, dtype, kind, gates):
 
 
 def tune(n, gates, dtype=None, kind=None, prefix_gates=None, candidates=None,
-         time_budget_s=60.0, path=None, verbose=False):
+         time_budget_s=60.0, path=None, verbose=False, native=False, native_top=3):
     """Fastest qs_fuse_opts for this circuit shape (dict usable as Program
-    keyword arguments).  `gates`: a workloads gate list."""
+    keyword arguments).  `gates`: a workloads gate list.  native: the
+    program will run as specialised pass kernels (Program.compile_native),
+    so the `native_top` fastest distinct reg_bits-4 candidates on the
+    general kernel are timed again as specialised kernels (seconds of
+    compilation each) and the fastest of those is kept."""
     import qpp_amd as qa
     dtype = qa.C128 if dtype is None else dtype
     kind = qa.KET if kind is None else kind
-    key = _key(n, dtype, kind, gates)
+    key = _key(n, dtype, kind, gates) + (",native" if native else "")
     if path and not _CACHE and os.path.exists(path):
         with open(path) as f:
             _CACHE.update(json.load(f))
@@ -81,6 +85,31 @@ def tune(n, gates, dtype=None, kind=None, prefix_gates=None, candidates=None,
             print(f"autotune {key} {opt}: {ms:.3f} ms ({p.num_passes()} passes)")
         if ms < best_t:
             best, best_t = opt, ms
+    if native and tried:
+        tried.sort(key=lambda x: x[0])
+        picks, seen = [], set()
+        for _, opt in tried:
+            k = (opt["low_bits"], opt["max_ops"])
+            if opt["reg_bits"] != 4 or k in seen:
+                continue
+            seen.add(k)
+            picks.append(opt)
+            if len(picks) == native_top:
+                break
+        nbest, nbest_t = None, float("inf")
+        for opt in picks:
+            try:
+                p = qa.Program(n, sub, kind=kind, dtype=dtype, **opt).compile_native()
+            except qa.QppError:
+                continue
+            p.run(st)
+            ms = _events_ms(qa, lambda: p.run(st))
+            if verbose:
+                print(f"autotune {key} native {opt}: {ms:.3f} ms ({p.num_passes()} passes)")
+            if ms < nbest_t:
+                nbest, nbest_t = opt, ms
+        if nbest is not None:
+            best, best_t = nbest, nbest_t
     if best is None:
         best = dict(reg_bits=4, low_bits=5, max_ops=48, grid_cap=2048)
     _CACHE[key] = {"opts": best, "ms": best_t, "gates": len(sub),
