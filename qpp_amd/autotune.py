@@ -16,6 +16,7 @@ file.
 """
 import json
 import os
+import sys
 import time
 
 import numpy as np
@@ -82,7 +83,7 @@ def tune(n, gates, dtype=None, kind=None, prefix_gates=None, candidates=None,
         ms = _events_ms(qa, lambda: p.run(st))
         tried.append((ms, opt))
         if verbose:
-            print(f"autotune {key} {opt}: {ms:.3f} ms ({p.num_passes()} passes)")
+            print(f"autotune {key} {opt}: {ms:.3f} ms ({p.num_passes()} passes)", file=sys.stderr)
         if ms < best_t:
             best, best_t = opt, ms
     if native and tried:
@@ -105,7 +106,8 @@ def tune(n, gates, dtype=None, kind=None, prefix_gates=None, candidates=None,
             p.run(st)
             ms = _events_ms(qa, lambda: p.run(st))
             if verbose:
-                print(f"autotune {key} native {opt}: {ms:.3f} ms ({p.num_passes()} passes)")
+                print(f"autotune {key} native {opt}: {ms:.3f} ms ({p.num_passes()} passes)",
+                      file=sys.stderr)
             if ms < nbest_t:
                 nbest, nbest_t = opt, ms
         if nbest is not None:
@@ -140,7 +142,7 @@ def tune_batch(circuits, dtype=None, circuit_of=None, seeds=None, noise_p=1e-3,
         b.run(circuit_of, seeds, noise_p)
         ms = b.info(0)["device_ms"]
         if verbose:
-            print(f"autotune batch {opt}: {ms:.2f} ms")
+            print(f"autotune batch {opt}: {ms:.2f} ms", file=sys.stderr)
         if ms < best_t:
             best, best_t = opt, ms
     return dict(best or dict(reg_bits=4, low_bits=5, max_ops=48))
