@@ -243,7 +243,7 @@ uint64_t improve_set(const std::vector<uint64_t>& need, const std::vector<uint64
 // which are global bits 0..lc-1 (lc <= low_bits), so every 2^lc lanes access
 // 16 * 2^lc contiguous bytes.
 FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
-                  const std::vector<uint32_t>& Sv, uint32_t lc) {
+                  const std::vector<uint32_t>& Sv, uint32_t lc, const FPhase* hint) {
     FPhase ph{};
     std::vector<uint32_t> Rv(Rset.begin(), Rset.end());
     ph.coalesced = 1;
@@ -275,7 +275,11 @@ FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
     // slots of a 256-byte LDS row (no bank conflicts); the lowest-first
     // choice keeps tile bits 0..1 on lanes 0..1 whenever they are free,
     // which direct HBM access needs.  The other tile bits follow in
-    // ascending order.
+    // ascending order, except that a tile bit which sits at thread bit j >= 4
+    // of the neighbouring phase `hint` and is a thread bit here too keeps
+    // position j: a layout change between the two phases can then move the
+    // tile in two halves split on that bit (whole waves or lanes skipping a
+    // half), through half the LDS.
     std::vector<uint32_t> rest;
     for (uint32_t b = 0; b < m; ++b)
         if (!Rset.count(b)) rest.push_back(b);
@@ -286,7 +290,22 @@ FPhase make_phase(const std::set<uint32_t>& Rset, uint32_t m, uint32_t RB,
         order.push_back(*it);
         rest.erase(it);
     }
-    order.insert(order.end(), rest.begin(), rest.end());
+    const size_t nt = order.size() + rest.size();
+    std::vector<int> fixed(nt, -1);
+    if (hint)
+        for (size_t j = std::max<size_t>(4, order.size()); j < nt && j < 8; ++j) {
+            auto it = std::find(rest.begin(), rest.end(), uint32_t(hint->tbits[j]));
+            if (it == rest.end()) continue;
+            fixed[j] = int(*it);
+            rest.erase(it);
+        }
+    for (size_t j = order.size(), r = 0; j < nt; ++j) {
+        if (fixed[j] >= 0) {
+            order.push_back(uint32_t(fixed[j]));
+        } else {
+            order.push_back(rest[r++]);
+        }
+    }
     for (uint32_t j = 0; j < order.size(); ++j) {
         ph.tbits[j] = static_cast<uint8_t>(order[j]);
         ph.g_thr[j] = uint64_t{1} << Sv[order[j]];
@@ -426,6 +445,11 @@ void bundle_phase(FusedProgram& P, FPhase& ph, uint32_t RB, std::vector<C>& mats
 }
 
 }  // namespace
+
+FPhase layout_phase(const std::set<uint32_t>& R, uint32_t m, uint32_t RB,
+                    const std::vector<uint32_t>& Sv, uint32_t lc, const FPhase* hint) {
+    return make_phase(R, m, RB, Sv, lc, hint);
+}
 
 int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOptions& opt,
                   FusedProgram& P) {
@@ -578,11 +602,12 @@ int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOpt
                 std::set<uint32_t> Rg = R;
                 ptake = greedy(pit, ptodo, Rg, RB, SIZE_MAX);
             }
-            FPhase ph = make_phase(R, m, RB, Sv, lc);
+            const FPhase* prev = first ? nullptr : &P.phases.back();
+            FPhase ph = make_phase(R, m, RB, Sv, lc, prev);
             if (first && !ph.coalesced) {
                 std::set<uint32_t> R0;
                 for (uint32_t k = 0; k < RB; ++k) R0.insert(m - RB + k);
-                FPhase ldp = make_phase(R0, m, RB, Sv, lc);
+                FPhase ldp = make_phase(R0, m, RB, Sv, lc, &ph);
                 ldp.op_begin = ldp.op_end = static_cast<int>(P.ops.size());
                 P.phases.push_back(ldp);
             }

@@ -4,6 +4,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -137,12 +138,18 @@ struct FusedProgram {
     qs_dtype native_dtype = QS_C128;
     void* native_tab = nullptr;          // the passes' constants (device)
     std::vector<uint64_t> native_off;   // each pass's first constant
+    std::vector<uint32_t> native_lds;   // each pass kernel's LDS bytes per workgroup
 };
 
 namespace qs {
 int fused_compile(const std::vector<BitGate>& gates, uint32_t nb, const FusedOptions& opt,
                   FusedProgram& P);
 int fused_upload(FusedProgram& P, qs_dtype dtype);
+// The layout of a phase with register tile bits R (tile bits 0..m-1 at
+// global bits Sv; lanes 0..lc-1 on tile bits 0..lc-1 when R allows); thread
+// bits j >= 4 that are thread bits of `hint` too keep their position there.
+FPhase layout_phase(const std::set<uint32_t>& R, uint32_t m, uint32_t RB,
+                    const std::vector<uint32_t>& Sv, uint32_t lc, const FPhase* hint);
 void fused_free(FusedProgram& P);
 // ev: nullptr, or passes + 1 events recorded on the state's stream before
 // the first pass and after each pass (per-pass kernel timing)
@@ -152,7 +159,8 @@ int fused_run(const FusedProgram& P, qs_state* st, uint32_t grid_cap, hipEvent_t
 std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double>* tab = nullptr,
                           std::vector<uint64_t>* off = nullptr,
                           std::vector<std::string>* pass_src = nullptr,
-                          std::string* preamble = nullptr);
+                          std::string* preamble = nullptr,
+                          std::vector<uint32_t>* lds = nullptr);
 int native_build(FusedProgram& P, qs_dtype dt);
 // the program with runs of ops on one or two register bits of a phase
 // multiplied into one op where that costs fewer FP64 instructions

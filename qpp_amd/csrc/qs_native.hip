@@ -3,10 +3,9 @@
 //
 // k_tile (qs_fused.hip) interprets a pass: for every op it loads the op's
 // head, dispatches on its code and writes all registers of the other of two
-// register arrays.  A specialised kernel has the same structure (persistent
-// workgroups, the next tile loaded while the current one is computed, LDS
-// layout changes between phases) with the pass's ops fixed when the code is
-// generated:
+// register arrays.  A specialised kernel runs the same phases and LDS
+// layout changes, one workgroup per tile, with the pass's ops fixed when the
+// code is generated:
 //  - matrix entries and diagonal factors come from a per-pass table at fixed
 //    offsets, read with scalar loads where each op needs them (the address
 //    is made opaque once per tile, so the compiler cannot hoist every
@@ -16,7 +15,11 @@
 //  - a controlled op computes only the register groups its control admits;
 //  - no dispatch and no copies between register arrays;
 //  - tile-base runs and the global and LDS offsets of every register group
-//    and thread bit are literals.
+//    and thread bit are literals;
+//  - a layout change whose two layouts share a tile bit (a thread bit at the
+//    same position or a register bit) moves the tile in two halves through
+//    half the LDS, so complex128 passes run 4 workgroups per CU instead of 2;
+//  - the pass stores from a layout chosen next to its last phase.
 // Before generation, the ops of each phase that act on one or two register
 // bits are multiplied into fewer ops where that saves FP64 instructions
 // (native_fuse).  The arithmetic of each op is written as in
@@ -523,11 +526,9 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op) {
                 const int i[4] = {rho, rho | (1 << J0), rho | (1 << J1), rho | (1 << J0) | (1 << J1)};
                 o << "    { const T v0 = " << g.X(i[0]) << ", v1 = " << g.X(i[1]) << ", v2 = " << g.X(i[2])
                   << ", v3 = " << g.X(i[3]) << ";\n";
-                for (int r = 0; r < 4; ++r) {
-                    o << "    T o" << r << " = mac2(m[" << r << "], v0, m[" << r + 4 << "], v1); { const T t2 = mac2(m["
-                      << r + 8 << "], v2, m[" << r + 12 << "], v3); o" << r << ".x += t2.x; o" << r
-                      << ".y += t2.y; }\n";
-                }
+                for (int r = 0; r < 4; ++r)
+                    o << "    T o" << r << " = mac4(m[" << r << "], v0, m[" << r + 4 << "], v1, m[" << r + 8
+                      << "], v2, m[" << r + 12 << "], v3);\n";
                 for (int r = 0; r < 4; ++r) {
                     if (has_t)
                         o << "    " << g.X(i[r]) << " = sel(tk, o" << r << ", v" << r << ");\n";
@@ -557,25 +558,163 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op) {
     o << "    __builtin_amdgcn_sched_barrier(0);\n";
 }
 
-void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& sfrom,
-               const std::string& sto) {
-    std::ostringstream& o = g.o;
-    for (int rho = 0; rho < kNR; ++rho)
-        o << "    lds[" << sfrom << " ^ " << sreg(from, rho) << "u] = " << g.X(rho) << ";\n";
-    o << "    __syncthreads();\n";
-    g.identity();
-    for (int rho = 0; rho < kNR; ++rho)
-        o << "    x[" << rho << "] = lds[" << sto << " ^ " << sreg(to, rho) << "u];\n";
-    o << "    __syncthreads();\n";
+// How a layout change moves the tile through LDS.  Whole: every thread
+// writes its 16 registers, then reads 16 (the tile's 2^12 amplitudes in LDS
+// at once).  Split on a tile bit h >= 4: the amplitudes with bit h = 0 go
+// first, then those with h = 1, each half through the same 2^11 slots (the
+// swizzled index with bit h removed; the swizzle leaves bits >= 4 alone, so
+// this is a bijection on each half and keeps the bank pattern).  h is
+//  - a thread bit at the same position j >= 4 in both layouts: the threads
+//    with bit j = k move all their registers in round k (whole waves for
+//    j >= 6, half the lanes for j = 4, 5), or
+//  - a register bit of both layouts: every thread moves the 8 register
+//    groups with h = k in round k, and the groups read land in the
+//    registers just written (a renaming).
+struct Split {
+    int kind = 0;  // 0 whole, 1 thread bit, 2 register bit
+    int h = 0;     // the tile bit
+    int j = 0;     // thread bit (kind 1)
+    int kf = 0, kt = 0;  // register bit of h in the old and new layout (kind 2)
+};
+
+Split find_split(const FPhase& A, const FPhase& B) {
+    Split r;
+    for (int j = 7; j >= 6; --j)
+        if (A.tbits[j] == B.tbits[j] && A.tbits[j] >= 4) {
+            r.kind = 1;
+            r.h = A.tbits[j];
+            r.j = j;
+            return r;
+        }
+    for (int kf = 3; kf >= 0; --kf)
+        for (int kt = 0; kt < 4; ++kt)
+            if (A.regbits[kf] == B.regbits[kt] && A.regbits[kf] >= 4) {
+                r.kind = 2;
+                r.h = A.regbits[kf];
+                r.kf = kf;
+                r.kt = kt;
+                return r;
+            }
+    for (int j = 5; j >= 4; --j)
+        if (A.tbits[j] == B.tbits[j] && A.tbits[j] >= 4) {
+            r.kind = 1;
+            r.h = A.tbits[j];
+            r.j = j;
+            return r;
+        }
+    return r;
 }
 
-void gen_pass(Gen& g, const FusedProgram& P, int pi) {
+// a swizzled LDS index with bit h removed
+uint32_t drop_bit(uint32_t s, int h) { return ((s >> (h + 1)) << h) | (s & ((1u << h) - 1)); }
+std::string drop_bit(const std::string& s, int h) {
+    return "(((" + s + " >> " + std::to_string(h + 1) + ") << " + std::to_string(h) + ") | (" + s + " & " +
+           std::to_string((1u << h) - 1) + "u))";
+}
+
+void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& sfrom,
+               const std::string& sto, const Split& sp) {
+    std::ostringstream& o = g.o;
+    if (sp.kind == 0) {
+        for (int rho = 0; rho < kNR; ++rho)
+            o << "    lds[" << sfrom << " ^ " << sreg(from, rho) << "u] = " << g.X(rho) << ";\n";
+        o << "    __syncthreads();\n";
+        g.identity();
+        for (int rho = 0; rho < kNR; ++rho)
+            o << "    x[" << rho << "] = lds[" << sto << " ^ " << sreg(to, rho) << "u];\n";
+        o << "    __syncthreads();\n";
+        return;
+    }
+    o << "    { const unsigned cf = " << drop_bit(sfrom, sp.h) << ", ct = " << drop_bit(sto, sp.h) << ";\n";
+    if (sp.kind == 1) {
+        const int old[kNR] = {g.perm[0], g.perm[1], g.perm[2],  g.perm[3],  g.perm[4],  g.perm[5],
+                              g.perm[6], g.perm[7], g.perm[8],  g.perm[9],  g.perm[10], g.perm[11],
+                              g.perm[12], g.perm[13], g.perm[14], g.perm[15]};
+        // a wave bit: the branch is uniform per wave (a scalar branch)
+        if (sp.j >= 6)
+            o << "    const unsigned hb = __builtin_amdgcn_readfirstlane((t >> " << sp.j << ") & 1u);\n";
+        else
+            o << "    const unsigned hb = (t >> " << sp.j << ") & 1u;\n";
+        for (int k = 0; k < 2; ++k) {
+            o << "    if (hb == " << k << "u) {\n";
+            for (int rho = 0; rho < kNR; ++rho)
+                o << "    lds[cf ^ " << drop_bit(sreg(from, rho), sp.h) << "u] = x[" << old[rho] << "];\n";
+            o << "    }\n    __syncthreads();\n    if (hb == " << k << "u) {\n";
+            for (int rho = 0; rho < kNR; ++rho)
+                o << "    x[" << rho << "] = lds[ct ^ " << drop_bit(sreg(to, rho), sp.h) << "u];\n";
+            o << "    }\n    __syncthreads();\n";
+        }
+        g.identity();
+    } else {
+        int np[kNR];
+        for (int k = 0; k < 2; ++k) {
+            std::vector<int> freed;
+            for (int rho = 0; rho < kNR; ++rho)
+                if (((rho >> sp.kf) & 1) == k) {
+                    o << "    lds[cf ^ " << drop_bit(sreg(from, rho), sp.h) << "u] = " << g.X(rho) << ";\n";
+                    freed.push_back(g.perm[rho]);
+                }
+            o << "    __syncthreads();\n";
+            size_t f = 0;
+            for (int rho = 0; rho < kNR; ++rho)
+                if (((rho >> sp.kt) & 1) == k) {
+                    np[rho] = freed[f++];
+                    o << "    x[" << np[rho] << "] = lds[ct ^ " << drop_bit(sreg(to, rho), sp.h) << "u];\n";
+                }
+            o << "    __syncthreads();\n";
+        }
+        for (int rho = 0; rho < kNR; ++rho) g.perm[rho] = np[rho];
+    }
+    o << "    }\n";
+}
+
+// The layout a pass stores from when its last phase cannot store directly:
+// the last phase's register bits above tile bit 1, completed with the
+// highest tile bits that are not its wave bits, so lanes 0 and 1 sit on tile
+// bits 0 and 1 (HBM runs of 64 B and more) and the change to it splits.
+FPhase store_layout(const FusedProgram& P, const FPass& ps, const FPhase& last) {
+    const uint32_t m = uint32_t(ps.m);
+    std::vector<uint32_t> Sv(m);
+    for (uint32_t i = 0; i < m; ++i) Sv[i] = ps.S[i];
+    std::set<uint32_t> R;
+    for (uint32_t k = 0; k < P.reg_bits; ++k)
+        if (last.regbits[k] >= 2) R.insert(last.regbits[k]);
+    for (int b = int(m) - 1; R.size() < P.reg_bits && b >= 2; --b)
+        if (uint32_t(b) != last.tbits[6] && uint32_t(b) != last.tbits[7]) R.insert(uint32_t(b));
+    for (int b = int(m) - 1; R.size() < P.reg_bits && b >= 2; --b) R.insert(uint32_t(b));
+    return qs::layout_phase(R, m, P.reg_bits, Sv, 2, &last);
+}
+
+// Kernel of one pass.  Each workgroup loads a tile (the pass's first
+// layout), runs the phases with their layout changes and stores the tile;
+// the launch gives every tile its own workgroup (a loop over tiles would let
+// the compiler hoist every change's LDS addresses out of it, into spills).
+// When every layout change of a complex128 pass splits, the kernel needs
+// 32 KiB of LDS instead of 64 and runs 4 workgroups per CU (at most 128
+// VGPRs: 4 waves per SIMD).
+void gen_pass(Gen& g, const FusedProgram& P, int pi, uint32_t* lds_bytes) {
     std::ostringstream& o = g.o;
     const FPass& ps = P.passes[pi];
     const int pb = ps.phase_begin, pe = ps.phase_end;
     const FPhase& ld = P.phases[pb];
     g.tab.clear();
-    o << "extern \"C\" __global__ __launch_bounds__(256) void qs_pass_" << (g.dbl ? "c128_" : "c64_") << pi
+    // the layout changes, in order, and how each moves the tile
+    const FPhase& lastp = P.phases[pe - 1];
+    const bool direct_store = lastp.coalesced != 0;
+    const FPhase st = direct_store ? lastp : store_layout(P, ps, lastp);
+    std::vector<Split> sp;
+    for (int ph = pb + 1; ph < pe; ++ph) sp.push_back(find_split(P.phases[ph - 1], P.phases[ph]));
+    if (!direct_store) sp.push_back(find_split(lastp, st));
+    bool half = g.dbl;
+    for (const Split& x : sp)
+        if (x.kind == 0) half = false;
+    if (!half)
+        for (Split& x : sp) x.kind = 0;
+    const size_t es = g.dbl ? 16 : 8;
+    *lds_bytes = uint32_t((half ? es << (ps.m - 1) : es << ps.m));
+    o << "extern \"C\" __global__ __launch_bounds__(256) ";
+    if (half) o << "__attribute__((amdgpu_waves_per_eu(4))) ";
+    o << "void qs_pass_" << (g.dbl ? "c128_" : "c64_") << pi
       << "(T* __restrict__ a, unsigned long long ntiles, const R* __restrict__ mtab) {\n";
     o << "    extern __shared__ unsigned char smem_raw[];\n";
     o << "    T* lds = reinterpret_cast<T*>(smem_raw);\n";
@@ -584,43 +723,29 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi) {
     // the LDS thread parts of the phases that take part in a layout change
     for (int ph = pb; ph < pe; ++ph)
         o << "    const unsigned s_" << ph - pb << " = " << thr_s(P.phases[ph]) << ";\n";
-    o << "    T x[16], y[16];\n";
-    o << "    unsigned long long tile = blockIdx.x;\n";
-    o << "    if (tile < ntiles) {\n";
-    o << "        const T* p = a + (" << tile_base(ps, "tile") << " | g_ld);\n";
-    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = ldnt(p + " << u64(greg(ld, rho)) << ");\n";
-    o << "    }\n";
-    o << "    __builtin_amdgcn_s_waitcnt(0x0F70);\n";
-    o << "    for (; tile < ntiles; tile += gridDim.x) {\n";
+    if (!direct_store) o << "    const unsigned s_st = " << thr_s(st) << ";\n";
+    o << "    T x[16];\n";
+    o << "    const unsigned long long tile = blockIdx.x;\n";
+    o << "    if (tile >= ntiles) return;\n";
+    o << "    {\n";
     o << "    const unsigned long long base = " << tile_base(ps, "tile") << ";\n";
+    o << "    { const T* p = a + (base | g_ld);\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "    x[" << rho << "] = ldnt(p + " << u64(greg(ld, rho)) << ");\n";
+    o << "    }\n";
     o << "    int z;\n    asm volatile(\"s_mov_b32 %0, 0\" : \"=s\"(z));\n";
     o << "    const R* __restrict__ cm = mtab + z;\n";
-    o << "    for (int r = 0; r < 16; ++r) x[r] = y[r];\n";
-    o << "    { const unsigned long long nx = tile + gridDim.x;\n";
-    o << "    if (nx < ntiles) {\n";
-    o << "        const T* p = a + (" << tile_base(ps, "nx") << " | g_ld);\n";
-    for (int rho = 0; rho < kNR; ++rho) o << "        y[" << rho << "] = ldnt(p + " << u64(greg(ld, rho)) << ");\n";
-    o << "    } }\n";
     g.identity();
-    int ci = pb;
+    size_t si = 0;
     for (int ph = pb; ph < pe; ++ph) {
-        if (ph != ci) {
-            gen_remap(g, P.phases[ci], P.phases[ph], "s_" + std::to_string(ci - pb),
-                      "s_" + std::to_string(ph - pb));
-            ci = ph;
+        if (ph != pb) {
+            gen_remap(g, P.phases[ph - 1], P.phases[ph], "s_" + std::to_string(ph - 1 - pb),
+                      "s_" + std::to_string(ph - pb), sp[si++]);
         }
         for (int oi = P.phases[ph].op_begin; oi < P.phases[ph].op_end; ++oi) gen_op(g, P, P.ops[oi]);
     }
-    const FPhase* cur = &P.phases[ci];
-    if (!cur->coalesced) {
-        gen_remap(g, *cur, ld, "s_" + std::to_string(ci - pb), "s_0");
-        cur = &ld;
-    }
-    if (cur == &ld)
-        o << "    T* q = a + (base | g_ld);\n";
-    else
-        o << "    T* q = a + (base | " << thr_g(*cur) << ");\n";
-    for (int rho = 0; rho < kNR; ++rho) o << "    stnt(q + " << u64(greg(*cur, rho)) << ", " << g.X(rho) << ");\n";
+    if (!direct_store) gen_remap(g, lastp, st, "s_" + std::to_string(pe - 1 - pb), "s_st", sp[si++]);
+    o << "    T* q = a + (base | " << thr_g(st) << ");\n";
+    for (int rho = 0; rho < kNR; ++rho) o << "    stnt(q + " << u64(greg(st, rho)) << ", " << g.X(rho) << ");\n";
     o << "    }\n}\n\n";
 }
 
@@ -645,6 +770,13 @@ __device__ __forceinline__ T mac2(T u0, T a, T u1, T b) {
     r.y = u0.x * a.y + u0.y * a.x + u1.x * b.y + u1.y * b.x;
     return r;
 }
+// one chain per real part: 1 multiply and 7 FMA
+__device__ __forceinline__ T mac4(T u0, T a, T u1, T b, T u2, T c, T u3, T d) {
+    T r;
+    r.x = u0.x * a.x - u0.y * a.y + u1.x * b.x - u1.y * b.y + u2.x * c.x - u2.y * c.y + u3.x * d.x - u3.y * d.y;
+    r.y = u0.x * a.y + u0.y * a.x + u1.x * b.y + u1.y * b.x + u2.x * c.y + u2.y * c.x + u3.x * d.y + u3.y * d.x;
+    return r;
+}
 __device__ __forceinline__ T cmul(T u, T a) { return mk(u.x * a.x - u.y * a.y, u.x * a.y + u.y * a.x); }
 __device__ __forceinline__ T sel(bool c, T a, T b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
 )";
@@ -657,7 +789,7 @@ namespace qs {
 // has each pass's kernel alone, for compiling groups of passes apart).
 std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double>* tab,
                           std::vector<uint64_t>* off, std::vector<std::string>* pass_src,
-                          std::string* preamble) {
+                          std::string* preamble, std::vector<uint32_t>* lds) {
     Gen g;
     g.dbl = dt == QS_C128;
     std::string pre = g.dbl ? "typedef double2 T;\ntypedef double R;\n" : "typedef float2 T;\ntypedef float R;\n";
@@ -667,10 +799,13 @@ std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double
     if (tab) tab->clear();
     if (off) off->clear();
     if (pass_src) pass_src->clear();
+    if (lds) lds->clear();
     std::string all = pre;
     for (size_t p = 0; p < P.passes.size(); ++p) {
         g.o.str("");
-        gen_pass(g, P, int(p));
+        uint32_t lb = 0;
+        gen_pass(g, P, int(p), &lb);
+        if (lds) lds->push_back(lb);
         const std::string k = g.o.str();
         all += k;
         if (pass_src) pass_src->push_back(k);
@@ -739,7 +874,8 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     std::vector<uint64_t> off;
     std::vector<std::string> psrc;
     std::string pre;
-    (void)native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre);
+    std::vector<uint32_t> lds;
+    (void)native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre, &lds);
     // groups of passes of about equal source size, one per thread
     const size_t np = P.passes.size();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -816,6 +952,7 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     P.native_dtype = dt;
     P.native_tab = dtab;
     P.native_off = std::move(off);
+    P.native_lds = std::move(lds);
     return QS_OK;
 }
 
@@ -830,15 +967,20 @@ void native_free(FusedProgram& P) {
     P.native_tab = nullptr;
     P.native_fn.clear();
     P.native_off.clear();
+    P.native_lds.clear();
 }
 
-// One pass through its specialised kernel.
+// One pass through its specialised kernel: a workgroup per tile (grid_cap is
+// for k_tile's persistent launch; here a finished workgroup's CU slot takes
+// the next tile's workgroup, which measured faster than persistent loops).
 int native_launch(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_cap) {
+    (void)grid_cap;
     const FPass& ps = P.passes[pass];
     const size_t es = st->dtype == QS_C128 ? 16 : 8;
-    const size_t shmem = es << ps.m;
-    uint64_t grid = ps.ntiles;
-    if (grid > grid_cap) grid = grid_cap;
+    const size_t shmem = size_t(pass) < P.native_lds.size() ? P.native_lds[pass] : es << ps.m;
+    if (ps.ntiles > 0xffffffffull)
+        return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::fused", "more than 2^32 tiles in one pass");
+    const uint64_t grid = ps.ntiles;
     void* a = st->d;
     unsigned long long nt = ps.ntiles;
     void* tab = static_cast<unsigned char*>(P.native_tab) + P.native_off[pass] * (es / 2);
