@@ -79,11 +79,11 @@ std::string thr_g(const FPhase& ph) {
     return e + ")";
 }
 
-// thread part of a phase's swizzled LDS index
-std::string thr_s(const FPhase& ph) {
+// thread part of a phase's swizzled LDS index, in bytes (es bytes a slot)
+std::string thr_s(const FPhase& ph, uint32_t es) {
     std::string e = "(0u";
     for (int j = 0; j < 8; ++j)
-        e += " ^ (((t >> " + std::to_string(j) + ") & 1u) * " + std::to_string(ph.s_thr[j]) + "u)";
+        e += " ^ (((t >> " + std::to_string(j) + ") & 1u) * " + std::to_string(ph.s_thr[j] * es) + "u)";
     return e + ")";
 }
 
@@ -605,27 +605,43 @@ Split find_split(const FPhase& A, const FPhase& B) {
     return r;
 }
 
-// a swizzled LDS index with bit h removed
+// a swizzled LDS index with bit h removed (as a slot index; the run-time
+// form on a byte index, bit h at byte bit h + sh)
 uint32_t drop_bit(uint32_t s, int h) { return ((s >> (h + 1)) << h) | (s & ((1u << h) - 1)); }
-std::string drop_bit(const std::string& s, int h) {
-    return "(((" + s + " >> " + std::to_string(h + 1) + ") << " + std::to_string(h) + ") | (" + s + " & " +
-           std::to_string((1u << h) - 1) + "u))";
+std::string drop_bit(const std::string& s, int h, int sh) {
+    const int b = h + sh;
+    return "(((" + s + " >> " + std::to_string(b + 1) + ") << " + std::to_string(b) + ") | (" + s + " & " +
+           std::to_string((1u << b) - 1) + "u))";
+}
+
+// The LDS slot of a thread part (bytes, run time) XOR a register part
+// (slot index, a constant).  The two parts share only the low nibble of the
+// slot index (the swizzle), so the rest of the register part adds and goes
+// into the instruction's offset: one XOR per access.
+std::string lds_at(const std::string& tb, uint32_t reg, uint32_t es) {
+    const uint32_t lo = (reg & 15u) * es, hi = (reg & ~15u) * es;
+    std::string a = "smem_raw + ";
+    a += lo ? "(" + tb + " ^ " + std::to_string(lo) + "u)" : tb;
+    if (hi) a += " + " + std::to_string(hi) + "u";
+    return "*reinterpret_cast<T*>(" + a + ")";
 }
 
 void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& sfrom,
                const std::string& sto, const Split& sp) {
     std::ostringstream& o = g.o;
+    const uint32_t es = g.dbl ? 16 : 8;
+    const int sh = g.dbl ? 4 : 3;
     if (sp.kind == 0) {
         for (int rho = 0; rho < kNR; ++rho)
-            o << "    lds[" << sfrom << " ^ " << sreg(from, rho) << "u] = " << g.X(rho) << ";\n";
+            o << "    " << lds_at(sfrom, sreg(from, rho), es) << " = " << g.X(rho) << ";\n";
         o << "    __syncthreads();\n";
         g.identity();
         for (int rho = 0; rho < kNR; ++rho)
-            o << "    x[" << rho << "] = lds[" << sto << " ^ " << sreg(to, rho) << "u];\n";
+            o << "    x[" << rho << "] = " << lds_at(sto, sreg(to, rho), es) << ";\n";
         o << "    __syncthreads();\n";
         return;
     }
-    o << "    { const unsigned cf = " << drop_bit(sfrom, sp.h) << ", ct = " << drop_bit(sto, sp.h) << ";\n";
+    o << "    { const unsigned cf = " << drop_bit(sfrom, sp.h, sh) << ", ct = " << drop_bit(sto, sp.h, sh) << ";\n";
     if (sp.kind == 1) {
         const int old[kNR] = {g.perm[0], g.perm[1], g.perm[2],  g.perm[3],  g.perm[4],  g.perm[5],
                               g.perm[6], g.perm[7], g.perm[8],  g.perm[9],  g.perm[10], g.perm[11],
@@ -638,10 +654,10 @@ void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& 
         for (int k = 0; k < 2; ++k) {
             o << "    if (hb == " << k << "u) {\n";
             for (int rho = 0; rho < kNR; ++rho)
-                o << "    lds[cf ^ " << drop_bit(sreg(from, rho), sp.h) << "u] = x[" << old[rho] << "];\n";
+                o << "    " << lds_at("cf", drop_bit(sreg(from, rho), sp.h), es) << " = x[" << old[rho] << "];\n";
             o << "    }\n    __syncthreads();\n    if (hb == " << k << "u) {\n";
             for (int rho = 0; rho < kNR; ++rho)
-                o << "    x[" << rho << "] = lds[ct ^ " << drop_bit(sreg(to, rho), sp.h) << "u];\n";
+                o << "    x[" << rho << "] = " << lds_at("ct", drop_bit(sreg(to, rho), sp.h), es) << ";\n";
             o << "    }\n    __syncthreads();\n";
         }
         g.identity();
@@ -651,7 +667,7 @@ void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& 
             std::vector<int> freed;
             for (int rho = 0; rho < kNR; ++rho)
                 if (((rho >> sp.kf) & 1) == k) {
-                    o << "    lds[cf ^ " << drop_bit(sreg(from, rho), sp.h) << "u] = " << g.X(rho) << ";\n";
+                    o << "    " << lds_at("cf", drop_bit(sreg(from, rho), sp.h), es) << " = " << g.X(rho) << ";\n";
                     freed.push_back(g.perm[rho]);
                 }
             o << "    __syncthreads();\n";
@@ -659,7 +675,7 @@ void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& 
             for (int rho = 0; rho < kNR; ++rho)
                 if (((rho >> sp.kt) & 1) == k) {
                     np[rho] = freed[f++];
-                    o << "    x[" << np[rho] << "] = lds[ct ^ " << drop_bit(sreg(to, rho), sp.h) << "u];\n";
+                    o << "    x[" << np[rho] << "] = " << lds_at("ct", drop_bit(sreg(to, rho), sp.h), es) << ";\n";
                 }
             o << "    __syncthreads();\n";
         }
@@ -716,14 +732,14 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi, uint32_t* lds_bytes) {
     if (half) o << "__attribute__((amdgpu_waves_per_eu(4))) ";
     o << "void qs_pass_" << (g.dbl ? "c128_" : "c64_") << pi
       << "(T* __restrict__ a, unsigned long long ntiles, const R* __restrict__ mtab) {\n";
-    o << "    extern __shared__ unsigned char smem_raw[];\n";
-    o << "    T* lds = reinterpret_cast<T*>(smem_raw);\n";
+    // static LDS: its base address is a constant the compiler folds away
+    o << "    __shared__ __attribute__((aligned(16))) unsigned char smem_raw[" << *lds_bytes << "];\n";
     o << "    const unsigned t = threadIdx.x;\n";
     o << "    const unsigned long long g_ld = " << thr_g(ld) << ";\n";
     // the LDS thread parts of the phases that take part in a layout change
     for (int ph = pb; ph < pe; ++ph)
-        o << "    const unsigned s_" << ph - pb << " = " << thr_s(P.phases[ph]) << ";\n";
-    if (!direct_store) o << "    const unsigned s_st = " << thr_s(st) << ";\n";
+        o << "    const unsigned s_" << ph - pb << " = " << thr_s(P.phases[ph], uint32_t(es)) << ";\n";
+    if (!direct_store) o << "    const unsigned s_st = " << thr_s(st, uint32_t(es)) << ";\n";
     o << "    T x[16];\n";
     o << "    const unsigned long long tile = blockIdx.x;\n";
     o << "    if (tile >= ntiles) return;\n";
@@ -977,7 +993,7 @@ int native_launch(const FusedProgram& P, qs_state* st, int pass, uint32_t grid_c
     (void)grid_cap;
     const FPass& ps = P.passes[pass];
     const size_t es = st->dtype == QS_C128 ? 16 : 8;
-    const size_t shmem = size_t(pass) < P.native_lds.size() ? P.native_lds[pass] : es << ps.m;
+    const size_t shmem = 0;  // the kernels declare their LDS statically
     if (ps.ntiles > 0xffffffffull)
         return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::fused", "more than 2^32 tiles in one pass");
     const uint64_t grid = ps.ntiles;
