@@ -6,8 +6,9 @@
 //                bit set holds the low output bits and the output bits fed by
 //                the low input bits, so both the gather from HBM and the
 //                scatter to HBM run over contiguous 2^L-element segments; the
-//                transposition happens in LDS.  The per-element index parts
-//                come from a table k_bitperm_table builds once per call.
+//                transposition happens in LDS.  Each tile bit's share of
+//                the indices is a kernel argument; the tile number's comes
+//                from a table k_bitperm_table builds once per call.
 //   k_digitperm  the same map for qudit states (mixed-radix digits).
 //   k_ket_pt     ptranspose of a ket: out(i, j) = psi(i') conj(psi(j')).
 //   k_ptrace     partial trace; `LANES` lanes cooperate on one output entry
@@ -44,9 +45,6 @@ struct BitPerm {
     uint32_t tbits;        // |T|
     uint32_t nrest;        // bits outside T
     uint8_t src[64];       // input bit of output bit b
-    uint8_t load_out[16];  // load order: e bit k -> output bit load_out[k]
-    uint8_t store_out[16]; // store order: e bit k -> output bit store_out[k]
-    uint8_t tpos[16];      // T bits ascending (LDS packing order)
     uint8_t rest[64];      // output bits outside T, ascending
 };
 
@@ -61,59 +59,64 @@ __device__ __forceinline__ uint64_t gather_bits(uint64_t y, const uint8_t* src,
 constexpr uint32_t kBpMaxTileBits = 12;
 constexpr int kBpPer = 1 << (kBpMaxTileBits - 8);
 
-// The part of every index that comes from an element's position e inside
-// the tile (the bits in T); G and the LDS packing act bit by bit, so the
-// part from the tile number (bits outside T) is simply OR-ed in.  Both
-// parts come from a table built once per call (it stays in L2): one entry
-// per position e, and the tile number's contribution byte by byte.
-struct BpEntry {
-    uint64_t in_off, out_off;  // input / output index bits from e
-    uint32_t ld_pk, st_pk;     // LDS slots of the e-th load / store
+// Every index is XOR-linear (here: OR of disjoint bits) in the element's
+// position e inside the tile and in the tile number: G and the LDS packing
+// act bit by bit.  A table built once per call (it stays in L2) holds the
+// contributions of each thread number (bits 0..7 of e), of each register
+// index i (bits 8.. of e, e = t + 256 i) and of each tile-number byte value
+// to the input index, the output index and the LDS slots of the load and
+// store orders: a thread reads its entry once, and each element ORs in the
+// (uniform) entry of its register index.
+struct BpBits {
+    uint64_t in_bit[kBpMaxTileBits];   // input index bit of e bit k (load order)
+    uint64_t out_bit[kBpMaxTileBits];  // output index bit of e bit k (store order)
+    uint32_t ld_bit[kBpMaxTileBits];   // LDS slot bit of e bit k, load order
+    uint32_t st_bit[kBpMaxTileBits];   // LDS slot bit of e bit k, store order
+};
+struct BpPart {
+    uint64_t in, out;
+    uint32_t ld, st;
 };
 constexpr int kBpTileBytes = 8;  // up to 64 tile-number bits, 8 at a time
 struct BpTable {
     uint64_t xt[kBpTileBytes][256];  // input bits of tile-number byte c = v
     uint64_t yt[kBpTileBytes][256];  // output bits of tile-number byte c = v
-    BpEntry e[1 << kBpMaxTileBits];
+    BpPart thr[256];                 // thread number t
+    BpPart reg[1 << (kBpMaxTileBits - 8)];  // register index i
 };
 
-__global__ __launch_bounds__(256) void k_bitperm_table(BitPerm bp, BpTable* __restrict__ tab) {
-    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
-    if (blockIdx.x < kBpTileBytes) {  // tile-number byte blockIdx.x, value threadIdx.x
-        const uint32_t c = blockIdx.x, v = threadIdx.x;
-        uint64_t x = 0, y = 0;
-        for (uint32_t k = 8 * c; k < 8 * c + 8 && k < bp.nrest; ++k) {
-            const uint64_t bit = (v >> (k - 8 * c)) & 1;
-            y |= bit << bp.rest[k];
-            x |= bit << bp.src[bp.rest[k]];
+__global__ __launch_bounds__(256) void k_bitperm_table(BitPerm bp, BpBits bb, BpTable* __restrict__ tab) {
+    const uint32_t c = blockIdx.x, v = threadIdx.x;  // tile-number byte c, value v
+    uint64_t x = 0, y = 0;
+    for (uint32_t k = 8 * c; k < 8 * c + 8 && k < bp.nrest; ++k) {
+        const uint64_t bit = (v >> (k - 8 * c)) & 1;
+        y |= bit << bp.rest[k];
+        x |= bit << bp.src[bp.rest[k]];
+    }
+    tab->xt[c][v] = x;
+    tab->yt[c][v] = y;
+    // e bits of v: thread number v (block 0) or register index v (block 1)
+    if (c < 2 && (c == 0 || v < (1u << (kBpMaxTileBits - 8)))) {
+        BpPart r{0, 0, 0, 0};
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t kb = 8 * c + k;
+            if (kb >= kBpMaxTileBits || !((v >> k) & 1)) continue;
+            r.in |= bb.in_bit[kb];
+            r.out |= bb.out_bit[kb];
+            r.ld |= bb.ld_bit[kb];
+            r.st |= bb.st_bit[kb];
         }
-        tab->xt[c][v] = x;
-        tab->yt[c][v] = y;
+        (c == 0 ? tab->thr[v] : tab->reg[v]) = r;
     }
-    if (e >= (1u << bp.tbits)) return;
-    uint64_t yl = 0, ys = 0;
-    for (uint32_t k = 0; k < bp.tbits; ++k) {
-        yl |= uint64_t((e >> k) & 1) << bp.load_out[k];
-        ys |= uint64_t((e >> k) & 1) << bp.store_out[k];
-    }
-    uint32_t pl = 0, pst = 0;
-    for (uint32_t k = 0; k < bp.tbits; ++k) {
-        pl |= uint32_t((yl >> bp.tpos[k]) & 1) << k;
-        pst |= uint32_t((ys >> bp.tpos[k]) & 1) << k;
-    }
-    BpEntry r;
-    r.in_off = gather_bits(yl, bp.src, bp.nb);
-    r.out_off = ys;
-    r.ld_pk = pl;
-    r.st_pk = pst;
-    tab->e[e] = r;
 }
 
 // One tile per workgroup over a grid of all tiles (the dispatcher then
 // walks the state in order, which streams HBM better than a persistent
 // grid): every load of the tile is issued, the elements go to their LDS
-// slots, and after one barrier every store.
-template <typename T, int PER>
+// slots, and after one barrier every store.  FULL: the tile has exactly
+// 256 PER elements (no per-element test, so the compiler keeps every load
+// ahead of the LDS writes instead of sinking each next to its write).
+template <typename T, int PER, bool FULL>
 __global__ __launch_bounds__(256) void k_bitperm(const T* __restrict__ in,
                                                  T* __restrict__ out,
                                                  const BpTable* __restrict__ tab,
@@ -123,33 +126,27 @@ __global__ __launch_bounds__(256) void k_bitperm(const T* __restrict__ in,
     const uint32_t tsz = 1u << tbits;
     const uint32_t t = threadIdx.x;
     const uint64_t tile = tile0 + blockIdx.x;
-    uint64_t ybase = 0, xbase = 0;
+    const BpPart th = tab->thr[t];
+    uint64_t xb = th.in, yb = th.out;
     for (uint32_t c = 0; 8 * c < nrest; ++c) {
         const uint32_t v = uint32_t(tile >> (8 * c)) & 255u;
-        xbase |= tab->xt[c][v];
-        ybase |= tab->yt[c][v];
+        xb |= tab->xt[c][v];
+        yb |= tab->yt[c][v];
     }
     // PER = tsz / 256 elements per thread when tsz >= 256; a smaller tile
-    // leaves threads without elements (they read entry 0 and store nothing)
-    uint64_t oo[PER];
-    uint32_t lp[PER], sp[PER];
+    // (PER 1, not FULL) leaves threads without elements (bits of e from
+    // tbits up contribute 0, so their loads stay inside the state; they
+    // store nothing)
     T v[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {  // all loads in flight together
-        const uint32_t e = t + 256u * i < tsz ? t + 256u * i : 0u;
-        const BpEntry* en = tab->e + e;
-        oo[i] = en->out_off;
-        lp[i] = en->ld_pk;
-        sp[i] = en->st_pk;
-        v[i] = in[xbase | en->in_off];
-    }
+    for (int i = 0; i < PER; ++i) v[i] = in[xb | tab->reg[i].in];  // all loads in flight
 #pragma unroll
     for (int i = 0; i < PER; ++i)
-        if (t + 256u * i < tsz) s[lp[i]] = v[i];
+        if (FULL || t + 256u * i < tsz) s[th.ld | tab->reg[i].ld] = v[i];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PER; ++i)
-        if (t + 256u * i < tsz) out[ybase | oo[i]] = s[sp[i]];
+        if (FULL || t + 256u * i < tsz) out[yb | tab->reg[i].out] = s[th.st | tab->reg[i].st];
 }
 
 // ---------------------------------------------------------------------------
@@ -346,7 +343,6 @@ int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t
     for (uint32_t b = 0; b < nb; ++b)
         if (inT[b]) tset.push_back(b);
     bp.tbits = static_cast<uint32_t>(tset.size());
-    for (size_t k = 0; k < tset.size(); ++k) bp.tpos[k] = static_cast<uint8_t>(tset[k]);
     // load order: low e bits feed the output bits sourced by input bits 0..L-1
     std::vector<uint32_t> load(low_in_fed.begin(), low_in_fed.end());
     for (uint32_t b : tset)
@@ -355,10 +351,6 @@ int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t
     std::vector<uint32_t> store(low_out.begin(), low_out.end());
     for (uint32_t b : tset)
         if (std::find(store.begin(), store.end(), b) == store.end()) store.push_back(b);
-    for (size_t k = 0; k < tset.size(); ++k) {
-        bp.load_out[k] = static_cast<uint8_t>(load[k]);
-        bp.store_out[k] = static_cast<uint8_t>(store[k]);
-    }
     std::vector<uint32_t> rest;
     for (uint32_t b = 0; b < nb; ++b)
         if (!inT[b]) rest.push_back(b);
@@ -366,28 +358,56 @@ int launch_bitperm(const qs_state* in, qs_state* out, const std::vector<uint32_t
     for (size_t k = 0; k < rest.size(); ++k) bp.rest[k] = static_cast<uint8_t>(rest[k]);
     // the kernel keeps at most 2^kBpMaxTileBits / 256 elements per thread
     if (bp.tbits > kBpMaxTileBits) return fail(QS_ERR_NOT_IMPLEMENTED, "qpp_amd::bitperm", "tile");
+    // each tile bit's contribution to the four index parts
+    BpBits bb{};
+    auto slot_of = [&](uint32_t b) {
+        return uint32_t(std::find(tset.begin(), tset.end(), b) - tset.begin());
+    };
+    for (uint32_t k = 0; k < bp.tbits; ++k) {
+        bb.in_bit[k] = uint64_t{1} << src[load[k]];
+        bb.out_bit[k] = uint64_t{1} << store[k];
+        bb.ld_bit[k] = 1u << slot_of(load[k]);
+        bb.st_bit[k] = 1u << slot_of(store[k]);
+    }
     const uint64_t ntiles = uint64_t{1} << bp.nrest;
     const size_t shmem = sizeof(T) << bp.tbits;
     constexpr const char* W = "qpp_amd::bitperm";
     BpTable* tab = nullptr;
     QS_HIP(hipMallocAsync(reinterpret_cast<void**>(&tab), sizeof(BpTable), in->stream), W);
-    const uint32_t tgrid = std::max<uint32_t>(kBpTileBytes, ((1u << bp.tbits) + 255) / 256);
-    hipLaunchKernelGGL(k_bitperm_table, dim3(tgrid), dim3(256), 0, in->stream, bp, tab);
+    hipLaunchKernelGGL(k_bitperm_table, dim3(kBpTileBytes), dim3(256), 0, in->stream, bp, bb, tab);  // >= 2 blocks
     // one tile per workgroup, at most 2^30 workgroups per launch
     for (uint64_t t0 = 0; t0 < ntiles; t0 += uint64_t{1} << 30) {
         const uint64_t g = std::min<uint64_t>(ntiles - t0, uint64_t{1} << 30);
         const dim3 gd(static_cast<uint32_t>(g));
         const T* src = static_cast<const T*>(in->d);
         T* dst = static_cast<T*>(out->d);
-        if (bp.tbits <= 10)
-            hipLaunchKernelGGL((k_bitperm<T, 4>), gd, dim3(256), shmem, in->stream, src, dst, tab,
-                               t0, bp.tbits, bp.nrest);
-        else if (bp.tbits == 11)
-            hipLaunchKernelGGL((k_bitperm<T, 8>), gd, dim3(256), shmem, in->stream, src, dst, tab,
-                               t0, bp.tbits, bp.nrest);
-        else
-            hipLaunchKernelGGL((k_bitperm<T, 16>), gd, dim3(256), shmem, in->stream, src, dst,
-                               tab, t0, bp.tbits, bp.nrest);
+        // 256 threads x PER elements: a full tile for 8 to 12 tile bits
+        switch (bp.tbits) {
+            case 12:
+                hipLaunchKernelGGL((k_bitperm<T, 16, true>), gd, dim3(256), shmem, in->stream, src,
+                                   dst, tab, t0, bp.tbits, bp.nrest);
+                break;
+            case 11:
+                hipLaunchKernelGGL((k_bitperm<T, 8, true>), gd, dim3(256), shmem, in->stream, src,
+                                   dst, tab, t0, bp.tbits, bp.nrest);
+                break;
+            case 10:
+                hipLaunchKernelGGL((k_bitperm<T, 4, true>), gd, dim3(256), shmem, in->stream, src,
+                                   dst, tab, t0, bp.tbits, bp.nrest);
+                break;
+            case 9:
+                hipLaunchKernelGGL((k_bitperm<T, 2, true>), gd, dim3(256), shmem, in->stream, src,
+                                   dst, tab, t0, bp.tbits, bp.nrest);
+                break;
+            case 8:
+                hipLaunchKernelGGL((k_bitperm<T, 1, true>), gd, dim3(256), shmem, in->stream, src,
+                                   dst, tab, t0, bp.tbits, bp.nrest);
+                break;
+            default:
+                hipLaunchKernelGGL((k_bitperm<T, 1, false>), gd, dim3(256), shmem, in->stream, src,
+                                   dst, tab, t0, bp.tbits, bp.nrest);
+                break;
+        }
     }
     const hipError_t e = hipGetLastError();
     (void)hipFreeAsync(tab, in->stream);
