@@ -664,6 +664,159 @@ def bench_sample(args, qa, sizes=(20, 24, 26), shots=1024, reps=5):
     return {"metric": "qpp::sample of 1024 shots, ms per call", "shots": shots, "rows": rows}
 
 
+FP64_PROBE_SRC = r"""
+typedef double d4 __attribute__((ext_vector_type(4)));
+// 16 complex rotations per thread per iteration: the compiled loop is 32
+// v_mul_f64, 32 v_fmac_f64 and 32 v_add_f64
+extern "C" __global__ __launch_bounds__(256) void p_valu(const double* __restrict__ in,
+        const double* __restrict__ cs, double* __restrict__ out, int iters) {
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    double x[16], y[16];
+    for (int k = 0; k < 16; ++k) { x[k] = in[i * 32u + 2u * k]; y[k] = in[i * 32u + 2u * k + 1u]; }
+    const double c = cs[2u * i], s = cs[2u * i + 1u];
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double nx = fma(c, x[k], -(s * y[k]));
+            const double ny = fma(s, x[k], c * y[k]);
+            x[k] = nx; y[k] = ny;
+        }
+    }
+    double acc = 0;
+    for (int k = 0; k < 16; ++k) acc += x[k] + y[k];
+    out[i] = acc;
+}
+// four independent v_mfma_f64_16x16x4f64 chains per wave
+extern "C" __global__ __launch_bounds__(256) void p_mfma(const double* __restrict__ in,
+        const double* __restrict__ cs, double* __restrict__ out, int iters) {
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    const double a = in[i * 32u], b0 = in[i * 32u + 1u], b1 = in[i * 32u + 2u];
+    d4 q0 = {0, 0, 0, 0}, q1 = q0, q2 = q0, q3 = q0;
+    for (int it = 0; it < iters; ++it) {
+        const double b = (it & 1) ? b1 : b0;
+        q0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, q0, 0, 0, 0);
+        q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, q1, 0, 0, 0);
+        q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, q2, 0, 0, 0);
+        q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, q3, 0, 0, 0);
+    }
+    out[i] = q0[0] + q1[1] + q2[2] + q3[3];
+}
+// both in one wave: the same rotations and MFMA chains interleaved
+extern "C" __global__ __launch_bounds__(256) void p_both(const double* __restrict__ in,
+        const double* __restrict__ cs, double* __restrict__ out, int iters) {
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    double x[16], y[16];
+    for (int k = 0; k < 16; ++k) { x[k] = in[i * 32u + 2u * k]; y[k] = in[i * 32u + 2u * k + 1u]; }
+    const double c = cs[2u * i], s = cs[2u * i + 1u];
+    const double a = in[i * 32u], b0 = in[i * 32u + 1u], b1 = in[i * 32u + 2u];
+    d4 q0 = {0, 0, 0, 0}, q1 = q0, q2 = q0, q3 = q0;
+    for (int it = 0; it < iters; ++it) {
+        const double b = (it & 1) ? b1 : b0;
+        q0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, q0, 0, 0, 0);
+        q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, q1, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double nx = fma(c, x[k], -(s * y[k]));
+            const double ny = fma(s, x[k], c * y[k]);
+            x[k] = nx; y[k] = ny;
+        }
+        q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, q2, 0, 0, 0);
+        q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, q3, 0, 0, 0);
+#pragma unroll
+        for (int k = 8; k < 16; ++k) {
+            const double nx = fma(c, x[k], -(s * y[k]));
+            const double ny = fma(s, x[k], c * y[k]);
+            x[k] = nx; y[k] = ny;
+        }
+    }
+    double acc = q0[0] + q1[1] + q2[2] + q3[3];
+    for (int k = 0; k < 16; ++k) acc += x[k] + y[k];
+    out[i] = acc;
+}
+"""
+
+
+def bench_fp64(args, qa):
+    """FP64 on the vector ALUs against FP64 on the matrix cores, with zero
+    and with random operands: whether complex128 gate arithmetic could move
+    to v_mfma_f64_16x16x4f64, and how much the clock depends on the data.
+    Kernels compiled here with hipRTC; 2048 workgroups of 256 threads."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    rtc = ctypes.CDLL("libhiprtc.so")
+    vp = ctypes.c_void_p
+    prog = vp()
+    assert rtc.hiprtcCreateProgram(ctypes.byref(prog), FP64_PROBE_SRC.encode(), b"probe.hip", 0,
+                                   None, None) == 0
+    opts = [b"--offload-arch=gfx950", b"-O3", b"-Xclang", b"-target-feature", b"-Xclang",
+            b"-packed-fp32-ops"]
+    if rtc.hiprtcCompileProgram(prog, len(opts), (ctypes.c_char_p * len(opts))(*opts)) != 0:
+        raise RuntimeError("fp64 probe: hipRTC compile failed")
+    size = ctypes.c_size_t()
+    rtc.hiprtcGetCodeSize(prog, ctypes.byref(size))
+    code = ctypes.create_string_buffer(size.value)
+    rtc.hiprtcGetCode(prog, code)
+    mod = vp()
+    assert hip.hipModuleLoadData(ctypes.byref(mod), code) == 0
+    G, iters = 2048, 20000
+    T = G * 256
+    bufs = {}
+    for name, n in (("in", 32 * T), ("cs", 2 * T), ("out", T)):
+        bufs[name] = vp()
+        assert hip.hipMalloc(ctypes.byref(bufs[name]), ctypes.c_size_t(8 * n)) == 0
+    ev0, ev1 = vp(), vp()
+    hip.hipEventCreate(ctypes.byref(ev0))
+    hip.hipEventCreate(ctypes.byref(ev1))
+    rng = np.random.default_rng(args.seed)
+    # per thread-iteration: VALU FP64 instructions (p_valu's compiled loop)
+    # and MFMA flops (4 x 16x16x4 x 2 / 64 lanes)
+    cases = [("p_valu", 96.0, 0.0), ("p_mfma", 0.0, 128.0), ("p_both", 96.0, 128.0)]
+    rows = []
+    for data in ("zeros", "random"):
+        if data == "zeros":
+            hin, hcs = np.zeros(32 * T), np.zeros(2 * T)
+        else:
+            hin = rng.standard_normal(32 * T)
+            th = rng.random(T) * 2 * np.pi
+            hcs = np.stack([np.cos(th), np.sin(th)], 1).reshape(-1).copy()
+        for name, arr in (("in", hin), ("cs", hcs)):
+            assert hip.hipMemcpy(bufs[name], arr.ctypes.data_as(vp), ctypes.c_size_t(arr.nbytes), 1) == 0
+        for kname, vi, mf in cases:
+            f = vp()
+            assert hip.hipModuleGetFunction(ctypes.byref(f), mod, kname.encode()) == 0
+            it = ctypes.c_int(iters)
+            argv = (vp * 4)(*[ctypes.cast(ctypes.byref(x), vp) for x in (bufs["in"], bufs["cs"], bufs["out"], it)])
+            best = float("inf")
+            for _ in range(4):  # the first is a warm-up
+                hip.hipEventRecord(ev0, None)
+                assert hip.hipModuleLaunchKernel(f, G, 1, 1, 256, 1, 1, 0, None, argv, None) == 0
+                hip.hipEventRecord(ev1, None)
+                hip.hipEventSynchronize(ev1)
+                ms = ctypes.c_float()
+                hip.hipEventElapsedTime(ctypes.byref(ms), ev0, ev1)
+                best = min(best, ms.value)
+            sec = best / 1e3
+            out = np.empty(T)
+            hip.hipMemcpy(out.ctypes.data_as(vp), bufs["out"], ctypes.c_size_t(out.nbytes), 2)
+            # FMA-rate issue: 16 FP64 lanes per clock per SIMD at 2.4 GHz
+            valu = vi * T * iters / sec / (256 * 4 * 16 * 2.4e9)
+            mtf = mf * T * iters / sec / 1e12
+            rows.append({"data": data, "kernel": kname, "ms": round(best, 3),
+                         "valu_fp64_issue_vs_fma_peak": round(valu, 3),
+                         "mfma_f64_TFLOPs": round(mtf, 1), "mfma_frac": round(mtf / 78.6, 3),
+                         "finite": bool(np.isfinite(out).all())})
+            log(f"fp64 probe {data} {kname}: {rows[-1]}")
+    for b in bufs.values():
+        hip.hipFree(b)
+    return {"metric": "FP64 vector vs matrix-core throughput (probe)", "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+            "config": {"workload": f"{G} workgroups x 256 threads, {iters} iterations; "
+                                   "p_valu: 16 complex rotations per thread per iteration; "
+                                   "p_mfma: 4 v_mfma_f64_16x16x4f64 chains per wave; p_both: both "
+                                   "interleaved in one wave"},
+            "rows": rows}
+
+
 def bench_copy(args, qa):
     """Reference streaming kernel: a 30-qubit complex128 state (16 GiB) copied
     into another with qs_state_copy (each byte read once, written once).
@@ -966,7 +1119,8 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--seed", type=int, default=30)
-    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy", "block", "sample"],
+    ap.add_argument("--mode", choices=["fused", "gates", "batch", "serve", "rho", "copy", "block", "sample",
+                                       "fp64"],
                     default="fused")
     ap.add_argument("--requests", type=int, default=200, help="serve mode: requests")
     ap.add_argument("--cal-requests", type=int, default=40,
@@ -1035,6 +1189,9 @@ def main():
         return
     if args.mode == "sample":
         print(json.dumps(bench_sample(args, qa)), flush=True)
+        return
+    if args.mode == "fp64":
+        print(json.dumps(bench_fp64(args, qa)), flush=True)
         return
     out = bench_circuit(args, qa, args.qubits, args.depth, primary=True)
     if args.mode == "fused" and args.secondary:
