@@ -391,6 +391,12 @@ int qs_program_run(qs_program* p, qs_state* st);
  * runs (reg_bits 4 programs).  Takes seconds; a failure leaves the program
  * on the general tile kernel. */
 int qs_program_compile_native(qs_program* p);
+/* No qpp counterpart: diagnostics of the specialised kernels built by
+ * qs_program_compile_native.  counts[0..4]: layout changes moved whole,
+ * split on a wave bit, split on a lane bit, split on a register bit, and
+ * passes whose kernels need half the LDS (all their changes split).  All
+ * zero before a successful compile. */
+int qs_program_native_stats(const qs_program* p, uint64_t counts[5]);
 /* qs_program_run with a HIP event recorded on the state's stream before the
  * first pass and after each pass; waits for the last one and writes each
  * pass's elapsed milliseconds to pass_ms[0 .. num_passes-1] (the kernel

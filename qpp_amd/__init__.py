@@ -824,6 +824,15 @@ class _FuseOpts(ctypes.Structure):
                 ("grid_cap", c_uint32)]
 
 
+def _native_stats(h):
+    L = lib()
+    L.qs_program_native_stats.argtypes = [c_void_p, POINTER(c_uint64)]
+    c = (c_uint64 * 5)()
+    _check(L.qs_program_native_stats(h, c))
+    return {"whole": c[0], "wave_split": c[1], "lane_split": c[2], "register_split": c[3],
+            "half_lds_passes": c[4]}
+
+
 class Program:
     """Gate steps of a circuit compiled into tile-fused passes."""
 
@@ -857,6 +866,13 @@ class Program:
         _check(L.qs_program_compile_native(self._h))
         self._native = True
         return self
+
+    def native_stats(self):
+        """How the specialised kernels move their layout changes through LDS
+        (qs_program_native_stats): counts of changes moved whole, split on a
+        wave bit, a lane bit or a register bit, and of passes at half the
+        LDS."""
+        return _native_stats(self._h)
 
     def run_timed(self, st):
         """Run once; the milliseconds of each pass, from HIP events recorded
@@ -979,6 +995,10 @@ class BitProgram:
         L.qs_program_compile_native.argtypes = [c_void_p]
         _check(L.qs_program_compile_native(self._h))
         return self
+
+    def native_stats(self):
+        """See Program.native_stats."""
+        return _native_stats(self._h)
 
     def info(self):
         g, p, h = c_uint64(), c_uint64(), c_uint64()

@@ -139,6 +139,9 @@ struct FusedProgram {
     void* native_tab = nullptr;          // the passes' constants (device)
     std::vector<uint64_t> native_off;   // each pass's first constant
     std::vector<uint32_t> native_lds;   // each pass kernel's LDS bytes per workgroup
+    // layout changes moved whole / split on a wave, lane, register bit;
+    // passes at half the LDS
+    uint64_t native_stats[5] = {0, 0, 0, 0, 0};
 };
 
 namespace qs {
@@ -160,7 +163,7 @@ std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double
                           std::vector<uint64_t>* off = nullptr,
                           std::vector<std::string>* pass_src = nullptr,
                           std::string* preamble = nullptr,
-                          std::vector<uint32_t>* lds = nullptr);
+                          std::vector<uint32_t>* lds = nullptr, uint64_t* stats = nullptr);
 int native_build(FusedProgram& P, qs_dtype dt);
 // the program with runs of ops on one or two register bits of a phase
 // multiplied into one op where that costs fewer FP64 instructions

@@ -54,6 +54,7 @@ struct Gen {
     bool dbl = true;
     int perm[kNR];
     std::vector<double> tab;  // the pass's constants (re, im pairs)
+    uint64_t stats[5] = {0, 0, 0, 0, 0};  // as FusedProgram::native_stats
 
     // a complex constant: its place in the pass's table
     std::string cx(double re, double im) {
@@ -726,6 +727,8 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi, uint32_t* lds_bytes) {
         if (x.kind == 0) half = false;
     if (!half)
         for (Split& x : sp) x.kind = 0;
+    for (const Split& x : sp) ++g.stats[x.kind == 0 ? 0 : x.kind == 2 ? 3 : x.j >= 6 ? 1 : 2];
+    if (half) ++g.stats[4];
     const size_t es = g.dbl ? 16 : 8;
     *lds_bytes = uint32_t((half ? es << (ps.m - 1) : es << ps.m));
     o << "extern \"C\" __global__ __launch_bounds__(256) ";
@@ -805,7 +808,7 @@ namespace qs {
 // has each pass's kernel alone, for compiling groups of passes apart).
 std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double>* tab,
                           std::vector<uint64_t>* off, std::vector<std::string>* pass_src,
-                          std::string* preamble, std::vector<uint32_t>* lds) {
+                          std::string* preamble, std::vector<uint32_t>* lds, uint64_t* stats) {
     Gen g;
     g.dbl = dt == QS_C128;
     std::string pre = g.dbl ? "typedef double2 T;\ntypedef double R;\n" : "typedef float2 T;\ntypedef float R;\n";
@@ -828,6 +831,8 @@ std::string native_source(const FusedProgram& P, qs_dtype dt, std::vector<double
         if (off) off->push_back(tab ? tab->size() : 0);
         if (tab) tab->insert(tab->end(), g.tab.begin(), g.tab.end());
     }
+    if (stats)
+        for (int i = 0; i < 5; ++i) stats[i] = g.stats[i];
     return all;
 }
 
@@ -891,7 +896,8 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     std::vector<std::string> psrc;
     std::string pre;
     std::vector<uint32_t> lds;
-    (void)native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre, &lds);
+    uint64_t stats[5];
+    (void)native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre, &lds, stats);
     // groups of passes of about equal source size, one per thread
     const size_t np = P.passes.size();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -969,6 +975,7 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     P.native_tab = dtab;
     P.native_off = std::move(off);
     P.native_lds = std::move(lds);
+    for (int i = 0; i < 5; ++i) P.native_stats[i] = stats[i];
     return QS_OK;
 }
 
@@ -984,6 +991,7 @@ void native_free(FusedProgram& P) {
     P.native_fn.clear();
     P.native_off.clear();
     P.native_lds.clear();
+    for (uint64_t& c : P.native_stats) c = 0;
 }
 
 // One pass through its specialised kernel: a workgroup per tile (grid_cap is

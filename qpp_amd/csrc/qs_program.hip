@@ -217,6 +217,13 @@ int qs_program_info(const qs_program* p, uint64_t* num_gates, uint64_t* num_pass
     return QS_OK;
 }
 
+int qs_program_native_stats(const qs_program* p, uint64_t counts[5]) {
+    if (!p) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
+    if (!counts) return fail(QS_ERR_SIZE_MISMATCH, "qpp_amd::program", "counts is null");
+    for (int i = 0; i < 5; ++i) counts[i] = p->P.native_stats[i];
+    return QS_OK;
+}
+
 int qs_program_destroy(qs_program* p) {
     if (!p) return QS_OK;
     qs::native_free(p->P);
