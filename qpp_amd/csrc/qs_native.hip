@@ -434,7 +434,10 @@ namespace {
 
 // y = op(x) for one op, in place on the register array under the current
 // renaming (qs_tile_ops.hpp run_op_h / run_op_ct / do_*)
-void gen_op(Gen& g, const FusedProgram& P, const FOp& op) {
+// fx: an X on one register bit under a lane or tile condition, folded into
+// this op (the next op of the phase on that bit): the op applies U X where
+// the condition holds, U elsewhere (see fold_target).
+void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullptr) {
     std::ostringstream& o = g.o;
     const uint32_t R = op.regok;
     const bool ct = op.ctrl != 0;
@@ -469,8 +472,32 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op) {
     };
     switch (op.kind) {
         case OP_U1: {
+            if (fx && fx->omask) {
+                // U and U X (columns swapped) side by side in the table, the
+                // tile's condition picks one: a uniform offset, no VALU work
+                const size_t i0 = g.tab.size();
+                const int src[8] = {0, 2, 4, 6, 4, 6, 0, 2};  // u00 u10 u01 u11, then u01 u11 u00 u10
+                for (int k = 0; k < 8; ++k) (void)g.cx(op.m[src[k]], op.m[src[k] + 1]);
+                o << "    const int so = (base & " << u64(fx->omask) << ") == " << u64(fx->oval) << " ? 8 : 0;\n";
+                auto at = [&](int k) {
+                    return "mk(cm[" + std::to_string(i0 + 2 * k) + " + so], cm[" + std::to_string(i0 + 2 * k + 1) + " + so])";
+                };
+                o << "    const T u00 = " << at(0) << ", u10 = " << at(1) << ", u01 = " << at(2) << ", u11 = " << at(3)
+                  << ";\n";
+                u1(op.j0, "u00", "u10", "u01", "u11", true);
+                break;
+            }
             o << "    const T u00 = " << g.cx(op.m[0], op.m[1]) << ", u10 = " << g.cx(op.m[2], op.m[3])
               << ", u01 = " << g.cx(op.m[4], op.m[5]) << ", u11 = " << g.cx(op.m[6], op.m[7]) << ";\n";
+            if (fx && fx->tmask) {
+                // U X on the lanes whose condition holds: the columns swapped
+                // per lane (16 selects instead of 64 for the register swap)
+                o << "    const bool xk = (t & " << fx->tmask << "u) == " << fx->tval << "u;\n";
+                o << "    const T w00 = sel(xk, u01, u00), w10 = sel(xk, u11, u10), w01 = sel(xk, u00, u01), "
+                     "w11 = sel(xk, u10, u11);\n";
+                u1(op.j0, "w00", "w10", "w01", "w11", true);
+                break;
+            }
             u1(op.j0, "u00", "u10", "u01", "u11", true);
             break;
         }
@@ -518,9 +545,23 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op) {
         case OP_U2: {
             const int J0 = op.j0, J1 = op.j1;
             const C* M = P.mats.data() + op.moff;
-            o << "    const T m[16] = {";
-            for (int i = 0; i < 16; ++i) o << (i ? ", " : "") << g.cx(M[i]);
-            o << "};\n";
+            if (fx && fx->omask) {
+                // M and M (X on the folded bit) side by side, picked by the
+                // tile's condition
+                const int bk = fx->j0 == J0 ? 1 : 2;
+                const size_t i0 = g.tab.size();
+                for (int i = 0; i < 16; ++i) (void)g.cx(M[i]);
+                for (int i = 0; i < 16; ++i) (void)g.cx(M[(i & 3) + 4 * ((i >> 2) ^ bk)]);
+                o << "    const int so = (base & " << u64(fx->omask) << ") == " << u64(fx->oval) << " ? 32 : 0;\n";
+                o << "    const T m[16] = {";
+                for (int i = 0; i < 16; ++i)
+                    o << (i ? ", " : "") << "mk(cm[" << i0 + 2 * i << " + so], cm[" << i0 + 2 * i + 1 << " + so])";
+                o << "};\n";
+            } else {
+                o << "    const T m[16] = {";
+                for (int i = 0; i < 16; ++i) o << (i ? ", " : "") << g.cx(M[i]);
+                o << "};\n";
+            }
             for (int rho = 0; rho < kNR; ++rho) {
                 if (((rho >> J0) & 1) || ((rho >> J1) & 1)) continue;
                 if (!rok(rho)) continue;
@@ -625,6 +666,40 @@ std::string lds_at(const std::string& tb, uint32_t reg, uint32_t es) {
     a += lo ? "(" + tb + " ^ " + std::to_string(lo) + "u)" : tb;
     if (hi) a += " + " + std::to_string(hi) + "u";
     return "*reinterpret_cast<T*>(" + a + ")";
+}
+
+// An X on register bit k whose only control is a lane condition or a tile
+// condition (bits outside the tile) commutes with the phase's ops on other
+// register bits, so it can wait for the next op on k and fold into it: a
+// plain 1-bit op on k becomes U X where the condition holds (a per-lane
+// select of its 4 entries, or a uniform choice between two tables for a
+// tile condition), and so does a plain 2-bit op for a tile condition.  The
+// X then costs no register swap.  Returns, per op of the phase, the index
+// of the op it folds into (-1: emitted as it is).
+bool plain_op(const FOp& o) {
+    return o.tmask == 0 && o.omask == 0 && (o.regok & 0xFFFFu) == 0xFFFFu;
+}
+
+std::vector<int> fold_targets(const FusedProgram& P, const FPhase& ph) {
+    const int n = ph.op_end - ph.op_begin;
+    std::vector<int> into(size_t(n), -1);
+    std::vector<bool> taken(size_t(n), false);
+    for (int i = 0; i < n; ++i) {
+        const FOp& x = P.ops[size_t(ph.op_begin + i)];
+        if (x.kind != OP_X || (x.regok & 0xFFFFu) != 0xFFFFu || (x.tmask != 0) == (x.omask != 0)) continue;
+        for (int j = i + 1; j < n; ++j) {
+            const FOp& y = P.ops[size_t(ph.op_begin + j)];
+            if (!((op_bits(y) >> x.j0) & 1u)) continue;
+            const bool ok = plain_op(y) && !taken[size_t(j)] &&
+                            ((y.kind == OP_U1 && y.j0 == x.j0) || (y.kind == OP_U2 && x.omask != 0));
+            if (ok) {
+                into[size_t(i)] = j;
+                taken[size_t(j)] = true;
+            }
+            break;
+        }
+    }
+    return into;
 }
 
 void gen_remap(Gen& g, const FPhase& from, const FPhase& to, const std::string& sfrom,
@@ -760,7 +835,16 @@ void gen_pass(Gen& g, const FusedProgram& P, int pi, uint32_t* lds_bytes) {
             gen_remap(g, P.phases[ph - 1], P.phases[ph], "s_" + std::to_string(ph - 1 - pb),
                       "s_" + std::to_string(ph - pb), sp[si++]);
         }
-        for (int oi = P.phases[ph].op_begin; oi < P.phases[ph].op_end; ++oi) gen_op(g, P, P.ops[oi]);
+        const FPhase& f = P.phases[ph];
+        const std::vector<int> into = fold_targets(P, f);
+        std::vector<int> from(into.size(), -1);
+        for (size_t i = 0; i < into.size(); ++i)
+            if (into[i] >= 0) from[size_t(into[i])] = int(i);
+        for (int i = 0; i < f.op_end - f.op_begin; ++i) {
+            if (into[size_t(i)] >= 0) continue;  // folded into a later op
+            const FOp* fx = from[size_t(i)] >= 0 ? &P.ops[size_t(f.op_begin + from[size_t(i)])] : nullptr;
+            gen_op(g, P, P.ops[size_t(f.op_begin + i)], fx);
+        }
     }
     if (!direct_store) gen_remap(g, lastp, st, "s_" + std::to_string(pe - 1 - pb), "s_st", sp[si++]);
     o << "    T* q = a + (base | " << thr_g(st) << ");\n";
