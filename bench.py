@@ -1058,6 +1058,8 @@ def bench_circuit(args, qa, n, depth, primary):
     if args.cpu_gates != 0:
         if n <= 22:
             picks = list(range(len(gates)))
+        elif args.cpu_gates > 0:  # the circuit's first N gates (45: one whole layer at 30 qubits)
+            picks = list(range(min(args.cpu_gates, len(gates))))
         else:  # dense gates on high and low qubits, and CNOTs, of layer 0
             picks = [0, 1, n - 2, n - 1, n, n + 1]
         log(f"cpu baseline: {len(picks)} gates at {n} qubits ...")
@@ -1147,8 +1149,9 @@ def main():
                     help="time the streaming copy of the same state as the HBM ceiling")
     ap.add_argument("--cpu-gates", type=int, default=-1,
                     help="0 skips the CPU baseline; otherwise it runs the whole circuit up "
-                         "to 22 qubits, else 4 dense gates and 2 CNOTs of the first layer, "
-                         "extrapolated per gate kind")
+                         "to 22 qubits, else (-1) 4 dense gates and 2 CNOTs of the first "
+                         "layer or (N > 0) the first N gates (45: one whole layer at 30 "
+                         "qubits, about 3 minutes), extrapolated per gate kind")
     ap.add_argument("--precision", type=int, default=1,
                     help="fused mode: also run the circuit in complex64 and compare with complex128 (1/0)")
     ap.add_argument("--reg-bits", type=int, default=4)
