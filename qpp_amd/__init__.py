@@ -1000,6 +1000,15 @@ class BitProgram:
         """See Program.native_stats."""
         return _native_stats(self._h)
 
+    def run_timed(self, st):
+        """See Program.run_timed."""
+        L = lib()
+        L.qs_program_run_timed.argtypes = [c_void_p, c_void_p, POINTER(c_double)]
+        np_ = self.info()["passes"]
+        ms = np.zeros(max(1, np_))
+        _check(L.qs_program_run_timed(self._h, st.handle, ms.ctypes.data_as(POINTER(c_double))))
+        return ms[:np_]
+
     def info(self):
         g, p, h = c_uint64(), c_uint64(), c_uint64()
         _check(lib().qs_program_info(self._h, byref(g), byref(p), byref(h)))

@@ -602,13 +602,15 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
 
 // How a layout change moves the tile through LDS.  Whole: every thread
 // writes its 16 registers, then reads 16 (the tile's 2^12 amplitudes in LDS
-// at once).  Split on a tile bit h >= 4: the amplitudes with bit h = 0 go
-// first, then those with h = 1, each half through the same 2^11 slots (the
-// swizzled index with bit h removed; the swizzle leaves bits >= 4 alone, so
-// this is a bijection on each half and keeps the bank pattern).  h is
-//  - a thread bit at the same position j >= 4 in both layouts: the threads
-//    with bit j = k move all their registers in round k (whole waves for
-//    j >= 6, half the lanes for j = 4, 5), or
+// at once).  Split on a tile bit h: the amplitudes with bit h = 0 go first,
+// then those with h = 1, each half through the same 2^11 slots: the
+// swizzled index with bit h removed.  That is a bijection on each half (the
+// swizzle is linear and maps the unit vector of h to itself for h < 4 and
+// keeps bit h for h >= 4, so two indices of one half never meet); for
+// h >= 4 it keeps the bank pattern, for h < 4 lanes may meet in a bank.  h is
+//  - a thread bit at the same position j in both layouts: the threads with
+//    bit j = k move all their registers in round k (whole waves for j >= 6,
+//    half the lanes for j < 6), or
 //  - a register bit of both layouts: every thread moves the 8 register
 //    groups with h = k in round k, and the groups read land in the
 //    registers just written (a renaming).
@@ -621,29 +623,25 @@ struct Split {
 
 Split find_split(const FPhase& A, const FPhase& B) {
     Split r;
-    for (int j = 7; j >= 6; --j)
-        if (A.tbits[j] == B.tbits[j] && A.tbits[j] >= 4) {
-            r.kind = 1;
-            r.h = A.tbits[j];
-            r.j = j;
-            return r;
-        }
+    auto thread_bit = [&](int j) {
+        if (A.tbits[j] != B.tbits[j]) return false;
+        r.kind = 1;
+        r.h = A.tbits[j];
+        r.j = j;
+        return true;
+    };
+    if (thread_bit(7) || thread_bit(6)) return r;  // whole waves skip a round
     for (int kf = 3; kf >= 0; --kf)
         for (int kt = 0; kt < 4; ++kt)
-            if (A.regbits[kf] == B.regbits[kt] && A.regbits[kf] >= 4) {
+            if (A.regbits[kf] == B.regbits[kt]) {
                 r.kind = 2;
                 r.h = A.regbits[kf];
                 r.kf = kf;
                 r.kt = kt;
                 return r;
             }
-    for (int j = 5; j >= 4; --j)
-        if (A.tbits[j] == B.tbits[j] && A.tbits[j] >= 4) {
-            r.kind = 1;
-            r.h = A.tbits[j];
-            r.j = j;
-            return r;
-        }
+    for (int j = 5; j >= 0; --j)  // half the lanes of every wave
+        if (thread_bit(j)) return r;
     return r;
 }
 

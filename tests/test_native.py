@@ -113,20 +113,23 @@ def test_native_full_size_circuit_and_inverse(qa):
 
 
 def test_native_split_layout_changes_vs_oracle(qa, orc):
-    """Layout changes moved whole and split in halves on a wave bit, a lane
-    bit and a register bit (the passes whose changes all split run at half
-    the LDS): circuits where every form occurs, against the oracle."""
+    """Layout changes split in halves on a wave bit, a lane bit and a
+    register bit (complex128; the passes whose changes all split run at half
+    the LDS) and moved whole (complex64 keeps whole changes): circuits where
+    every form occurs, against the oracle."""
     from qpp_amd import workloads
     seen = dict.fromkeys(("whole", "wave_split", "lane_split", "register_split", "half_lds_passes"), 0)
-    for n, seed, low, maxops in ((13, 1, 3, 16), (14, 2, 3, 48), (16, 1, 5, 48)):
+    for n, seed, low, maxops, dt in ((13, 1, 3, 16, qa.C128), (14, 2, 3, 48, qa.C128),
+                                     (16, 1, 5, 48, qa.C128), (14, 3, 3, 48, qa.C64)):
         qc = qa.QCircuit.from_gates(n, workloads.random_layers(n, 12, seed))
         psi = randket(2 ** n, np.random.default_rng(seed + 50))
-        st = qa.State.from_array(psi, [2] * n)
-        p = qa.Program(qc, low_bits=low, max_ops=maxops).compile_native()
+        st = qa.State.from_array(psi, [2] * n, dtype=dt)
+        p = qa.Program(qc, dtype=dt, low_bits=low, max_ops=maxops).compile_native()
         stats = p.native_stats()
-        print(f"native splits n={n} seed={seed} low={low} max_ops={maxops}: {stats}", flush=True)
+        print(f"native splits n={n} seed={seed} low={low} max_ops={maxops} dtype={dt}: {stats}", flush=True)
         p.run(st)
-        assert maxabs(st.download(), oracle_run(orc, psi, qc)) <= TOL, (n, seed, stats)
+        tol = TOL if dt == qa.C128 else 5e-6
+        assert maxabs(st.download().astype(np.complex128), oracle_run(orc, psi, qc)) <= tol, (n, seed, stats)
         for k in seen:
             seen[k] += stats[k]
     assert all(v > 0 for v in seen.values()), seen
