@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <array>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -71,6 +72,12 @@ struct Gen {
 };
 
 std::string u64(uint64_t v) { return std::to_string(v) + "ull"; }
+
+// +1 or -1 for a diagonal entry that is exactly that, else 0
+int sign_of(double re, double im) {
+    if (im != 0.0) return 0;
+    return re == 1.0 ? 1 : re == -1.0 ? -1 : 0;
+}
 
 // thread part of a phase's global index (single-bit offsets per thread bit)
 std::string thr_g(const FPhase& ph) {
@@ -522,6 +529,19 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
         }
         case OP_DIAG_R: {
             const int J = op.j0;
+            const int s0 = sign_of(op.m[0], op.m[1]), s1 = sign_of(op.m[2], op.m[3]);
+            if (s0 && s1) {
+                // entries +-1 (Z, CZ): a factor 1 is nothing, -1 flips the
+                // sign bits (on the lanes the op's lane condition admits)
+                if (has_t) o << "    const unsigned nm = tk ? 0x80000000u : 0u;\n";
+                for (int rho = 0; rho < kNR; ++rho) {
+                    if (!rok(rho) || ((rho >> J) & 1 ? s1 : s0) > 0) continue;
+                    const std::string m = has_t ? "nm" : "0x80000000u";
+                    o << "    " << g.X(rho) << " = mk(flip(" << g.X(rho) << ".x, " << m << "), flip(" << g.X(rho)
+                      << ".y, " << m << "));\n";
+                }
+                break;
+            }
             o << "    const T d0 = " << g.cx(op.m[0], op.m[1]) << ", d1 = " << g.cx(op.m[2], op.m[3]) << ";\n";
             for (int rho = 0; rho < kNR; ++rho) {
                 if (!rok(rho)) continue;
@@ -531,6 +551,22 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
         }
         case OP_DIAG_T:
         case OP_DIAG_O: {
+            const int s0 = sign_of(op.m[0], op.m[1]), s1 = sign_of(op.m[2], op.m[3]);
+            if (s0 && s1) {
+                // a factor +-1 per lane (thread bit) or per tile (bit outside
+                // the tile): one sign mask, then the flips
+                const std::string m0 = s0 < 0 ? "0x80000000u" : "0u", m1 = s1 < 0 ? "0x80000000u" : "0u";
+                if (op.kind == OP_DIAG_T)
+                    o << "    unsigned nm = (t & " << op.tbit << "u) != 0u ? " << m1 << " : " << m0 << ";\n";
+                else
+                    o << "    unsigned nm = (base & " << u64(op.dbit) << ") ? " << m1 << " : " << m0 << ";\n";
+                if (has_t) o << "    nm = tk ? nm : 0u;\n";
+                for (int rho = 0; rho < kNR; ++rho) {
+                    if (!rok(rho)) continue;
+                    o << "    " << g.X(rho) << " = mk(flip(" << g.X(rho) << ".x, nm), flip(" << g.X(rho) << ".y, nm));\n";
+                }
+                break;
+            }
             o << "    const T d0 = " << g.cx(op.m[0], op.m[1]) << ", d1 = " << g.cx(op.m[2], op.m[3]) << ";\n";
             if (op.kind == OP_DIAG_T)
                 o << "    const T dd = sel((t & " << op.tbit << "u) != 0u, d1, d0);\n";
@@ -880,6 +916,13 @@ __device__ __forceinline__ T mac4(T u0, T a, T u1, T b, T u2, T c, T u3, T d) {
 }
 __device__ __forceinline__ T cmul(T u, T a) { return mk(u.x * a.x - u.y * a.y, u.x * a.y + u.y * a.x); }
 __device__ __forceinline__ T sel(bool c, T a, T b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
+// the sign bit of v XOR bit 31 of m (a factor -1 where m has it)
+__device__ __forceinline__ double flip(double v, unsigned m) {
+    return __longlong_as_double(__double_as_longlong(v) ^ ((long long)(unsigned long long)m << 32));
+}
+__device__ __forceinline__ float flip(float v, unsigned m) {
+    return __int_as_float(__float_as_int(v) ^ (int)m);
+}
 )";
 
 }  // namespace
@@ -979,7 +1022,15 @@ int native_build(FusedProgram& P, qs_dtype dt) {
     std::string pre;
     std::vector<uint32_t> lds;
     uint64_t stats[5];
-    (void)native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre, &lds, stats);
+    const std::string whole = native_source(native_fuse(P), dt, &tab, &off, &psrc, &pre, &lds, stats);
+    // QPP_AMD_NATIVE_DUMP=<file>: write the generated source there (for
+    // reading the kernels; appended, one program after another)
+    if (const char* dump = std::getenv("QPP_AMD_NATIVE_DUMP")) {
+        if (FILE* f = std::fopen(dump, "a")) {
+            std::fputs(whole.c_str(), f);
+            std::fclose(f);
+        }
+    }
     // groups of passes of about equal source size, one per thread
     const size_t np = P.passes.size();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());

@@ -133,3 +133,33 @@ def test_native_split_layout_changes_vs_oracle(qa, orc):
         for k in seen:
             seen[k] += stats[k]
     assert all(v > 0 for v in seen.values()), seen
+
+
+def test_native_sign_diagonals_vs_oracle(qa, orc):
+    """Diagonal entries of +-1 (Z, CZ, multi-controlled Z) become sign flips:
+    on register bits, under lane and tile conditions, and as per-lane and
+    per-tile factors (targets on thread bits and outside the tile), for
+    complex128 and complex64, against the oracle."""
+    from qpp_amd import workloads
+    Z = np.diag([1, -1]).astype(complex)
+    for n, seed, dt in ((14, 5, qa.C128), (16, 6, qa.C128), (15, 7, qa.C64)):
+        rng = np.random.default_rng(seed)
+        gates = []
+        for kind, t, c, U in workloads.random_layers(n, 8, seed):
+            if c:
+                gates.append(("cz", t, c, Z))
+            elif rng.random() < 0.3:
+                gates.append(("z", t, [], Z))
+            else:
+                gates.append((kind, t, c, U))
+        for _ in range(n):  # doubly controlled Z on random qubits
+            q = [int(v) for v in rng.choice(n, 3, replace=False)]
+            gates.append(("ccz", [q[0]], q[1:], Z))
+        qc = qa.QCircuit.from_gates(n, gates)
+        psi = randket(2 ** n, rng)
+        st = qa.State.from_array(psi, [2] * n, dtype=dt)
+        p = qa.Program(qc, dtype=dt).compile_native()
+        print(f"native sign diagonals n={n} dtype={dt}: {p.info()}", flush=True)
+        p.run(st)
+        tol = TOL if dt == qa.C128 else 5e-6
+        assert maxabs(st.download().astype(np.complex128), oracle_run(orc, psi, qc)) <= tol, (n, seed)
