@@ -19,9 +19,15 @@ enum FOpKind : int {
     OP_DIAG_T = 3,  // diag(d0, d1) on a thread-number bit (tbit)
     OP_DIAG_O = 4,  // diag(d0, d1) on a bit outside the tile (dbit)
     OP_U2 = 5,      // 4x4 dense on register bits j0 < j1 (j1 = gate MSB)
-    OP_U1B = 6      // uncontrolled 2x2 dense on each register bit k set in
+    OP_U1B = 6,     // uncontrolled 2x2 dense on each register bit k set in
                     // tbit, matrices at mats + moff + 4k (commuting ops
                     // grouped so the kernel applies them in one straight run)
+    // specialised kernels only (made by native_fuse, never uploaded): a
+    // product on register bits that holds an X under a lane condition
+    // (tmask, tval) or a tile condition (omask, oval); mats + moff holds the
+    // product where the condition fails, then the one where it holds
+    OP_U1S = 7,     // 2x2 on j0: mats + moff, + 4
+    OP_U2S = 8      // 4x4 on j0 < j1: mats + moff, + 16
 };
 
 // One gate of a phase.  Its control (global mask, value) is split by where

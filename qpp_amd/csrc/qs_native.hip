@@ -192,7 +192,21 @@ struct Pending {
     bool diag = true;         // single: every op diagonal
     int cost = 0;             // the held ops' FP64 cost
     std::vector<FOp> held;    // the held ops, in order
+    // an X under a lane or tile condition taken in: s / m hold the product
+    // where the condition fails, s1 / m1 where it holds
+    bool cond = false;
+    uint32_t tmask = 0, tval = 0;
+    uint64_t omask = 0, oval = 0;
+    M2 s1{};
+    M4 m1{};
 };
+
+bool same_cond(const Pending& p, const FOp& x) {
+    return p.tmask == x.tmask && p.tval == x.tval && p.omask == x.omask && p.oval == x.oval;
+}
+bool same_cond(const Pending& p, const Pending& q) {
+    return p.tmask == q.tmask && p.tval == q.tval && p.omask == q.omask && p.oval == q.oval;
+}
 
 // the 2x2 an uncontrolled 1-bit op applies (U1, DIAG_R, X)
 M2 op_m2(const FOp& o) {
@@ -226,8 +240,8 @@ uint32_t regok_bits(const FOp& o) {
 uint32_t op_bits(const FOp& o) {
     uint32_t m = regok_bits(o);
     switch (o.kind) {
-        case OP_U1: case OP_X: case OP_DIAG_R: m |= 1u << o.j0; break;
-        case OP_U2: m |= (1u << o.j0) | (1u << o.j1); break;
+        case OP_U1: case OP_X: case OP_DIAG_R: case OP_U1S: m |= 1u << o.j0; break;
+        case OP_U2: case OP_U2S: m |= (1u << o.j0) | (1u << o.j1); break;
         case OP_U1B: m |= 15u; break;
         default: break;
     }
@@ -250,6 +264,32 @@ struct Fuser {
         pend.erase(pend.begin() + long(i));
         FOp o{};
         o.regok = 0xFFFFu;
+        if (p.cond) {
+            // a product with an X under a condition: one select op when it
+            // costs no more FP64 than the held ops (it also saves the X's
+            // register swap); a lone X stays an X.  Pairs only take tile
+            // conditions (16 entries selected per lane would take 64 more
+            // VGPRs than a kernel at 4 waves per SIMD has)
+            if (p.cost == 0 || (p.pair && p.cost < kCostU2)) {
+                out.insert(out.end(), p.held.begin(), p.held.end());
+                return;
+            }
+            o.tmask = p.tmask, o.tval = p.tval, o.omask = p.omask, o.oval = p.oval;
+            o.j0 = p.a;
+            o.moff = static_cast<uint32_t>(mats.size());
+            if (!p.pair) {
+                o.kind = OP_U1S;
+                mats.insert(mats.end(), p.s.begin(), p.s.end());
+                mats.insert(mats.end(), p.s1.begin(), p.s1.end());
+            } else {
+                o.kind = OP_U2S;
+                o.j1 = p.b;
+                mats.insert(mats.end(), p.m.begin(), p.m.end());
+                mats.insert(mats.end(), p.m1.begin(), p.m1.end());
+            }
+            out.push_back(o);
+            return;
+        }
         if (!p.pair) {
             const int cost = p.diag ? kCostDiag : kCostU1;
             if (p.held.size() == 1 || cost >= p.cost) {
@@ -297,6 +337,7 @@ struct Fuser {
         if (ia >= 0 && ia == ib && pend[ia].pair) {
             Pending& p = pend[ia];
             p.m = m4_mul(G, p.m);
+            if (p.cond) p.m1 = m4_mul(G, p.m1);
             p.cost += op_cost(o);
             p.held.push_back(o);
             return;
@@ -307,26 +348,42 @@ struct Fuser {
         }
         ib = find(b);
         if (ib >= 0 && pend[ib].pair) emit(ib);
+        // singles under different conditions cannot share a product, and a
+        // single under a lane condition goes out before a pair forms
+        ia = find(a);
+        if (ia >= 0 && pend[size_t(ia)].cond && pend[size_t(ia)].tmask != 0) emit(size_t(ia));
+        ib = find(b);
+        if (ib >= 0 && pend[size_t(ib)].cond && pend[size_t(ib)].tmask != 0) emit(size_t(ib));
+        ia = find(a);
+        ib = find(b);
+        if (ia >= 0 && ib >= 0 && pend[ia].cond && pend[ib].cond && !same_cond(pend[ia], pend[ib])) {
+            emit(size_t(ib));
+            ia = find(a);
+        }
         Pending np;
         np.pair = true;
         np.a = a;
         np.b = b;
         np.m = m4_id();
+        np.m1 = m4_id();
+        auto take = [&](int idx, int which) {
+            Pending& q = pend[size_t(idx)];
+            if (q.cond && !np.cond) {
+                np.cond = true;
+                np.tmask = q.tmask, np.tval = q.tval, np.omask = q.omask, np.oval = q.oval;
+            }
+            np.m = m4_mul(m4_on(q.s, which), np.m);
+            np.m1 = m4_mul(m4_on(q.cond ? q.s1 : q.s, which), np.m1);
+            np.cost += q.cost;
+            np.held.insert(np.held.end(), q.held.begin(), q.held.end());
+            pend.erase(pend.begin() + idx);
+        };
         ia = find(a);
-        if (ia >= 0) {
-            np.m = m4_mul(m4_on(pend[ia].s, 0), np.m);
-            np.cost += pend[ia].cost;
-            np.held.insert(np.held.end(), pend[ia].held.begin(), pend[ia].held.end());
-            pend.erase(pend.begin() + ia);
-        }
+        if (ia >= 0) take(ia, 0);
         ib = find(b);
-        if (ib >= 0) {
-            np.m = m4_mul(m4_on(pend[ib].s, 1), np.m);
-            np.cost += pend[ib].cost;
-            np.held.insert(np.held.end(), pend[ib].held.begin(), pend[ib].held.end());
-            pend.erase(pend.begin() + ib);
-        }
+        if (ib >= 0) take(ib, 1);
         np.m = m4_mul(G, np.m);
+        np.m1 = m4_mul(G, np.m1);
         np.cost += op_cost(o);
         np.held.push_back(o);
         pend.push_back(std::move(np));
@@ -346,8 +403,47 @@ struct Fuser {
             }
             return;
         }
-        const bool plain = o.tmask == 0 && o.omask == 0;
         const uint32_t rb = regok_bits(o);
+        // an X whose only control is a lane or a tile condition joins the
+        // product on its bit as a second product (where the condition holds)
+        if (o.kind == OP_X && rb == 0 && (o.regok & 0xFFFFu) == 0xFFFFu && (o.tmask != 0) != (o.omask != 0)) {
+            const int k = o.j0;
+            int i = find(k);
+            // (a pair under a lane condition would select 16 entries per
+            // lane: the pair goes out first and the X starts a single)
+            if (i >= 0 && ((pend[size_t(i)].cond && !same_cond(pend[size_t(i)], o)) ||
+                           (pend[size_t(i)].pair && o.tmask != 0))) {
+                emit(size_t(i));
+                i = -1;
+            }
+            const M2 X = op_m2(o);
+            if (i < 0) {
+                Pending p;
+                p.a = k;
+                p.s = {C(1, 0), C(0, 0), C(0, 0), C(1, 0)};
+                p.s1 = X;
+                p.diag = false;
+                p.cond = true;
+                p.tmask = o.tmask, p.tval = o.tval, p.omask = o.omask, p.oval = o.oval;
+                p.held.push_back(o);
+                pend.push_back(std::move(p));
+                return;
+            }
+            Pending& p = pend[size_t(i)];
+            if (!p.pair) {
+                p.s1 = m2_mul(X, p.cond ? p.s1 : p.s);
+                p.diag = false;
+            } else {
+                p.m1 = m4_mul(m4_on(X, k == p.a ? 0 : 1), p.cond ? p.m1 : p.m);
+            }
+            if (!p.cond) {
+                p.cond = true;
+                p.tmask = o.tmask, p.tval = o.tval, p.omask = o.omask, p.oval = o.oval;
+            }
+            p.held.push_back(o);
+            return;
+        }
+        const bool plain = o.tmask == 0 && o.omask == 0;
         const bool one_bit = o.kind == OP_U1 || o.kind == OP_X || o.kind == OP_DIAG_R;
         if (plain && one_bit && rb == 0 && (o.regok & 0xFFFFu) == 0xFFFFu) {
             const int j = o.j0;
@@ -360,12 +456,14 @@ struct Fuser {
             if (i >= 0 && pend[i].pair) {
                 Pending& p = pend[i];
                 p.m = m4_mul(m4_on(U, j == p.a ? 0 : 1), p.m);
+                if (p.cond) p.m1 = m4_mul(m4_on(U, j == p.a ? 0 : 1), p.m1);
                 p.cost += op_cost(o);
                 p.held.push_back(o);
             } else if (i >= 0) {
                 Pending& p = pend[i];
                 p.s = m2_mul(U, p.s);
-                p.diag = p.diag && o.kind == OP_DIAG_R;
+                if (p.cond) p.s1 = m2_mul(U, p.s1);
+                p.diag = p.diag && o.kind == OP_DIAG_R && !p.cond;
                 p.cost += op_cost(o);
                 p.held.push_back(o);
             } else {
@@ -448,10 +546,13 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
     std::ostringstream& o = g.o;
     const uint32_t R = op.regok;
     const bool ct = op.ctrl != 0;
-    const bool has_t = op.tmask != 0;
+    // OP_U1S / OP_U2S: tmask / omask pick one of two products, not a control
+    const bool selk = op.kind == OP_U1S || op.kind == OP_U2S;
+    const bool has_t = op.tmask != 0 && !selk;
+    const bool has_o = op.omask != 0 && !selk;
     auto rok = [&](int rho) { return !ct || ((R >> rho) & 1u); };
     o << "    {\n";
-    if (op.omask) o << "    if ((base & " << u64(op.omask) << ") == " << u64(op.oval) << ") {\n";
+    if (has_o) o << "    if ((base & " << u64(op.omask) << ") == " << u64(op.oval) << ") {\n";
     if (has_t) o << "    const bool tk = (t & " << op.tmask << "u) == " << op.tval << "u;\n";
     auto put = [&](int rho, const std::string& v) {
         if (has_t)
@@ -506,6 +607,30 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
                 break;
             }
             u1(op.j0, "u00", "u10", "u01", "u11", true);
+            break;
+        }
+        case OP_U1S: {
+            // the product where the condition fails, then where it holds
+            const C* M = P.mats.data() + op.moff;
+            const size_t i0 = g.tab.size();
+            for (int i = 0; i < 8; ++i) (void)g.cx(M[i]);
+            if (op.omask) {  // a tile condition: a uniform offset into the table
+                o << "    const int so = (base & " << u64(op.omask) << ") == " << u64(op.oval) << " ? 8 : 0;\n";
+                auto at = [&](int k) {
+                    return "mk(cm[" + std::to_string(i0 + 2 * k) + " + so], cm[" + std::to_string(i0 + 2 * k + 1) + " + so])";
+                };
+                o << "    const T u00 = " << at(0) << ", u10 = " << at(1) << ", u01 = " << at(2) << ", u11 = " << at(3)
+                  << ";\n";
+            } else {  // a lane condition: a select per entry
+                auto at = [&](int k) {
+                    return "mk(cm[" + std::to_string(i0 + 2 * k) + "], cm[" + std::to_string(i0 + 2 * k + 1) + "])";
+                };
+                o << "    const bool xk = (t & " << op.tmask << "u) == " << op.tval << "u;\n";
+                o << "    const T u00 = sel(xk, " << at(4) << ", " << at(0) << "), u10 = sel(xk, " << at(5) << ", "
+                  << at(1) << "), u01 = sel(xk, " << at(6) << ", " << at(2) << "), u11 = sel(xk, " << at(7) << ", "
+                  << at(3) << ");\n";
+            }
+            u1(op.j0, "u00", "u10", "u01", "u11", false);
             break;
         }
         case OP_X: {
@@ -578,10 +703,31 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
             }
             break;
         }
-        case OP_U2: {
+        case OP_U2:
+        case OP_U2S: {
             const int J0 = op.j0, J1 = op.j1;
             const C* M = P.mats.data() + op.moff;
-            if (fx && fx->omask) {
+            if (op.kind == OP_U2S) {
+                // the product where the condition fails, then where it holds
+                const size_t i0 = g.tab.size();
+                for (int i = 0; i < 32; ++i) (void)g.cx(M[i]);
+                auto at = [&](size_t k, const char* off) {
+                    return "mk(cm[" + std::to_string(i0 + 2 * k) + off + "], cm[" + std::to_string(i0 + 2 * k + 1) +
+                           off + "])";
+                };
+                if (op.omask) {
+                    o << "    const int so = (base & " << u64(op.omask) << ") == " << u64(op.oval) << " ? 32 : 0;\n";
+                    o << "    const T m[16] = {";
+                    for (size_t i = 0; i < 16; ++i) o << (i ? ", " : "") << at(i, " + so");
+                    o << "};\n";
+                } else {
+                    o << "    const bool xk = (t & " << op.tmask << "u) == " << op.tval << "u;\n";
+                    o << "    const T m[16] = {";
+                    for (size_t i = 0; i < 16; ++i)
+                        o << (i ? ", " : "") << "sel(xk, " << at(i + 16, "") << ", " << at(i, "") << ")";
+                    o << "};\n";
+                }
+            } else if (fx && fx->omask) {
                 // M and M (X on the folded bit) side by side, picked by the
                 // tile's condition
                 const int bk = fx->j0 == J0 ? 1 : 2;
@@ -630,7 +776,7 @@ void gen_op(Gen& g, const FusedProgram& P, const FOp& op, const FOp* fx = nullpt
         default:
             break;
     }
-    if (op.omask) o << "    }\n";
+    if (has_o) o << "    }\n";
     o << "    }\n";
     // each op's constants are loaded next to its arithmetic
     o << "    __builtin_amdgcn_sched_barrier(0);\n";
