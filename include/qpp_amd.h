@@ -385,6 +385,12 @@ int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates
                            const double* mats, qs_dtype dtype,
                            const qs_fuse_opts* opts, qs_program** out);
 int qs_program_run(qs_program* p, qs_state* st);
+/* No qpp counterpart: capture the program's pass launches for this state's
+ * buffer on its stream as a HIP graph (the passes run once, as in
+ * qs_program_run).  Later qs_program_run calls on the same buffer and
+ * stream replay the graph (one launch); other states run the passes
+ * directly.  A later capture or qs_program_compile_native drops it. */
+int qs_program_capture(qs_program* p, qs_state* st);
 /* No qpp counterpart: generate the program's passes as specialised kernels
  * (matrices and layouts as literals, register-only X gates as renamings, no
  * op dispatch), compile them with hipRTC for gfx950 and use them in later

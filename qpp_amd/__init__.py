@@ -867,6 +867,15 @@ class Program:
         self._native = True
         return self
 
+    def capture(self, st):
+        """Run once while capturing the pass launches as a HIP graph for this
+        state's buffer and stream (qs_program_capture); later runs on the
+        same state replay the graph."""
+        L = lib()
+        L.qs_program_capture.argtypes = [c_void_p, c_void_p]
+        _check(L.qs_program_capture(self._h, st.handle))
+        return self
+
     def native_stats(self):
         """How the specialised kernels move their layout changes through LDS
         (qs_program_native_stats): counts of changes moved whole, split on a

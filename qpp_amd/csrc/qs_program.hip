@@ -14,7 +14,26 @@ struct qs_program {
     FusedProgram P;
     uint64_t ngates = 0;
     uint32_t grid_cap = 2048;
+    // the pass launches captured as a HIP graph for one state buffer on one
+    // stream (qs_program_capture); runs on that buffer and stream replay it
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    void* gd = nullptr;
+    hipStream_t gs = nullptr;
 };
+
+namespace {
+
+void drop_graph(qs_program* p) {
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    p->gd = nullptr;
+    p->gs = nullptr;
+}
+
+}  // namespace
 
 namespace {
 
@@ -178,11 +197,50 @@ int qs_program_create_bits(uint32_t nb, uint32_t ngates, const qs_bitgate* gates
 int qs_program_compile_native(qs_program* p) {
     if (!p) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
     qs::clear_error();
+    // a captured graph holds the old kernels (and native_build unloads
+    // earlier specialised ones): drop it
+    if (p->gexec && p->gs) (void)hipStreamSynchronize(p->gs);
+    drop_graph(p);
     return qs::native_build(p->P, p->P.dtype);
+}
+
+int qs_program_capture(qs_program* p, qs_state* st) {
+    constexpr const char* W = "qpp_amd::program";
+    if (!p || !st || !st->d) return fail(QS_ERR_INVALID_HANDLE, W);
+    if (st->dtype != p->P.dtype) return fail(QS_ERR_TYPE_MISMATCH, W, "dtype");
+    if (!st->qubits || st->nbits != p->P.nb) return fail(QS_ERR_DIMS_MISMATCH_CVECTOR, W, "state/program");
+    if (p->gexec && p->gs) (void)hipStreamSynchronize(p->gs);
+    drop_graph(p);
+    QS_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal), W);
+    const int rc = qs::fused_run(p->P, st, p->grid_cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st->stream, &g);  // always ends the capture
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : qs::fail_hip(e, W);
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return qs::fail_hip(ei, W);
+    }
+    p->graph = g;
+    p->gexec = x;
+    p->gd = st->d;
+    p->gs = st->stream;
+    // a capture records the launches without running them: run them once
+    QS_HIP(hipGraphLaunch(x, st->stream), W);
+    return QS_OK;
 }
 
 int qs_program_run(qs_program* p, qs_state* st) {
     if (!p || !st || !st->d) return fail(QS_ERR_INVALID_HANDLE, "qpp_amd::program");
+    if (p->gexec && st->d == p->gd && st->stream == p->gs && st->dtype == p->P.dtype && st->qubits &&
+        st->nbits == p->P.nb) {
+        QS_HIP(hipGraphLaunch(p->gexec, st->stream), "qpp_amd::program");
+        return QS_OK;
+    }
     return qs::fused_run(p->P, st, p->grid_cap);
 }
 
@@ -226,6 +284,8 @@ int qs_program_native_stats(const qs_program* p, uint64_t counts[5]) {
 
 int qs_program_destroy(qs_program* p) {
     if (!p) return QS_OK;
+    if (p->gexec && p->gs) (void)hipStreamSynchronize(p->gs);
+    drop_graph(p);
     qs::native_free(p->P);
     qs::fused_free(p->P);
     delete p;
