@@ -76,13 +76,22 @@ def test_rho12_first_two_layers_vs_oracle(orc):
     assert err <= 1e-12
 
 
-def test_thirty_qubit_closed_forms():
+@pytest.mark.parametrize("native", [False, True])
+def test_thirty_qubit_closed_forms(native):
+    """The general tile kernel and the specialised pass kernels (the
+    benchmark's path) against closed forms at 30 qubits, where a wrong
+    qubit-to-bit or tile-base mapping shows."""
     n = 30
     D = 2 ** n
     st = qa.State([2] * n)
     st.set_basis(0)
+
+    def program(gates):
+        p = qa.Program(n, gates, reg_bits=4, low_bits=3, max_ops=48)
+        return p.compile_native() if native else p
+
     ghz = [("u", [0], [], H)] + [("cx", [q + 1], [q], X) for q in range(n - 1)]
-    qa.Program(n, ghz, reg_bits=4, low_bits=3, max_ops=48).run(st)
+    program(ghz).run(st)
     a = st.download()
     nz = np.flatnonzero(a)
     assert list(nz) == [0, D - 1], nz[:8]
@@ -93,7 +102,7 @@ def test_thirty_qubit_closed_forms():
     U1 = [haar(2, rng) for _ in range(n)]
     U2 = [haar(2, rng) for _ in range(n)]
     layers = [("u", [q], [], U1[q]) for q in range(n)] + [("u", [q], [], U2[q]) for q in range(n)]
-    qa.Program(n, layers, reg_bits=4, low_bits=3, max_ops=48).run(st)
+    program(layers).run(st)
     assert abs(st.norm2() - 1) < 1e-12
     a = st.download()
     V = np.array([U2[q] @ U1[q] for q in range(n)])  # (n, 2, 2)
@@ -104,8 +113,8 @@ def test_thirty_qubit_closed_forms():
     col1 = V[np.arange(n)[None, :], bits, 1]
     expect = (np.prod(col0, axis=1) * H[0, 0] + np.prod(col1, axis=1) * H[1, 0])
     err = float(np.max(np.abs(a[idx] - expect)))
-    print(f"30-qubit GHZ + 2 Haar layers: max-abs over {len(idx)} amplitudes {err:.3e}",
-          flush=True)
+    print(f"30-qubit GHZ + 2 Haar layers (native={native}): max-abs over {len(idx)} "
+          f"amplitudes {err:.3e}", flush=True)
     assert err <= 1e-12
 
 
