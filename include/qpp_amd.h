@@ -310,13 +310,22 @@ typedef enum qs_step_type {
                                  all equal one value k; then U^k on the
                                  targets */
     QS_STEP_CCTRL_FAN = 14,   /* cCTRL_FAN: the same, U^k on each target */
-    QS_STEP_COND_WHILE = 15   /* runtime while (cond_while, qengine.hpp:837-880):
+    QS_STEP_COND_WHILE = 15,  /* runtime while (cond_while, qengine.hpp:837-880):
                                  the COND_IF condition; false jumps past the
                                  matching COND_END, whose visit re-evaluates it
                                  and, while true, goes back to the body.
                                  Destructive measurements, post-selections and
                                  discards inside a while are rejected
                                  (InvalidConditional, as QCircuit does) */
+    /* No qpp form of their own: COND_IF and COND_WHILE with the condition
+     * given as a truth table, the C-ABI form of qpp's predicate lambdas
+     * (cond_if(pred), qcircuit.hpp:1026; visit_runtime_step_,
+     * qengine.hpp:837-880) over the nctrl dits at ctrl_off: true when
+     * pool[ps_off + i] != 0, i = sum_j dit(ctrl[j]) * d^(nctrl-1-j).  The
+     * table has d^nctrl entries (at most 2^20); a dit outside [0, d) when the
+     * condition is evaluated is OutOfRange. */
+    QS_STEP_COND_IF_TABLE = 16,
+    QS_STEP_COND_WHILE_TABLE = 17
 } qs_step_type;
 
 typedef struct qs_step {

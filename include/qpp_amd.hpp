@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <functional>
 #include <cstdint>
 #include <map>
 #include <optional>
@@ -490,6 +491,17 @@ class QCircuit {
         steps_.back().type = QS_STEP_COND_WHILE;
         return *this;
     }
+    // qpp's predicate form (cond_if(pred) / cond_while(pred),
+    // qcircuit.hpp:968-1026): pred sees every classical dit, as qpp's dits
+    // proxy does; it is stored as a truth table over their values 0..d-1
+    // (d^nc entries, at most 2^20; a dit outside 0..d-1 when the condition
+    // is evaluated is OutOfRange)
+    QCircuit& cond_if(const std::function<bool(const std::vector<idx>&)>& pred) {
+        return cond_table(QS_STEP_COND_IF_TABLE, pred);
+    }
+    QCircuit& cond_while(const std::function<bool(const std::vector<idx>&)>& pred) {
+        return cond_table(QS_STEP_COND_WHILE_TABLE, pred);
+    }
     QCircuit& cond_else() { return marker(QS_STEP_COND_ELSE); }
     QCircuit& cond_end() { return marker(QS_STEP_COND_END); }
     // reset / discard (qcircuit.hpp:3745-3940)
@@ -566,6 +578,44 @@ class QCircuit {
         steps_.push_back(s);
         return *this;
     }
+    QCircuit& cond_table(uint32_t type, const std::function<bool(const std::vector<idx>&)>& pred) {
+        idx len = 1;
+        for (idx j = 0; j < nc_; ++j) {
+            len *= d_;
+            if (len > (idx{1} << 20))
+                throw std::invalid_argument("qpp_amd::QCircuit::cond_if(): more than 2^20 dit values");
+        }
+        std::vector<idx> dits(nc_), table(len), v(nc_, 0);
+        for (idx j = 0; j < nc_; ++j) dits[j] = j;
+        for (idx i = 0; i < len; ++i) {  // v = the digits of i, dit 0 most significant
+            idx r = i;
+            for (idx j = nc_; j-- > 0;) {
+                v[j] = r % d_;
+                r /= d_;
+            }
+            table[i] = pred(v) ? 1 : 0;
+        }
+        return cond_table(type, dits, table);
+    }
+
+  public:
+    // the truth-table step itself: table[i] over the values of `dits`,
+    // i = sum_j value(dits[j]) d^(k-1-j) (QS_STEP_COND_IF_TABLE /
+    // QS_STEP_COND_WHILE_TABLE)
+    QCircuit& cond_table(uint32_t type, const std::vector<idx>& dits, const std::vector<idx>& table) {
+        qs_step s{};
+        s.type = type;
+        s.nctrl = static_cast<uint32_t>(dits.size());
+        s.ctrl_off = static_cast<uint32_t>(pool_.size());
+        pool_.insert(pool_.end(), dits.begin(), dits.end());
+        s.ps_off = static_cast<uint32_t>(pool_.size());
+        pool_.insert(pool_.end(), table.begin(), table.end());
+        s.shift_off = UINT32_MAX;
+        steps_.push_back(s);
+        return *this;
+    }
+
+  private:
     idx nq_, nc_, d_;
     std::vector<qs_step> steps_;
     std::vector<uint64_t> dim_, off_, pool_;

@@ -980,7 +980,8 @@ struct Step {
     std::optional<std::vector<idx>> shift;
     idx c_reg;
     bool destructive;
-    std::vector<idx> ps;  // POST_SELECT values; COND_IF dit values
+    std::vector<idx> ps;  // POST_SELECT values; COND_IF dit values or truth table
+    bool table = false;   // COND_IF / WHILE: ps is a truth table over the ctrl dits
     idx jump = 0;         // COND_IF / WHILE: its ELSE, else its END; COND_ELSE:
                           // its END; the END of a WHILE: the WHILE
     bool loop_end = false;
@@ -1316,7 +1317,18 @@ struct Engine {
             st.qstate = apply_any(st.qstate, is_rho, U, trel, st.dims);
     }
 
+    // a truth table (qpp's predicate lambda evaluated on every value of
+    // the dits it reads, qengine.hpp:845-858) or "these dits equal these
+    // values"
     bool cond_true(const Step& c) const {
+        if (c.table) {
+            idx i = 0;
+            for (idx q : c.ctrl) {
+                if (st.dits[q] >= d) throw EngineError{QS_ERR_OUT_OF_RANGE};
+                i = i * d + st.dits[q];
+            }
+            return c.ps[i] != 0;
+        }
         for (idx j = 0; j < c.ctrl.size(); ++j)
             if (st.dits[c.ctrl[j]] != c.ps[j]) return false;
         return true;
@@ -1796,8 +1808,15 @@ int orc_engine_run_ps(const qs_circuit* qc, int is_rho, double noise_p,
         t.destructive = s.destructive != 0;
         if (s.type == QS_STEP_POST_SELECT)
             t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.ntarget);
-        if (s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_WHILE)
+        if (s.type == QS_STEP_COND_IF_TABLE || s.type == QS_STEP_COND_WHILE_TABLE) {
+            t.type = s.type == QS_STEP_COND_IF_TABLE ? QS_STEP_COND_IF : QS_STEP_COND_WHILE;
+            t.table = true;
+            idx len = 1;
+            for (uint32_t j = 0; j < s.nctrl; ++j) len *= qc->d;
+            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + len);
+        } else if (s.type == QS_STEP_COND_IF || s.type == QS_STEP_COND_WHILE) {
             t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.nctrl);
+        }
         e.steps.push_back(t);
     }
     // match IF / ELSE / END (QCircuit::validate_conditionals)

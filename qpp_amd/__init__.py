@@ -28,6 +28,7 @@ library has no CPU fallback: without a gfx950 device every compute call
 raises `NoDevice`.
 """
 import ctypes
+import itertools
 import math
 import os
 from ctypes import POINTER, byref, c_char_p, c_double, c_int, c_size_t, c_uint32, c_uint64, c_void_p
@@ -617,7 +618,8 @@ class Rng:
 # ---------------------------------------------------------------------------
 (STEP_NOP, STEP_GATE, STEP_FAN, STEP_CTRL, STEP_CTRL_FAN, STEP_MEASURE, STEP_MEASURE_MANY,
  STEP_RESET, STEP_DISCARD, STEP_POST_SELECT, STEP_COND_IF, STEP_COND_ELSE,
- STEP_COND_END, STEP_CCTRL, STEP_CCTRL_FAN, STEP_COND_WHILE) = range(16)
+ STEP_COND_END, STEP_CCTRL, STEP_CCTRL_FAN, STEP_COND_WHILE, STEP_COND_IF_TABLE,
+ STEP_COND_WHILE_TABLE) = range(18)
 
 
 class _Step(ctypes.Structure):
@@ -730,30 +732,40 @@ class QCircuit:
         self.steps.append((STEP_DISCARD, 0, [], t, None, 0, 1))
         return self
 
-    def cond_if(self, dits, values):
-        """QCircuit::cond_if (runtime conditional, qengine.hpp:837-860) with
-        the condition "dit dits[j] == values[j] for every j" -- the form
-        OpenQASM's if (c==n) takes.  Close with cond_end(), optionally after
-        cond_else()."""
+    def _cond(self, name, plain, table, dits, values):
         d = [int(dits)] if np.isscalar(dits) else list(dits)
+        if callable(values):
+            # qpp's predicate lambda (cond_if(pred), qcircuit.hpp:1026) as a
+            # truth table over the values 0..d-1 of the dits it reads:
+            # entry i = sum_j v_j d^(k-1-j) holds pred([v_0, ..., v_{k-1}])
+            if self.d ** len(d) > 1 << 20:
+                raise ValueError(f"{name}: a predicate reads at most 2^20 value combinations")
+            self.ps_vals[len(self.steps)] = [
+                int(bool(values(list(v)))) for v in itertools.product(range(self.d), repeat=len(d))]
+            self.steps.append((table, 0, d, [], None, 0, 0))
+            return self
         v = [int(values)] if np.isscalar(values) else list(values)
         if len(v) != len(d):
-            raise ValueError("cond_if: one value per dit")
+            raise ValueError(f"{name}: one value per dit")
         self.ps_vals[len(self.steps)] = v
-        self.steps.append((STEP_COND_IF, 0, d, [], None, 0, 0))
+        self.steps.append((plain, 0, d, [], None, 0, 0))
         return self
+
+    def cond_if(self, dits, values):
+        """QCircuit::cond_if (runtime conditional, qengine.hpp:837-860).
+        values: the condition "dit dits[j] == values[j] for every j" (the
+        form OpenQASM's if (c==n) takes), or a predicate pred(v) over the
+        list v of those dits' values (qpp's cond_if(pred); the dits must
+        hold values in 0..d-1 when the condition is evaluated, else the run
+        is OutOfRange).  Close with cond_end(), optionally after
+        cond_else()."""
+        return self._cond("cond_if", STEP_COND_IF, STEP_COND_IF_TABLE, dits, values)
 
     def cond_while(self, dits, values):
         """QCircuit::cond_while (qengine.hpp:837-880): repeat the steps up
-        to cond_end() while dit dits[j] == values[j] for every j.
+        to cond_end() while the condition holds (values as in cond_if).
         Destructive measurements and discards are not allowed inside."""
-        d = [int(dits)] if np.isscalar(dits) else list(dits)
-        v = [int(values)] if np.isscalar(values) else list(values)
-        if len(v) != len(d):
-            raise ValueError("cond_while: one value per dit")
-        self.ps_vals[len(self.steps)] = v
-        self.steps.append((STEP_COND_WHILE, 0, d, [], None, 0, 0))
-        return self
+        return self._cond("cond_while", STEP_COND_WHILE, STEP_COND_WHILE_TABLE, dits, values)
 
     def cond_else(self):
         self.steps.append((STEP_COND_ELSE, 0, [], [], None, 0, 0))

@@ -318,12 +318,27 @@ PYBIND11_MODULE(pyqpp, m) {
             "Runtime conditional: true when dit dits[j] equals vals[j] for every j",
             py::arg("dits"), py::arg("vals"), py::return_value_policy::reference)
         .def(
+            "cond_if",
+            [](QCircuit& c, const py::function& pred) -> QCircuit& {
+                return c.cond_if([&](const std::vector<idx>& v) { return pred(v).cast<bool>(); });
+            },
+            "Runtime conditional with pyqpp's predicate form: pred(dits) over the list of "
+            "every classical dit (stored as a truth table over the values 0..d-1)",
+            py::arg("pred"), py::return_value_policy::reference)
+        .def(
             "cond_while",
             [](QCircuit& c, const py::object& dits, const py::object& vals) -> QCircuit& {
                 return c.cond_while(idx_list(dits), idx_list(vals));
             },
             "Runtime while loop with the cond_if condition form", py::arg("dits"),
             py::arg("vals"), py::return_value_policy::reference)
+        .def(
+            "cond_while",
+            [](QCircuit& c, const py::function& pred) -> QCircuit& {
+                return c.cond_while([&](const std::vector<idx>& v) { return pred(v).cast<bool>(); });
+            },
+            "Runtime while loop with pyqpp's predicate form", py::arg("pred"),
+            py::return_value_policy::reference)
         .def("cond_else", &QCircuit::cond_else, py::return_value_policy::reference)
         .def("cond_end", &QCircuit::cond_end, py::return_value_policy::reference)
         .def("nop", &QCircuit::nop, py::return_value_policy::reference);
@@ -458,6 +473,10 @@ PYBIND11_MODULE(pyqpp, m) {
                 case QS_STEP_COND_IF: c.cond_if(ctrl, ps[py::int_(i)].cast<std::vector<idx>>()); break;
                 case QS_STEP_COND_WHILE:
                     c.cond_while(ctrl, ps[py::int_(i)].cast<std::vector<idx>>());
+                    break;
+                case QS_STEP_COND_IF_TABLE:
+                case QS_STEP_COND_WHILE_TABLE:
+                    c.cond_table(type, ctrl, ps[py::int_(i)].cast<std::vector<idx>>());
                     break;
                 case QS_STEP_COND_ELSE: c.cond_else(); break;
                 case QS_STEP_COND_END: c.cond_end(); break;

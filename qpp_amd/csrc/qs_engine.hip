@@ -63,7 +63,8 @@ struct EStep {
     bool has_shift = false;
     uint64_t c_reg = 0;
     bool destructive = true;
-    std::vector<uint64_t> ps;  // POST_SELECT values, COND_IF dit values
+    std::vector<uint64_t> ps;  // POST_SELECT values, COND_IF dit values or truth table
+    bool table = false;        // COND_IF / COND_WHILE: ps is a truth table over the ctrl dits
     size_t jump = 0;           // COND_IF / COND_WHILE: its COND_ELSE or else COND_END;
                                // COND_ELSE: its COND_END; COND_END of a while: the COND_WHILE
     bool loop_end = false;     // a COND_END that closes a COND_WHILE
@@ -187,10 +188,19 @@ struct qs_engine {
 
     // get_relative_pos_, qengine.hpp:514-526: a step on a destructively
     // measured qudit is QuditAlreadyMeasured
-    bool cond_true(const EStep& c) const {
+    // 1 true, 0 false, -1 a dit outside [0, d) under a truth table
+    int cond_true(const EStep& c) const {
+        if (c.table) {
+            uint64_t i = 0;
+            for (uint64_t q : c.ctrl) {
+                if (dits[q] >= d) return -1;
+                i = i * d + dits[q];
+            }
+            return c.ps[i] != 0;
+        }
         for (size_t j = 0; j < c.ctrl.size(); ++j)
-            if (dits[c.ctrl[j]] != c.ps[j]) return false;
-        return true;
+            if (dits[c.ctrl[j]] != c.ps[j]) return 0;
+        return 1;
     }
 
     bool on_measured(const EStep& x) const {
@@ -529,7 +539,12 @@ struct qs_engine {
             if (x.type == QS_STEP_COND_IF || x.type == QS_STEP_COND_WHILE) {
                 // visit_runtime_step_, qengine.hpp:837-860: on false, go on
                 // after the matching ELSE (or END)
-                if (!cond_true(x)) i = x.jump;
+                const int c = cond_true(x);
+                if (c < 0) {
+                    run_gates(pending, false);
+                    return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()", "condition dit outside [0, d)");
+                }
+                if (!c) i = x.jump;
             } else if (x.type == QS_STEP_COND_ELSE) {
                 i = x.jump;  // the true branch ends here (:862-866)
             } else if (x.type == QS_STEP_COND_END) {
@@ -538,7 +553,12 @@ struct qs_engine {
                 // never turns false would run for ever (as in qpp); it stops
                 // here once its body has run loop_give_up times in one run
                 // and the condition still holds.
-                if (x.loop_end && cond_true(s[x.jump])) {
+                const int c = x.loop_end ? cond_true(s[x.jump]) : 0;
+                if (c < 0) {
+                    run_gates(pending, false);
+                    return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()", "condition dit outside [0, d)");
+                }
+                if (c) {
                     if (++loops >= loop_give_up && loop_give_up) {
                         run_gates(pending, false);
                         return fail(QS_ERR_OUT_OF_RANGE, "qpp::QEngineT::execute()",
@@ -892,9 +912,14 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         t.c_reg = s.c_reg;
         t.destructive = s.destructive != 0;
-        if (s.type > QS_STEP_COND_WHILE) {
+        if (s.type > QS_STEP_COND_WHILE_TABLE) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, W, "step type");
+        }
+        if (s.type == QS_STEP_COND_IF_TABLE || s.type == QS_STEP_COND_WHILE_TABLE) {
+            // a truth-table condition runs as COND_IF / COND_WHILE
+            t.type = s.type == QS_STEP_COND_IF_TABLE ? QS_STEP_COND_IF : QS_STEP_COND_WHILE;
+            t.table = true;
         }
         if (is_meas(t) && t.c_reg + t.target.size() > qc->nc) {
             delete e;
@@ -909,8 +934,18 @@ int qs_engine_create(const qs_circuit* qc, qs_kind kind, qs_dtype dtype, double 
         }
         for (uint64_t q : t.ctrl) in_range = in_range && q < ctrl_lim;
         for (uint64_t q : t.target) in_range = in_range && q < qc->nq;
-        if (t.type == QS_STEP_COND_IF || t.type == QS_STEP_COND_WHILE)
-            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + s.nctrl);
+        if (t.type == QS_STEP_COND_IF || t.type == QS_STEP_COND_WHILE) {
+            uint64_t len = s.nctrl;
+            if (t.table) {
+                len = 1;
+                for (uint32_t j = 0; j < s.nctrl && len <= (uint64_t{1} << 20); ++j) len *= qc->d;
+                if (len > (uint64_t{1} << 20)) {
+                    delete e;
+                    return fail(QS_ERR_OUT_OF_RANGE, "qpp::QCircuit::cond_if()", "truth table above 2^20 entries");
+                }
+            }
+            t.ps.assign(qc->pool + s.ps_off, qc->pool + s.ps_off + len);
+        }
         if (!in_range) {
             delete e;
             return fail(QS_ERR_OUT_OF_RANGE, W, "ctrl/target");
