@@ -234,7 +234,9 @@ def bench_batch(args, qa):
         "config": {"workload": f"{ntraj} QNoisyEngine trajectories "
                                f"({', '.join(f'{max(1, int(c * scale))}x{n}q' for n, c in sizes)}), "
                                f"depth {depth}, depolarizing p={p}, measure all",
-                   "passes_per_circuit": dict(zip([f"{n}q" for n, _ in sizes], passes))},
+                   "passes_per_circuit": dict(zip([f"{n}q" for n, _ in sizes], passes)),
+                   "circuit_sha256": {f"{n}q": workloads.gates_digest(
+                       workloads.ladder_layers(n, depth, 300 + n)) for n, _ in sizes}},
         "plain_gpu_trajectories_per_s": round(ntraj / plain_sec, 3),
         "speedup_vs_plain": round(plain_sec / wall, 2),
         "host_plan_ms": round(inf["host_ms"], 2),
@@ -262,11 +264,9 @@ def bench_serve(args, qa):
     nreq = args.requests
 
     def make(k, r):
-        n = int(r.integers(10, 27))
-        depth = int(r.integers(10, 101))
-        dt = qa.C64 if r.random() < 0.5 else qa.C128
-        return {"n": n, "depth": depth, "dtype": dt, "seed": int(r.integers(1, 2 ** 31)),
-                "gseed": 7000 + k}
+        q = workloads.serve_request(k, r)
+        q["dtype"] = qa.C64 if q["c64"] else qa.C128
+        return q
 
     reqs = [make(k, rng) for k in range(nreq)]
     circs = [workloads.measured_circuit(qa, q["n"], q["depth"], q["gseed"]) for q in reqs]
@@ -395,7 +395,8 @@ def bench_serve(args, qa):
         "config": {"workload": f"{nreq} Poisson-arriving random circuits, n~U{{10..26}}, "
                                f"depth~U{{10..100}}, 1024 shots, c128/c64 50/50, "
                                f"{args.lanes} lanes",
-                   "arrival_rate_per_s": round(lam, 3), "capacity_per_s": round(cap, 3)},
+                   "arrival_rate_per_s": round(lam, 3), "capacity_per_s": round(cap, 3),
+                   "requests_sha256": workloads.requests_digest(reqs)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                        "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
@@ -533,6 +534,7 @@ def bench_rho(args, qa):
         "config": {"workload": f"rho 4096x4096, {depth} layers: {len(ops)} gates+channels fused "
                                f"into {npass} passes, ptrace {tr}, ptranspose {ptt}, syspermute",
                    "ops": len(ops), "passes": npass,
+                   "workload_sha256": workloads.dm_digest(rho0, ops, tr, ptt, perm),
                    "pass_kernels": (f"specialised per pass (hipRTC), compiled in {native_s:.1f} s "
                                     "outside the timed region") if native_s is not None
                                    else "general tile kernel k_tile"},
@@ -994,6 +996,7 @@ def bench_circuit(args, qa, n, depth, primary):
                         f"({n1q} Haar 1q + {ncx} CNOT), {'tile-fused passes' if mode == 'fused' else 'gate by gate'}",
             "qubits": n, "depth": depth, "gates": len(gates), "mode": mode,
             "passes_per_step": npasses, "seed": args.seed,
+            "circuit_sha256": workloads.gates_digest(gates),
             "fuse_opts": {"reg_bits": args.reg_bits, "low_bits": args.low_bits,
                           "max_ops": args.max_ops, "grid_cap": args.grid_cap,
                           "autotuned": tuned is not None},

@@ -9,6 +9,9 @@ Haar unitaries are QR decompositions of complex Gaussian matrices with the
 phases of R's diagonal divided out (the construction of qpp::randU,
 random.hpp:257-283), drawn from numpy's PCG64 with a fixed seed.
 """
+import hashlib
+import struct
+
 import numpy as np
 
 X = np.array([[0, 1], [1, 0]], dtype=complex)
@@ -158,3 +161,91 @@ def dm_bytes(n, ops, elem_bytes=16):
     """Algorithmic bytes of the ops applied one by one: every gate and every
     channel reads and writes rho once (2 s 4^n)."""
     return len(ops) * 2 * elem_bytes * 4 ** n
+
+
+def serve_request(k, rng):
+    """One serving request's parameters drawn from the numpy Generator
+    `rng`: n ~ U{10..26}, depth ~ U{10..100}, complex64 with probability 1/2
+    ("c64" True), an engine seed, and gseed = 7000 + k, the seed of the
+    request's measured_circuit."""
+    n = int(rng.integers(10, 27))
+    depth = int(rng.integers(10, 101))
+    c64 = bool(rng.random() < 0.5)
+    return {"n": n, "depth": depth, "c64": c64, "seed": int(rng.integers(1, 2 ** 31)),
+            "gseed": 7000 + k}
+
+
+def serve_requests(nreq, seed, first=0):
+    """`nreq` serve_request draws (k = first..first+nreq-1) from PCG64 with
+    `seed`."""
+    rng = np.random.default_rng(seed)
+    return [serve_request(first + k, rng) for k in range(nreq)]
+
+
+_KIND = {"u": 0, "cx": 1, "cz": 2, "kraus": 3}
+
+
+def _mats(M):
+    if isinstance(M, (list, tuple)):
+        return b"".join(np.ascontiguousarray(K, dtype="<c16").tobytes() for K in M)
+    return np.ascontiguousarray(M, dtype="<c16").tobytes()
+
+
+def gates_digest(gates):
+    """SHA-256 of a gate list (random_layers / dm_layers / ladder_layers):
+    per gate the kind code (u 0, cx 1, cz 2), the target and control lists
+    (counts as one byte, qubits as little-endian int32) and the matrix as
+    little-endian complex128, row-major."""
+    h = hashlib.sha256()
+    for kind, target, ctrl, U in gates:
+        h.update(struct.pack("<BB", _KIND[kind], len(target)))
+        h.update(struct.pack(f"<{len(target)}i", *target))
+        h.update(struct.pack("<B", len(ctrl)))
+        h.update(struct.pack(f"<{len(ctrl)}i", *ctrl))
+        h.update(_mats(U))
+    return h.hexdigest()
+
+
+def dm_digest(rho, ops, trace_out, pt_target, perm):
+    """SHA-256 of a dm_pipeline workload: rho's first row (it fixes the
+    seeded pure state up to a phase), then per op the kind code (u 0, cz 2,
+    kraus 3), its qubits and its matrices (little-endian complex128), then
+    the traced-out qubits, the ptranspose targets and the permutation."""
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(rho[0], dtype="<c16").tobytes())
+    for kind, q, M in ops:
+        h.update(struct.pack(f"<BB{len(q)}i", _KIND[kind], len(q), *q))
+        h.update(_mats(M))
+    for lst in (trace_out, pt_target, perm):
+        h.update(struct.pack(f"<B{len(lst)}i", len(lst), *lst))
+    return h.hexdigest()
+
+
+def requests_digest(reqs):
+    """SHA-256 of serving requests' parameters (n, depth, c64, seed, gseed per
+    request, as little-endian int64); each request's circuit follows from
+    random_layers(n, depth, gseed)."""
+    h = hashlib.sha256()
+    for q in reqs:
+        h.update(struct.pack("<5q", q["n"], q["depth"], int(q["c64"]), q["seed"], q["gseed"]))
+    return h.hexdigest()
+
+
+def save_gates(path, gates):
+    """Write a gate list of 1-target gates with at most one control as an
+    .npz of plain arrays (kind codes, targets, controls with -1 for none,
+    the 2x2 matrices as complex128); load_gates reads it back."""
+    kinds = np.array([_KIND[g[0]] for g in gates], dtype=np.uint8)
+    tg = np.array([g[1][0] for g in gates], dtype=np.int32)
+    ct = np.array([g[2][0] if g[2] else -1 for g in gates], dtype=np.int32)
+    mats = np.array([g[3] for g in gates], dtype=np.complex128)
+    np.savez_compressed(path, kinds=kinds, targets=tg, controls=ct, mats=mats)
+
+
+def load_gates(path):
+    """The gate list saved by save_gates, in the generators' form."""
+    names = {v: k for k, v in _KIND.items()}
+    with np.load(path) as f:
+        kinds, tg, ct, mats = f["kinds"], f["targets"], f["controls"], f["mats"]
+    return [(names[int(k)], [int(t)], [int(c)] if c >= 0 else [], mats[i].copy())
+            for i, (k, t, c) in enumerate(zip(kinds, tg, ct))]

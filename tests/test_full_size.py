@@ -1,14 +1,17 @@
 """The benchmark workloads at their real sizes, where the oracle still runs
 in seconds or where size-independent properties decide:
 
-* the 20-qubit complex128 random circuit of 20 layers, gate by gate and as
-  fused passes, against the oracle engine on the same gates;
+* the 20-qubit complex128 random circuit of 20 layers (read from the
+  committed fixture tests/golden/circuit_20q_depth20_seed20.npz), gate by
+  gate and as fused passes, against the oracle engine on the same gates;
 * the 12-qubit density-matrix pipeline (10 layers, a depolarizing channel
   after every gate, then ptrace / ptranspose / syspermute): fused passes
   against gate-by-gate application, unit trace and hermiticity of the
   result, and the three maps against the oracle on the device's own output
   (ptrace <= 1e-12, ptranspose and syspermute bit-exact).
 The 30-qubit circuit is covered by test_gpu_fused.py's forward/inverse test."""
+import os
+
 import numpy as np
 import pytest
 
@@ -20,7 +23,9 @@ pytestmark = pytest.mark.gpu
 
 def test_twenty_qubit_circuit(orc):
     n, depth = 20, 20
-    gates = workloads.random_layers(n, depth, seed=20)
+    gates = workloads.load_gates(os.path.join(os.path.dirname(__file__), "golden",
+                                              "circuit_20q_depth20_seed20.npz"))
+    assert len(gates) == depth * (n + n // 2)
     qc = qa.QCircuit.from_gates(n, gates)
     _, _, _, ref = orc.engine_run(qc, reps=1, seed=1)
     st = qa.State([2] * n)
