@@ -144,7 +144,9 @@ def bench_batch(args, qa):
     """Batched noisy trajectories: 4096 x 16 qubits + 512 x {12, 20, 22}
     qubits, depth 30 (Haar 1q on every qubit + CNOT ladder), depolarizing
     p = 1e-3 before every step on every qubit (QNoisyEngine), measurement of
-    every qubit; one persistent launch per run."""
+    every qubit; per run the 20- and 22-qubit members' passes (one launch
+    per pass and circuit, a workgroup per tile) and one persistent launch
+    for the rest and every measurement."""
     from qpp_amd import workloads
     sizes = [(16, 4096), (12, 512), (20, 512), (22, 512)]
     scale = args.traj_scale
@@ -245,10 +247,14 @@ def bench_batch(args, qa):
         "precision_noiseless_c64_vs_c128": prec,
         "outcomes_equal_c128_batch": agree,
         "oracle_outcomes_matched": f"{om}/{len(i12)} (12-qubit), {om16}/{len(i16)} (16-qubit)",
-        "roofline": {"bound": "hbm", "kernel": "k_traj", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_traj_pass (20- and 22-qubit members, pass by pass, a workgroup per "
+                               "tile) + k_traj (persistent: 12- and 16-qubit members, every measurement)",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_ms": round(float(np.mean(dev_ms)), 3)},
+                     "avg_launch_ms": round(float(np.mean(dev_ms)), 3),
+                     "timing": "HIP events around every launch of one batch run"},
         "cpu_baseline": cpu,
     }
 

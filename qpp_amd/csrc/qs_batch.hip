@@ -33,6 +33,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -91,6 +92,114 @@ __device__ __forceinline__ double pnorm(float2 v) {
     return xx + yy;
 }
 
+// One tile of one pass of a trajectory: x holds the tile in the load layout
+// of the pass's first phase (phases pb..pe); the phases' ops run with the
+// trajectory's matrix overrides ovr[ov..ovr_end) (sorted by op, so each one
+// is met in order) and the tile is stored back into the state a.  Returns
+// the override index after the pass's ops.
+template <typename R, int NR>
+__device__ __forceinline__ uint32_t traj_tile(cxt<R>* x, cxt<R>* z, cxt<R>* __restrict__ a, cxt<R>* lds,
+                                              const FPhase* __restrict__ phases, int pb, int pe,
+                                              const FOp* __restrict__ ops, const cxt<R>* __restrict__ mats,
+                                              const Ovr* __restrict__ ovr, uint32_t ov, uint32_t ovr_end,
+                                              uint64_t base, uint64_t gt_ld, uint32_t t) {
+    constexpr int RB = RBof<NR>::v;
+    const FPhase* ld = phases + pb;
+    uint32_t oc = ov;
+    uint32_t next_op = oc < ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
+    int ci = pb;
+    for (int pi = pb; pi < pe; ++pi) {
+        const FPhase* ph = phases + pi;
+        if (pi != ci) {
+            const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
+            const uint32_t sw = thr_s(cur, t);
+#pragma unroll
+            for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
+            __syncthreads();
+            const uint32_t sr = thr_s(ph, t);
+#pragma unroll
+            for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ph, rho)];
+            __syncthreads();
+            ci = pi;
+        }
+        // the matrix of op o: its own, or this trajectory's override
+        auto mu_of = [&](int o) -> const double* {
+            if (uint32_t(o) != next_op) return ops[o].m;
+            const double* mu = ovr[oc].m;
+            ++oc;
+            next_op = oc < ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
+            return mu;
+        };
+        // consecutive ops alternate between x and z (see run_op)
+        const int ob = ph->op_begin, oe = ph->op_end;
+        int o = ob;
+        for (; o + 1 < oe; o += 2) {
+            run_op<R, NR>(x, z, ops + o, mu_of(o), mats, t, base);
+            run_op<R, NR>(z, x, ops + o + 1, mu_of(o + 1), mats, t, base);
+        }
+        if (o < oe) {
+            run_op<R, NR>(x, z, ops + o, mu_of(o), mats, t, base);
+            copy_regs<R, NR>(z, x);
+        }
+    }
+    const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
+    if (!cur->coalesced) {
+        const uint32_t sw = thr_s(cur, t);
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
+        __syncthreads();
+        const uint32_t sr = thr_s(ld, t);
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ld, rho)];
+        __syncthreads();
+        cur = ld;
+    }
+    cxt<R>* q = a + (base | (cur == ld ? gt_ld : thr_g(cur, t)));
+#pragma unroll
+    for (int rho = 0; rho < NR; ++rho) q[greg<RB>(cur, rho)] = x[rho];
+    return oc;
+}
+
+// One pass of the wide trajectories of one circuit (states too large for a
+// workgroup each): workgroup (x, y) runs tile x of the state st[w0 + y]
+// through pass p; wov[w * stride + pl] is trajectory w's first override
+// of the pass (its pl-th), wend[w] the end of its overrides.  The first
+// pass of a circuit (zero_in) synthesises |0..0> instead of loading.
+template <typename R, int NR>
+__global__ __launch_bounds__(kThreads) void k_traj_pass(cxt<R>* const* __restrict__ st,
+                                                        const uint32_t* __restrict__ wov,
+                                                        const uint32_t* __restrict__ wend, uint32_t w0,
+                                                        uint32_t stride, uint32_t pl,
+                                                        const FPass* __restrict__ passes, int p,
+                                                        const FPhase* __restrict__ phases,
+                                                        const FOp* __restrict__ ops,
+                                                        const cxt<R>* __restrict__ mats,
+                                                        const Ovr* __restrict__ ovr, int zero_in) {
+    extern __shared__ unsigned char smem_raw[];
+    cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
+    constexpr int RB = RBof<NR>::v;
+    const uint32_t t = threadIdx.x;
+    const uint32_t w = w0 + blockIdx.y;
+    cxt<R>* __restrict__ a = st[w];
+    const FPass* ps = passes + p;
+    const int pb = ps->phase_begin, pe = ps->phase_end;
+    const FPhase* ld = phases + pb;
+    const uint64_t gt_ld = thr_g(ld, t);
+    const uint64_t base = tile_base_of(ps, uint64_t(blockIdx.x));
+    const uint64_t off = base | gt_ld;
+    cxt<R> x[NR], z[NR];
+    if (zero_in) {
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) x[rho] = mk<R>((off | greg<RB>(ld, rho)) == 0 ? R(1) : R(0), R(0));
+    } else {
+        const cxt<R>* q = a + off;
+#pragma unroll
+        for (int rho = 0; rho < NR; ++rho) x[rho] = q[greg<RB>(ld, rho)];
+    }
+    (void)traj_tile<R, NR>(x, z, a, lds, phases, pb, pe, ops, mats, ovr, wov[uint64_t(w) * stride + pl],
+                           wend[w], base, gt_ld, t);
+}
+
 template <typename R, int NR>
 __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint64_t slot,
                                                    const BClass* __restrict__ cls,
@@ -103,7 +212,8 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
                                                    const uint32_t* __restrict__ order,
                                                    uint32_t ntraj, TrajOut* __restrict__ out,
                                                    uint32_t* __restrict__ next, int force_exact,
-                                                   cxt<R>* const* __restrict__ dst) {
+                                                   cxt<R>* const* __restrict__ dst,
+                                                   cxt<R>* const* __restrict__ src) {
     extern __shared__ unsigned char smem_raw[];
     cxt<R>* lds = reinterpret_cast<cxt<R>*>(smem_raw);
     __shared__ double pre[kThreads + 1];
@@ -121,8 +231,11 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         if (k >= ntraj) break;
         const uint32_t tr = order[k];
         // the trajectory's state: the workgroup's slot, or (dst set) the
-        // caller's state of that trajectory, which keeps the final state
-        cxt<R>* __restrict__ a = dst ? dst[tr] : ws + uint64_t(blockIdx.x) * slot;
+        // caller's state of that trajectory, which keeps the final state, or
+        // (src[tr] set) a state whose passes k_traj_pass has already run,
+        // which is only measured here
+        cxt<R>* const sp = src ? src[tr] : nullptr;
+        cxt<R>* __restrict__ a = sp ? sp : dst ? dst[tr] : ws + uint64_t(blockIdx.x) * slot;
         const TrajIn ti = tin[tr];
         const BClass c = cls[ti.cls];
         uint32_t ov = ti.ovr_begin;
@@ -130,7 +243,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
         // reference would put the record in scratch memory
         const uint32_t ovr_end = ti.ovr_end;
         const double u_traj = ti.u;
-        for (int p = c.pass_begin; p < c.pass_end; ++p) {
+        for (int p = sp ? c.pass_end : c.pass_begin; p < c.pass_end; ++p) {
             const FPass* ps = passes + p;
             const bool zero_in = p == c.pass_begin;  // first pass: |0..0>
             constexpr int RB = RBof<NR>::v;
@@ -159,58 +272,7 @@ __global__ __launch_bounds__(kThreads) void k_traj(cxt<R>* __restrict__ ws, uint
 #pragma unroll
                 for (int rho = 0; rho < NR; ++rho) x[rho] = y[rho];
                 if (tile + 1 < ntiles) fetch(tile + 1, y);
-                oc = ov;
-                uint32_t next_op = oc < ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
-                int ci = pb;
-                for (int pi = pb; pi < pe; ++pi) {
-                    const FPhase* ph = phases + pi;
-                    if (pi != ci) {
-                        const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
-                        const uint32_t sw = thr_s(cur, t);
-#pragma unroll
-                        for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
-                        __syncthreads();
-                        const uint32_t sr = thr_s(ph, t);
-#pragma unroll
-                        for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ph, rho)];
-                        __syncthreads();
-                        ci = pi;
-                    }
-                    // the matrix of op o: its own, or this trajectory's override
-                    auto mu_of = [&](int o) -> const double* {
-                        if (uint32_t(o) != next_op) return ops[o].m;
-                        const double* mu = ovr[oc].m;
-                        ++oc;
-                        next_op = oc < ovr_end ? ovr[oc].op : 0xFFFFFFFFu;
-                        return mu;
-                    };
-                    // consecutive ops alternate between x and z (see run_op)
-                    const int ob = ph->op_begin, oe = ph->op_end;
-                    int o = ob;
-                    for (; o + 1 < oe; o += 2) {
-                        run_op<R, NR>(x, z, ops + o, mu_of(o), mats, t, base);
-                        run_op<R, NR>(z, x, ops + o + 1, mu_of(o + 1), mats, t, base);
-                    }
-                    if (o < oe) {
-                        run_op<R, NR>(x, z, ops + o, mu_of(o), mats, t, base);
-                        copy_regs<R, NR>(z, x);
-                    }
-                }
-                const FPhase* cur = phases + __builtin_amdgcn_readfirstlane(ci);
-                if (!cur->coalesced) {
-                    const uint32_t sw = thr_s(cur, t);
-#pragma unroll
-                    for (int rho = 0; rho < NR; ++rho) lds[sw ^ sreg<RB>(cur, rho)] = x[rho];
-                    __syncthreads();
-                    const uint32_t sr = thr_s(ld, t);
-#pragma unroll
-                    for (int rho = 0; rho < NR; ++rho) x[rho] = lds[sr ^ sreg<RB>(ld, rho)];
-                    __syncthreads();
-                    cur = ld;
-                }
-                cxt<R>* q = a + (base | (cur == ld ? gt_ld : thr_g(cur, t)));
-#pragma unroll
-                for (int rho = 0; rho < NR; ++rho) q[greg<RB>(cur, rho)] = x[rho];
+                oc = traj_tile<R, NR>(x, z, a, lds, phases, pb, pe, ops, mats, ovr, ov, ovr_end, base, gt_ld, t);
             }
             ov = oc;
             __syncthreads();  // the pass's stores are visible to the next pass
@@ -561,8 +623,22 @@ struct qs_batch {
     size_t c_ws = 0;
     void* d_dst = nullptr;  // per-trajectory destination states (run_states)
     size_t c_dst = 0;
-    // class -> offsets of its ops in the concatenated table
-    std::vector<int> op_base;
+    // wide trajectories (run pass by pass over all their tiles): their
+    // states, per-trajectory state pointers (null for the others), and
+    // per wide trajectory its states, first override of every pass and
+    // end of overrides
+    void* d_wide = nullptr;
+    size_t c_wide = 0;
+    void* d_src = nullptr;
+    size_t c_src = 0;
+    void* d_wst = nullptr;
+    size_t c_wst = 0;
+    void* d_wov = nullptr;
+    size_t c_wov = 0;
+    void* d_wend = nullptr;
+    size_t c_wend = 0;
+    // class -> offsets of its passes and ops in the concatenated tables
+    std::vector<int> pass_base, op_base;
     // last run
     uint64_t last_exact = 0;
     double last_host_ms = 0, last_device_ms = 0;
@@ -578,10 +654,12 @@ int upload_tables(qs_batch* b, hipStream_t s) {
     std::vector<FOp> op;
     std::vector<C> mats;
     b->op_base.clear();
+    b->pass_base.clear();
     for (const Cls& c : b->cls) {
         const int pbase = int(ps.size()), hbase = int(ph.size()), obase = int(op.size());
         const uint32_t mbase = uint32_t(mats.size());
         b->op_base.push_back(obase);
+        b->pass_base.push_back(pbase);
         bc.push_back(BClass{pbase, pbase + int(c.P.passes.size()), c.nb, 0});
         for (FPass p : c.P.passes) {
             p.phase_begin += hbase;
@@ -749,8 +827,8 @@ int qs_batch_create(const qs_circuit* const* circuits, uint32_t ncircuits, qs_dt
 
 int qs_batch_destroy(qs_batch* b) {
     if (!b) return QS_OK;
-    for (void* p : {b->d_tab, b->d_tin, b->d_ovr, b->d_order, b->d_out, b->d_ws, b->d_dst,
-                    static_cast<void*>(b->d_next)})
+    for (void* p : {b->d_tab, b->d_tin, b->d_ovr, b->d_order, b->d_out, b->d_ws, b->d_dst, b->d_wide,
+                    b->d_src, b->d_wst, b->d_wov, b->d_wend, static_cast<void*>(b->d_next)})
         if (p) (void)hipFreeAsync(p, b->stream);
     delete b;
     return QS_OK;
@@ -791,9 +869,21 @@ int qs_batch_plan(const qs_batch* b, uint32_t circuit, uint64_t seed, double noi
 
 namespace {
 
+// Trajectories of at least this many qubits run pass by pass over all their
+// tiles (k_traj_pass, one workgroup per tile of each such state) before
+// k_traj measures them: one workgroup streaming a state of 2^20 amplitudes
+// and more tile after tile is bound by its own latency.  The environment
+// variable QPP_AMD_TRAJ_WIDE_BITS sets another threshold (64: none).
+uint32_t wide_bits() {
+    const char* v = std::getenv("QPP_AMD_TRAJ_WIDE_BITS");
+    if (!v || !*v) return 18;
+    return static_cast<uint32_t>(std::strtoul(v, nullptr, 10));
+}
+
 // One launch of k_traj over ntraj trajectories (arguments validated by the
-// caller).  dst: nullptr (each trajectory ends in its measurement), or one
-// state per trajectory that receives its final state instead.
+// caller), after the passes of the wide ones.  dst: nullptr (each
+// trajectory ends in its measurement), or one state per trajectory that
+// receives its final state instead (every trajectory then runs in k_traj).
 int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint64_t* seeds,
              double noise_p, const std::vector<void*>* dst, uint64_t* out_index, double* out_prob) {
     constexpr const char* W = "qpp_amd::batch";
@@ -835,9 +925,23 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
     // queue order: largest work first (greedy longest-processing-time)
     std::vector<uint32_t> order(ntraj);
     std::iota(order.begin(), order.end(), 0u);
+    // wide trajectories, grouped by circuit: their passes run before k_traj
+    const uint32_t wbits = wide_bits();
+    std::vector<uint32_t> wide;
+    std::vector<char> is_wide(ntraj, 0);
+    if (!dst)
+        for (uint64_t i = 0; i < ntraj; ++i) {
+            const Cls& c = b->cls[circuit_of[i]];
+            if (c.nb >= wbits && !c.P.passes.empty()) {
+                wide.push_back(uint32_t(i));
+                is_wide[i] = 1;
+            }
+        }
+    std::stable_sort(wide.begin(), wide.end(),
+                     [&](uint32_t x, uint32_t y) { return circuit_of[x] < circuit_of[y]; });
     auto cost = [&](uint32_t i) {
         const Cls& c = b->cls[circuit_of[i]];
-        return double(c.P.passes.size() + 1) * std::ldexp(1.0, int(c.nb));
+        return double(is_wide[i] ? 1 : c.P.passes.size() + 1) * std::ldexp(1.0, int(c.nb));
     };
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
     const auto h1 = std::chrono::steady_clock::now();
@@ -845,7 +949,8 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
 
     // ---- device ----------------------------------------------------------
     uint32_t maxnb = 0;
-    for (uint64_t i = 0; i < ntraj; ++i) maxnb = std::max(maxnb, b->cls[circuit_of[i]].nb);
+    for (uint64_t i = 0; i < ntraj; ++i)
+        if (!is_wide[i]) maxnb = std::max(maxnb, b->cls[circuit_of[i]].nb);
     const size_t es = b->dtype == QS_C128 ? 16 : 8;
     const size_t shmem = es << (8 + b->reg_bits);
     int per_cu = 0;
@@ -874,6 +979,51 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
     }
     if (dst)
         if (int rc = dev_put(b->d_dst, b->c_dst, *dst, s)) return rc;
+    // wide trajectories: a state each, and every pass's first override
+    uint32_t stride = 1;
+    for (uint32_t i : wide) stride = std::max<uint32_t>(stride, uint32_t(b->cls[circuit_of[i]].P.passes.size()));
+    std::vector<void*> src;
+    std::vector<void*> wst(wide.size());
+    std::vector<uint32_t> wov(wide.size() * size_t(stride), 0), wend(wide.size());
+    if (!wide.empty()) {
+        size_t wbytes = 0;
+        for (uint32_t i : wide) wbytes += es << b->cls[circuit_of[i]].nb;
+        if (wbytes > b->c_wide) {
+            if (b->d_wide) (void)hipFreeAsync(b->d_wide, s);
+            b->d_wide = nullptr;
+            b->c_wide = 0;
+            QS_HIP(hipMallocAsync(&b->d_wide, wbytes, s), W);
+            b->c_wide = wbytes;
+        }
+        src.assign(ntraj, nullptr);
+        size_t off = 0;
+        for (size_t w = 0; w < wide.size(); ++w) {
+            const uint32_t i = wide[w];
+            const uint32_t ci = circuit_of[i];
+            const Cls& c = b->cls[ci];
+            wst[w] = static_cast<unsigned char*>(b->d_wide) + off;
+            src[i] = wst[w];
+            off += es << c.nb;
+            const TrajIn& ti = tin[i];
+            wend[w] = ti.ovr_end;
+            // the overrides are sorted by op and the passes' ops are
+            // consecutive: pass pl starts at the first override at or after
+            // its first op
+            uint32_t k = ti.ovr_begin;
+            for (size_t pl = 0; pl < c.P.passes.size(); ++pl) {
+                const FPass& ps = c.P.passes[pl];
+                if (ps.phase_begin >= ps.phase_end) return fail(QS_ERR_NOT_IMPLEMENTED, W, "empty pass");
+                if (ps.ntiles == 0 || ps.ntiles > 0xFFFFFFFFull) return fail(QS_ERR_OUT_OF_RANGE, W, "tiles per pass");
+                const uint32_t first = uint32_t(b->op_base[ci] + c.P.phases[size_t(ps.phase_begin)].op_begin);
+                while (k < ti.ovr_end && ovr[k].op < first) ++k;
+                wov[w * stride + pl] = k;
+            }
+        }
+        if (int rc = dev_put(b->d_src, b->c_src, src, s)) return rc;
+        if (int rc = dev_put(b->d_wst, b->c_wst, wst, s)) return rc;
+        if (int rc = dev_put(b->d_wov, b->c_wov, wov, s)) return rc;
+        if (int rc = dev_put(b->d_wend, b->c_wend, wend, s)) return rc;
+    }
     if (int rc = dev_put(b->d_tin, b->c_tin, tin, s)) return rc;
     if (int rc = dev_put(b->d_ovr, b->c_ovr, ovr, s)) return rc;
     if (int rc = dev_put(b->d_order, b->c_order, order, s)) return rc;
@@ -898,28 +1048,71 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
     const uint32_t nt = static_cast<uint32_t>(ntraj);
     const int force = qs::force_exact_sampling() ? 1 : 0;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, b->d_next);
+    // the wide trajectories' passes, circuit by circuit, pass by pass: grid
+    // (tiles of the pass, trajectories of the circuit, at most 65535 a launch)
+    for (size_t w0 = 0; w0 < wide.size();) {
+        const uint32_t ci = circuit_of[wide[w0]];
+        size_t w1 = w0;
+        while (w1 < wide.size() && circuit_of[wide[w1]] == ci) ++w1;
+        const Cls& c = b->cls[ci];
+        for (size_t pl = 0; pl < c.P.passes.size(); ++pl) {
+            const uint64_t ntiles = c.P.passes[pl].ntiles;
+            const int p = b->pass_base[ci] + int(pl);
+            const int zin = pl == 0 ? 1 : 0;
+            for (size_t y0 = w0; y0 < w1; y0 += 65535) {
+                const dim3 g(static_cast<uint32_t>(ntiles), static_cast<uint32_t>(std::min<size_t>(65535, w1 - y0)));
+                const uint32_t wy = static_cast<uint32_t>(y0), pls = static_cast<uint32_t>(pl);
+                auto* wovp = static_cast<const uint32_t*>(b->d_wov);
+                auto* wendp = static_cast<const uint32_t*>(b->d_wend);
+                if (b->dtype == QS_C128) {
+                    auto* stp = static_cast<double2* const*>(b->d_wst);
+                    auto* mp = static_cast<const double2*>(b->d_mats);
+                    if (b->reg_bits == 3)
+                        hipLaunchKernelGGL((k_traj_pass<double, 8>), g, dim3(kThreads), shmem, s, stp, wovp, wendp,
+                                           wy, stride, pls, b->d_passes, p, b->d_phases, b->d_ops, mp, ovrp, zin);
+                    else
+                        hipLaunchKernelGGL((k_traj_pass<double, 16>), g, dim3(kThreads), shmem, s, stp, wovp, wendp,
+                                           wy, stride, pls, b->d_passes, p, b->d_phases, b->d_ops, mp, ovrp, zin);
+                } else {
+                    auto* stp = static_cast<float2* const*>(b->d_wst);
+                    auto* mp = static_cast<const float2*>(b->d_mats);
+                    if (b->reg_bits == 3)
+                        hipLaunchKernelGGL((k_traj_pass<float, 8>), g, dim3(kThreads), shmem, s, stp, wovp, wendp,
+                                           wy, stride, pls, b->d_passes, p, b->d_phases, b->d_ops, mp, ovrp, zin);
+                    else
+                        hipLaunchKernelGGL((k_traj_pass<float, 16>), g, dim3(kThreads), shmem, s, stp, wovp, wendp,
+                                           wy, stride, pls, b->d_passes, p, b->d_phases, b->d_ops, mp, ovrp, zin);
+                }
+            }
+        }
+        w0 = w1;
+    }
     if (b->dtype == QS_C128) {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<double, 8>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
                                static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
-                               static_cast<double2* const*>(dst ? b->d_dst : nullptr));
+                               static_cast<double2* const*>(dst ? b->d_dst : nullptr),
+                               static_cast<double2* const*>(wide.empty() ? nullptr : b->d_src));
         else
             hipLaunchKernelGGL((k_traj<double, 16>), gd, bd, shmem, s, static_cast<double2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
                                static_cast<const double2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
-                               static_cast<double2* const*>(dst ? b->d_dst : nullptr));
+                               static_cast<double2* const*>(dst ? b->d_dst : nullptr),
+                               static_cast<double2* const*>(wide.empty() ? nullptr : b->d_src));
     } else {
         if (b->reg_bits == 3)
             hipLaunchKernelGGL((k_traj<float, 8>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
                                static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
-                               static_cast<float2* const*>(dst ? b->d_dst : nullptr));
+                               static_cast<float2* const*>(dst ? b->d_dst : nullptr),
+                               static_cast<float2* const*>(wide.empty() ? nullptr : b->d_src));
         else
             hipLaunchKernelGGL((k_traj<float, 16>), gd, bd, shmem, s, static_cast<float2*>(b->d_ws), slot,
                                b->d_cls, b->d_passes, b->d_phases, b->d_ops,
                                static_cast<const float2*>(b->d_mats), tinp, ovrp, ordp, nt, outp, b->d_next, force,
-                               static_cast<float2* const*>(dst ? b->d_dst : nullptr));
+                               static_cast<float2* const*>(dst ? b->d_dst : nullptr),
+                               static_cast<float2* const*>(wide.empty() ? nullptr : b->d_src));
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, s);

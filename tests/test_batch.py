@@ -4,7 +4,9 @@ CPU: the host plan (Pauli frame absorbed into 1-qubit gates, conjugated
 through Clifford gates) reproduces an explicit simulation of the same
 trajectory with the errors applied where QNoisyEngine applies them, drawn
 from the same std::mt19937 stream (oracle Rng).  GPU: every trajectory's
-outcome equals the oracle QNoisyEngine's for the same seed."""
+outcome equals the oracle QNoisyEngine's for the same seed, in the
+persistent launch and on the wide path (QPP_AMD_TRAJ_WIDE_BITS sets the
+qubit count from which trajectories run pass by pass; default 18)."""
 import numpy as np
 import pytest
 
@@ -149,6 +151,47 @@ def test_batch_vs_oracle_engine_mixed_sizes(orc, p):
         c = int(cof[i])
         got = b.dits(c, idx[i])
         assert got == oracle_outcome(orc, circs[c], int(seeds[i]), p), (i, c, int(seeds[i]))
+        assert 0 < prob[i] <= 1
+
+
+@gpu
+@pytest.mark.parametrize("dt", [qa.C128, qa.C64])
+@pytest.mark.parametrize("reg_bits", [3, 4])
+def test_wide_trajectories_equal_single_launch(monkeypatch, dt, reg_bits):
+    """Trajectories at or above the wide threshold run pass by pass over
+    all their tiles (many workgroups per state) and are then measured in
+    the persistent launch: outcomes and probabilities are bit-identical to
+    running every trajectory inside the persistent launch, with wide and
+    narrow circuits mixed in one batch."""
+    circs = [noisy_circuit(12, 3, seed=71), noisy_circuit(13, 3, seed=72, extra=True),
+             noisy_circuit(14, 2, seed=73, cz=True)]
+    rng = np.random.default_rng(7)
+    ntr = 60
+    cof = rng.integers(0, 3, size=ntr).astype(np.uint32)
+    seeds = rng.integers(0, 2 ** 32, size=ntr, dtype=np.uint64)
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "64")
+    i0, p0 = qa.Batch(circs, dtype=dt, reg_bits=reg_bits).run(cof, seeds, 0.01)
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "13")
+    b = qa.Batch(circs, dtype=dt, reg_bits=reg_bits)
+    i1, p1 = b.run(cof, seeds, 0.01)
+    assert np.array_equal(i0, i1) and np.array_equal(p0, p1)
+    # the same batch object again (buffers reused), all of it wide
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "12")
+    i2, p2 = b.run(cof, seeds, 0.01)
+    assert np.array_equal(i0, i2) and np.array_equal(p0, p2)
+
+
+@gpu
+def test_wide_trajectories_vs_oracle_engine(orc, monkeypatch):
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "12")
+    circs = [noisy_circuit(12, 3, seed=81, cz=True), noisy_circuit(13, 2, seed=82, extra=True)]
+    b = qa.Batch(circs, dtype=qa.C128)
+    cof = np.array([i % 2 for i in range(24)], dtype=np.uint32)
+    seeds = np.arange(900, 924, dtype=np.uint64)
+    idx, prob = b.run(cof, seeds, 0.01)
+    for i in range(24):
+        c = int(cof[i])
+        assert b.dits(c, idx[i]) == oracle_outcome(orc, circs[c], int(seeds[i]), 0.01), i
         assert 0 < prob[i] <= 1
 
 
