@@ -883,7 +883,8 @@ uint32_t wide_bits() {
 // One launch of k_traj over ntraj trajectories (arguments validated by the
 // caller), after the passes of the wide ones.  dst: nullptr (each
 // trajectory ends in its measurement), or one state per trajectory that
-// receives its final state instead (every trajectory then runs in k_traj).
+// receives its final state instead (the wide ones' passes then run on it
+// and k_traj leaves them alone).
 int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint64_t* seeds,
              double noise_p, const std::vector<void*>* dst, uint64_t* out_index, double* out_prob) {
     constexpr const char* W = "qpp_amd::batch";
@@ -929,14 +930,13 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
     const uint32_t wbits = wide_bits();
     std::vector<uint32_t> wide;
     std::vector<char> is_wide(ntraj, 0);
-    if (!dst)
-        for (uint64_t i = 0; i < ntraj; ++i) {
-            const Cls& c = b->cls[circuit_of[i]];
-            if (c.nb >= wbits && !c.P.passes.empty()) {
-                wide.push_back(uint32_t(i));
-                is_wide[i] = 1;
-            }
+    for (uint64_t i = 0; i < ntraj; ++i) {
+        const Cls& c = b->cls[circuit_of[i]];
+        if (c.nb >= wbits && !c.P.passes.empty()) {
+            wide.push_back(uint32_t(i));
+            is_wide[i] = 1;
         }
+    }
     std::stable_sort(wide.begin(), wide.end(),
                      [&](uint32_t x, uint32_t y) { return circuit_of[x] < circuit_of[y]; });
     auto cost = [&](uint32_t i) {
@@ -987,7 +987,8 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
     std::vector<uint32_t> wov(wide.size() * size_t(stride), 0), wend(wide.size());
     if (!wide.empty()) {
         size_t wbytes = 0;
-        for (uint32_t i : wide) wbytes += es << b->cls[circuit_of[i]].nb;
+        if (!dst)
+            for (uint32_t i : wide) wbytes += es << b->cls[circuit_of[i]].nb;
         if (wbytes > b->c_wide) {
             if (b->d_wide) (void)hipFreeAsync(b->d_wide, s);
             b->d_wide = nullptr;
@@ -1001,9 +1002,13 @@ int run_impl(qs_batch* b, uint64_t ntraj, const uint32_t* circuit_of, const uint
             const uint32_t i = wide[w];
             const uint32_t ci = circuit_of[i];
             const Cls& c = b->cls[ci];
-            wst[w] = static_cast<unsigned char*>(b->d_wide) + off;
+            if (dst) {
+                wst[w] = (*dst)[i];
+            } else {
+                wst[w] = static_cast<unsigned char*>(b->d_wide) + off;
+                off += es << c.nb;
+            }
             src[i] = wst[w];
-            off += es << c.nb;
             const TrajIn& ti = tin[i];
             wend[w] = ti.ovr_end;
             // the overrides are sorted by op and the passes' ops are

@@ -271,6 +271,23 @@ def test_run_states_vs_oracle(orc, dt):
 
 
 @gpu
+@pytest.mark.parametrize("dt", [qa.C128, qa.C64])
+def test_run_states_wide_equal_single_launch(monkeypatch, dt):
+    """run_states with the wide path (passes of the 13- and 14-qubit entries
+    run on the callers' states ahead of the persistent launch) leaves
+    bit-identical states to running every entry in the persistent launch."""
+    circs = [noisy_circuit(12, 3, 91, cz=True), noisy_circuit(13, 2, 92, extra=True),
+             noisy_circuit(14, 2, 93)]
+    cof = [0, 1, 2, 1, 2, 0]
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "64")
+    ref = [st.download() for st in qa.Batch(circs, dtype=dt).run_states(cof)]
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "13")
+    got = [st.download() for st in qa.Batch(circs, dtype=dt).run_states(cof)]
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+@gpu
 def test_run_states_errors():
     circs = [noisy_circuit(12, 1, 81)]
     b = qa.Batch(circs, dtype=qa.C128)
