@@ -159,7 +159,7 @@ def bench_batch(args, qa):
     seeds = np.arange(len(cof), dtype=np.uint64) + 1000
     ntraj = len(cof)
     es = 8 if dt == qa.C64 else 16
-    b = qa.Batch(circs, dtype=dt)
+    b = qa.Batch(circs, dtype=dt, low_bits=args.traj_low_bits, max_ops=args.traj_max_ops)
     passes = [b.info(i)["passes"] for i in range(len(circs))]
     algo_bytes = float(sum(passes[c] * 2 * es * 2 ** sizes[c][0] for c in cof))
     for w in range(args.warmup):
@@ -199,7 +199,8 @@ def bench_batch(args, qa):
                          "max_abs": r["max_abs"]}
     # outcome agreement with the c128 batch on the same seeds
     sub = min(ntraj, 1024)
-    ref_idx, _ = qa.Batch(circs, dtype=qa.C128).run(cof[:sub], seeds[:sub], p)
+    ref_idx, _ = qa.Batch(circs, dtype=qa.C128, low_bits=args.traj_low_bits,
+                          max_ops=args.traj_max_ops).run(cof[:sub], seeds[:sub], p)
     agree = float(np.mean(ref_idx == idx[:sub]))
     # oracle QNoisyEngine on a few 12-qubit trajectories (same seeds)
     import oracle
@@ -237,6 +238,7 @@ def bench_batch(args, qa):
                                f"({', '.join(f'{max(1, int(c * scale))}x{n}q' for n, c in sizes)}), "
                                f"depth {depth}, depolarizing p={p}, measure all",
                    "passes_per_circuit": dict(zip([f"{n}q" for n, _ in sizes], passes)),
+                   "fuse_opts": {"low_bits": args.traj_low_bits, "max_ops": args.traj_max_ops},
                    "circuit_sha256": {f"{n}q": workloads.gates_digest(
                        workloads.ladder_layers(n, depth, 300 + n)) for n, _ in sizes}},
         "plain_gpu_trajectories_per_s": round(ntraj / plain_sec, 3),
@@ -1137,6 +1139,8 @@ def main():
     ap.add_argument("--cal-requests", type=int, default=40,
                     help="serve mode: closed-loop calibration requests")
     ap.add_argument("--lanes", type=int, default=4, help="serve mode: concurrent lanes (the device has 4 hardware queues)")
+    ap.add_argument("--traj-low-bits", type=int, default=5, help="batch mode: fused low_bits")
+    ap.add_argument("--traj-max-ops", type=int, default=48, help="batch mode: fused max_ops")
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c64",
                     help="batch mode: complex dtype of the trajectories")
     ap.add_argument("--traj-scale", type=float, default=1.0,
