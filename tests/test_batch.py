@@ -182,6 +182,22 @@ def test_wide_trajectories_equal_single_launch(monkeypatch, dt, reg_bits):
 
 
 @gpu
+def test_wide_trajectories_more_than_one_launch_row(monkeypatch):
+    """More wide trajectories of one circuit than one launch's second grid
+    dimension holds (65535): the passes go out in row chunks; every
+    trajectory still equals its single-launch result."""
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "64")
+    circs = [noisy_circuit(12, 2, seed=95)]
+    ntr = 70000
+    cof = np.zeros(ntr, dtype=np.uint32)
+    seeds = np.arange(ntr, dtype=np.uint64) + 12345
+    i0, p0 = qa.Batch(circs, dtype=qa.C64).run(cof, seeds, 0.01)
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "12")
+    i1, p1 = qa.Batch(circs, dtype=qa.C64).run(cof, seeds, 0.01)
+    assert np.array_equal(i0, i1) and np.array_equal(p0, p1)
+
+
+@gpu
 def test_wide_trajectories_vs_oracle_engine(orc, monkeypatch):
     monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "12")
     circs = [noisy_circuit(12, 3, seed=81, cz=True), noisy_circuit(13, 2, seed=82, extra=True)]
@@ -226,10 +242,13 @@ def test_batch_rerun_is_deterministic_and_order_free():
 
 
 @gpu
-def test_batch_exact_replay_path_vs_oracle(orc, monkeypatch):
+@pytest.mark.parametrize("wide", [False, True])
+def test_batch_exact_replay_path_vs_oracle(orc, monkeypatch, wide):
     """Every trajectory's outcome decided by the in-kernel exact sequential
-    replay (forced) still equals the oracle's."""
+    replay (forced) still equals the oracle's, also on states the wide
+    path's pass launches left."""
     monkeypatch.setenv("QPP_AMD_FORCE_EXACT", "1")
+    monkeypatch.setenv("QPP_AMD_TRAJ_WIDE_BITS", "12" if wide else "64")
     circs = [noisy_circuit(12, 2, seed=61), noisy_circuit(13, 2, seed=62)]
     cof = np.array([i % 2 for i in range(16)], dtype=np.uint32)
     seeds = np.arange(500, 516, dtype=np.uint64)
