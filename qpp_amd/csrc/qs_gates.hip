@@ -70,11 +70,17 @@ struct M2 {
 };
 
 // Insert zero bits at the sorted positions pos[0..np-1] (ascending) into x.
+// The positions (bytes of a kernel argument) are read four to a 32-bit
+// word: a uniform word load is a scalar load, where a byte load would be a
+// vector load whose wait also waits for every other load and store the
+// wave has in flight (the prefetched amplitudes among them).
 __device__ __forceinline__ uint64_t insert_zeros(uint64_t x, const uint8_t* pos,
                                                  int np) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(pos);
     for (int i = 0; i < np; ++i) {
-        const uint64_t lo = x & ((uint64_t{1} << pos[i]) - 1);
-        x = ((x >> pos[i]) << (pos[i] + 1)) | lo;
+        const uint32_t p = (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        const uint64_t lo = x & ((uint64_t{1} << p) - 1);
+        x = ((x >> p) << (p + 1)) | lo;
     }
     return x;
 }
@@ -277,31 +283,29 @@ __global__ __launch_bounds__(kBlock) void k_gatek_mfma(float2* __restrict__ a, u
         for (int v = 0; v < 4; ++v) out_off[mh][v] = inner[16 * mh + 4 * g + v];
     constexpr int kWaves = kBlock / 64;
     const uint64_t nwaves = uint64_t(gridDim.x) * kWaves;
-    // a wave's group `grp` covers columns 16 grp .. 16 grp + 15; the next
-    // group's amplitudes are loaded before the current group's products, so
-    // one group's loads are in flight while the matrix cores run
-    auto fetch = [&](uint64_t grp, float (&xr)[NX], float (&xi)[NX], uint64_t& base, bool& ok) {
+    // a wave's group `grp` covers columns 16 grp .. 16 grp + 15.  The
+    // groups after the current one are loaded before its products, into
+    // register buffers taken in turn (the loop unrolled, so no buffer is
+    // copied and no load has to land before its own group's turn)
+    struct Buf {
+        float xr[NX], xi[NX];
+        uint64_t base;
+        bool ok;
+    };
+    // a lane past the last column loads column 0's amplitudes (in bounds,
+    // no branch around the loads) and stores nothing
+    auto fetch = [&](uint64_t grp, Buf& b) {
         const uint64_t col = grp * 16 + j;
-        ok = col < ncols;
-        base = ok ? (insert_zeros(col, fx.pos, fx.np) | cval) : 0;
+        b.ok = col < ncols;
+        b.base = insert_zeros(b.ok ? col : 0, fx.pos, fx.np) | cval;
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
-            float2 v = make_float2(0.f, 0.f);
-            if (ok) v = a[base + in_off[q]];
-            xr[q] = v.x;
-            xi[q] = v.y;
+            const float2 v = a[b.base + in_off[q]];
+            b.xr[q] = v.x;
+            b.xi[q] = v.y;
         }
     };
-    uint64_t grp = uint64_t(blockIdx.x) * kWaves + threadIdx.x / 64;
-    if (grp * 16 >= ncols) return;
-    float xr[NX], xi[NX], yr[NX], yi[NX];
-    uint64_t base, nbase = 0;
-    bool ok, nok = false;
-    fetch(grp, xr, xi, base, ok);
-    for (;;) {
-        const uint64_t next = grp + nwaves;
-        const bool more = next * 16 < ncols;  // uniform over the wave
-        if (more) fetch(next, yr, yi, nbase, nok);
+    auto process = [&](const Buf& b) {
         floatx4 acc[NB];
 #pragma unroll
         for (int mb = 0; mb < NB; ++mb) acc[mb] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -322,29 +326,96 @@ __global__ __launch_bounds__(kBlock) void k_gatek_mfma(float2* __restrict__ a, u
 #pragma unroll
                 for (int mb = 0; mb < NB; ++mb) wn[mb] = wv[(mb * NS + s + 1) * 64 + lane];
             }
-            const float b = s < NX ? xr[s] : xi[s - NX];
+            const float bx = s < NX ? b.xr[s] : b.xi[s - NX];
 #pragma unroll
             for (int mb = 0; mb < NB; ++mb)
-                acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[mb], b, acc[mb], 0, 0, 0);
+                acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc[mb], bx, acc[mb], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int mb = 0; mb < NB; ++mb) wc[mb] = wn[mb];
         }
-        if (ok) {
+        if (b.ok) {
 #pragma unroll
             for (int mh = 0; mh < NO; ++mh)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    a[base + out_off[mh][v]] = make_float2(acc[mh][v], acc[mh + NO][v]);
+                    a[b.base + out_off[mh][v]] = make_float2(acc[mh][v], acc[mh + NO][v]);
         }
-        if (!more) break;
-        grp = next;
-        base = nbase;
-        ok = nok;
+    };
+    uint64_t grp = uint64_t(blockIdx.x) * kWaves + threadIdx.x / 64;
+    if (grp * 16 >= ncols) return;
+    // every fetch is issued, past the wave's last group on its current
+    // group again (in bounds, never used): with no load under a branch the
+    // waits before each group's products count exactly the loads and
+    // stores issued after its own loads
+    if constexpr (K == 4) {
+        // 4 amplitudes per lane and group, 8 waves per SIMD: the next group
+        // in flight, loaded under the lanes' bounds and copied into place
+        float yr[NX], yi[NX];
+        Buf cur;
+        cur.ok = false;
+        bool ok = (grp * 16 + j) < ncols;
+        uint64_t base = ok ? (insert_zeros(grp * 16 + j, fx.pos, fx.np) | cval) : 0;
 #pragma unroll
         for (int q = 0; q < NX; ++q) {
-            xr[q] = yr[q];
-            xi[q] = yi[q];
+            float2 v = make_float2(0.f, 0.f);
+            if (ok) v = a[base + in_off[q]];
+            cur.xr[q] = v.x;
+            cur.xi[q] = v.y;
+        }
+        for (;;) {
+            const uint64_t next = grp + nwaves;
+            const bool more = next * 16 < ncols;  // uniform over the wave
+            uint64_t nbase = 0;
+            bool nok = false;
+            if (more) {
+                const uint64_t col = next * 16 + j;
+                nok = col < ncols;
+                nbase = nok ? (insert_zeros(col, fx.pos, fx.np) | cval) : 0;
+#pragma unroll
+                for (int q = 0; q < NX; ++q) {
+                    float2 v = make_float2(0.f, 0.f);
+                    if (nok) v = a[nbase + in_off[q]];
+                    yr[q] = v.x;
+                    yi[q] = v.y;
+                }
+            }
+            cur.base = base;
+            cur.ok = ok;
+            process(cur);
+            if (!more) break;
+            grp = next;
+            base = nbase;
+            ok = nok;
+#pragma unroll
+            for (int q = 0; q < NX; ++q) {
+                cur.xr[q] = yr[q];
+                cur.xi[q] = yi[q];
+            }
+        }
+    } else {
+        // 8 amplitudes per lane and group, 4 waves per SIMD: the next two
+        // groups in flight (three buffers in turn)
+        Buf b0, b1, b2;
+        fetch(grp, b0);
+        bool has1 = (grp + nwaves) * 16 < ncols;
+        fetch(has1 ? grp + nwaves : grp, b1);
+        // `cur` holds group grp, the next buffer group grp + nwaves (when
+        // has1), and `far` receives group grp + 2 nwaves; false once grp was
+        // the last
+        auto step = [&](const Buf& cur, Buf& far) {
+            const bool has2 = (grp + 2 * nwaves) * 16 < ncols;
+            fetch(has2 ? grp + 2 * nwaves : grp, far);
+            process(cur);
+            if (!has1) return false;
+            grp += nwaves;
+            has1 = has2;
+            return true;
+        };
+        for (;;) {
+            if (!step(b0, b2)) break;
+            if (!step(b1, b0)) break;
+            if (!step(b2, b1)) break;
         }
     }
 }
@@ -558,12 +629,19 @@ int gatek_mfma(qs_state* st, uint32_t k, const std::vector<uint64_t>& inner, con
     std::vector<unsigned char> blob(inner_bytes + wp.size() * sizeof(float));
     std::memcpy(blob.data(), inner.data(), inner_bytes);
     std::memcpy(blob.data() + inner_bytes, wp.data(), wp.size() * sizeof(float));
+    // persistent waves (each walks many groups, two ahead in flight): no
+    // more workgroups than are resident at once, so none starts late
+    const uint64_t groups16 = (ncols + 15) / 16;
+    int per_cu = 0, dev = 0, ncu = 0;
+    QS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k == 4 ? k_gatek_mfma<4> : k_gatek_mfma<5>,
+                                                        kBlock, 0),
+           "qpp::apply()");
+    QS_HIP(hipGetDevice(&dev), "qpp::apply()");
+    QS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "qpp::apply()");
+    const uint32_t resident = static_cast<uint32_t>(std::max(1, per_cu) * std::max(1, ncu));
+    const uint32_t grid = std::min<uint32_t>(grid_for(groups16, kBlock / 64), std::min(kMfmaGrid, resident));
     void* dblob = nullptr;
     if (int rc = upload(st, blob.data(), blob.size(), &dblob)) return rc;
-    // persistent waves (each walks many groups, prefetching the next): a
-    // few workgroups per CU at most
-    const uint64_t groups16 = (ncols + 15) / 16;
-    const uint32_t grid = std::min<uint32_t>(grid_for(groups16, kBlock / 64), kMfmaGrid);
     auto* inn = static_cast<const uint64_t*>(dblob);
     auto* wpk = reinterpret_cast<const float*>(static_cast<unsigned char*>(dblob) + inner_bytes);
     if (k == 4)
